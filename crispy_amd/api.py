@@ -1,0 +1,256 @@
+"""ctypes binding of include/crispy_hip.h."""
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(PKG, "lib", "libcrispy_hip.so")
+
+KMER = 0  # CRISPY_READS_KMER
+NW = 1    # CRISPY_READS_NW
+MAX_READ_LEN = 2048
+MAX_NUM_READS = 1024 * 1024
+
+_ERRNAMES = {-1: "ARG", -2: "HIP", -3: "NOMEM", -4: "STATE", -5: "IO"}
+
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_f32p = ctypes.POINTER(ctypes.c_float)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+
+_lib = None
+
+
+class CrispyError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"crispy error {code} ({_ERRNAMES.get(code, '?')}): {msg}")
+        self.code = code
+
+
+def library_path():
+    return _LIB_PATH
+
+
+def load_library():
+    """Loads libcrispy_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        raise RuntimeError(f"{_LIB_PATH} is missing: build it with `python -m crispy_amd.build`")
+    L = ctypes.CDLL(_LIB_PATH)
+    sig = {
+        "crispy_ctx_create": ([ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
+        "crispy_ctx_destroy": ([_vp], None),
+        "crispy_ctx_error": ([_vp], ctypes.c_char_p),
+        "crispy_ctx_sync": ([_vp], ctypes.c_int),
+        "crispy_ctx_device_bytes": ([_vp], ctypes.c_int64),
+        "crispy_read_fasta": ([ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(_vp),
+                               ctypes.POINTER(_vp), _i32p], ctypes.c_int32),
+        "crispy_free": ([_vp], None),
+        "crispy_kmer_default_threshold": ([ctypes.c_int32], ctypes.c_float),
+        "crispy_nw_default_threshold": ([ctypes.c_int32], ctypes.c_float),
+        "crispy_pass": ([ctypes.c_float, ctypes.c_float], ctypes.c_int),
+        "crispy_band_cells": ([ctypes.c_int32, ctypes.c_int32, ctypes.c_int32], ctypes.c_int64),
+        "crispy_reads_upload": ([_vp, ctypes.c_int, ctypes.c_char_p, _i64p, ctypes.c_int32],
+                                ctypes.c_int),
+        "crispy_kmer_run": ([_vp, ctypes.c_int, ctypes.c_float, ctypes.c_int64, ctypes.c_int64,
+                             _i64p], ctypes.c_int),
+        "crispy_kmer_num_tiles": ([ctypes.c_int32], ctypes.c_int64),
+        "crispy_kmer_fetch": ([_vp, ctypes.c_int64, ctypes.c_int64, _i32p, _f32p], ctypes.c_int),
+        "crispy_nw_run": ([_vp, _i32p, ctypes.c_int64, ctypes.c_int, ctypes.c_float, _i64p],
+                          ctypes.c_int),
+        "crispy_nw_run_on_kmer": ([_vp, ctypes.c_int, ctypes.c_float, _i64p], ctypes.c_int),
+        "crispy_nw_fetch": ([_vp, ctypes.c_int64, ctypes.c_int64, _i32p, _f32p], ctypes.c_int),
+        "crispy_nw_dists": ([_vp, _i32p, ctypes.c_int64, ctypes.c_int, _f32p], ctypes.c_int),
+        "crispy_sort_by_dist": ([_vp, _i32p, _f32p, ctypes.c_int64], ctypes.c_int),
+        "crispy_timing_enable": ([_vp, ctypes.c_int], ctypes.c_int),
+        "crispy_timing_reset": ([_vp], ctypes.c_int),
+        "crispy_timing_query": ([_vp, ctypes.c_char_p, _f64p, _i64p, _f64p], ctypes.c_int),
+        "crispy_selftest_division": ([_vp, ctypes.c_int32, _f32p, _f32p], ctypes.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+class Reads:
+    """Reads as concatenated bytes plus int64 offsets (n + 1)."""
+
+    def __init__(self, bases: bytes, offsets, maxlen=None):
+        self.bases = bytes(bases)
+        self.offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        lens = np.diff(self.offsets)
+        self.maxlen = int(lens.max()) if maxlen is None and len(lens) else (maxlen or 0)
+
+    @property
+    def n(self):
+        return len(self.offsets) - 1
+
+    def seq(self, i):
+        return self.bases[self.offsets[i]:self.offsets[i + 1]]
+
+    def lengths(self):
+        return np.diff(self.offsets).astype(np.int32)
+
+    @classmethod
+    def from_list(cls, seqs):
+        seqs = [s.encode() if isinstance(s, str) else bytes(s) for s in seqs]
+        offs = np.zeros(len(seqs) + 1, np.int64)
+        offs[1:] = np.cumsum([len(s) for s in seqs])
+        return cls(b"".join(seqs), offs)
+
+
+def read_fasta(path, variant=KMER):
+    """The reference's FASTA reader (kmerMain.cpp:298-397 / genMain.cpp:184-277)."""
+    L = load_library()
+    b, o, ml = _vp(), _vp(), ctypes.c_int32()
+    n = L.crispy_read_fasta(os.fsencode(path), variant, ctypes.byref(b), ctypes.byref(o),
+                            ctypes.byref(ml))
+    if n < 0:
+        raise CrispyError(n, f"cannot read FASTA {path}")
+    try:
+        offs = np.ctypeslib.as_array(ctypes.cast(o, _i64p), shape=(n + 1,)).copy()
+        bases = ctypes.string_at(b, int(offs[-1])) if offs[-1] > 0 else b""
+    finally:
+        L.crispy_free(b)
+        L.crispy_free(o)
+    return Reads(bases, offs, ml.value)
+
+
+def kmer_default_threshold(n):
+    return float(load_library().crispy_kmer_default_threshold(n))
+
+
+def nw_default_threshold(n):
+    return float(load_library().crispy_nw_default_threshold(n))
+
+
+def passes(dist, thr):
+    return bool(load_library().crispy_pass(dist, thr))
+
+
+def band_cells(len1, len2, band):
+    return int(load_library().crispy_band_cells(len1, len2, band))
+
+
+class Context:
+    """One device context (stream + resident buffers)."""
+
+    def __init__(self, device=0):
+        self._L = load_library()
+        h = _vp()
+        rc = self._L.crispy_ctx_create(device, ctypes.byref(h))
+        if rc != 0:
+            raise CrispyError(rc, f"cannot open HIP device {device}")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.crispy_ctx_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc):
+        if rc != 0:
+            raise CrispyError(rc, self._L.crispy_ctx_error(self._h).decode(errors="replace"))
+
+    # -- reads
+    def upload_reads(self, slot, reads: Reads):
+        self._check(self._L.crispy_reads_upload(self._h, slot, reads.bases,
+                                                _p(reads.offsets, _i64p), reads.n))
+
+    # -- kmerDist
+    def kmer_run(self, K, threshold, tiles=None):
+        t0, t1 = (0, -1) if tiles is None else tiles
+        c = ctypes.c_int64()
+        self._check(self._L.crispy_kmer_run(self._h, K, threshold, t0, t1, ctypes.byref(c)))
+        self.kmer_count = c.value
+        return c.value
+
+    def kmer_fetch(self, first=0, count=None, dists=True):
+        if count is None:
+            count = self.kmer_count - first
+        p = np.zeros((count, 2), np.int32)
+        d = np.zeros(count, np.float32) if dists else None
+        self._check(self._L.crispy_kmer_fetch(self._h, first, count, _p(p, _i32p),
+                                              _p(d, _f32p) if dists else None))
+        return p, d
+
+    # -- genDist
+    def nw_run(self, pairs, band, threshold):
+        pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
+        c = ctypes.c_int64()
+        self._check(self._L.crispy_nw_run(self._h, _p(pairs, _i32p), len(pairs), band, threshold,
+                                          ctypes.byref(c)))
+        self.nw_count = c.value
+        return c.value
+
+    def nw_run_on_kmer(self, band, threshold):
+        c = ctypes.c_int64()
+        self._check(self._L.crispy_nw_run_on_kmer(self._h, band, threshold, ctypes.byref(c)))
+        self.nw_count = c.value
+        return c.value
+
+    def nw_fetch(self, first=0, count=None):
+        if count is None:
+            count = self.nw_count - first
+        p = np.zeros((count, 2), np.int32)
+        d = np.zeros(count, np.float32)
+        self._check(self._L.crispy_nw_fetch(self._h, first, count, _p(p, _i32p), _p(d, _f32p)))
+        return p, d
+
+    def nw_dists(self, pairs, band):
+        pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
+        d = np.zeros(len(pairs), np.float32)
+        self._check(self._L.crispy_nw_dists(self._h, _p(pairs, _i32p), len(pairs), band,
+                                            _p(d, _f32p)))
+        return d
+
+    def sort_by_dist(self, pairs, dists):
+        pairs = np.array(pairs, dtype=np.int32).reshape(-1, 2)
+        dists = np.array(dists, dtype=np.float32)
+        self._check(self._L.crispy_sort_by_dist(self._h, _p(pairs, _i32p), _p(dists, _f32p),
+                                                len(dists)))
+        return pairs, dists
+
+    # -- timing
+    def timing(self, enable=True):
+        self._check(self._L.crispy_timing_enable(self._h, int(enable)))
+
+    def timing_reset(self):
+        self._check(self._L.crispy_timing_reset(self._h))
+
+    def timing_query(self, kernel):
+        ms, n, u = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double()
+        self._check(self._L.crispy_timing_query(self._h, kernel.encode(), ctypes.byref(ms),
+                                                ctypes.byref(n), ctypes.byref(u)))
+        return ms.value, n.value, u.value
+
+    def sync(self):
+        self._check(self._L.crispy_ctx_sync(self._h))
+
+    def device_bytes(self):
+        return int(self._L.crispy_ctx_device_bytes(self._h))
+
+    def selftest_division(self, maxv):
+        a = np.zeros((maxv + 1) * (maxv + 1), np.float32)
+        b = np.zeros_like(a)
+        self._check(self._L.crispy_selftest_division(self._h, maxv, _p(a, _f32p), _p(b, _f32p)))
+        return a.reshape(maxv + 1, maxv + 1), b.reshape(maxv + 1, maxv + 1)

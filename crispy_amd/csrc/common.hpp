@@ -1,0 +1,162 @@
+// Shared host/device definitions of the crispy_amd HIP library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "crispy_hip.h"
+
+namespace crispy {
+
+// Scoring constants of genDist (genMain.h:32-35) and the out-of-band value
+// (genCPU.cpp:350,416,421).
+constexpr int kGO = -10;
+constexpr int kGE = -1;
+constexpr int kMismatch = -4;
+constexpr int kMatch = 5;
+constexpr int kNeg = -1000000;
+
+// Largest band half-width kB = (L1+L2)/(2 band) handled by the register
+// resident NW kernel (one template instance per kB).  Pairs above it, full
+// alignments (band <= 1) and pairs with non-ACGT bytes run the generic
+// kernel.
+constexpr int kKbRegMax = 32;
+constexpr int kNumBuckets = kKbRegMax + 2;  // [0, kKbRegMax] + generic
+constexpr int kGenericBucket = kKbRegMax + 1;
+
+// k-mer tile geometry.
+constexpr int kTile = 128;        // reads per tile side
+constexpr int kChunkWords = 16;   // bitmap words staged in LDS per step
+
+// Words of the k-mer presence bitmap of one read (4^K bits), padded to a
+// multiple of kChunkWords.
+__host__ __device__ constexpr int kmer_words(int K) {
+  return ((1 << (2 * K)) / 32) < kChunkWords ? kChunkWords : ((1 << (2 * K)) / 32);
+}
+
+// In-band cells #{1<=i<=L2, 1<=j<=L1 : |i-j| <= kb}.
+__host__ __device__ inline int64_t band_cells(int L1, int L2, int kb) {
+  // row i holds columns [max(1, i-kb), min(L1, i+kb)], non-empty for i <= L1+kb
+  int64_t total = 0;
+  const int iend = L2 < L1 + kb ? L2 : L1 + kb;
+  // piecewise linear in i with breakpoints at kb+1 and L1-kb
+  int bps[4] = {1, kb + 1, L1 - kb, iend + 1};
+  // sort the two inner breakpoints
+  if (bps[1] > bps[2]) { int t = bps[1]; bps[1] = bps[2]; bps[2] = t; }
+  int a = 1;
+  for (int s = 1; s <= 3; ++s) {
+    int b = bps[s];
+    if (b > iend + 1) b = iend + 1;
+    if (b > a) {
+      // rows [a, b): evaluate the count at both ends; it is linear inside
+      auto cnt = [&](int i) -> int64_t {
+        int hi = i + kb < L1 ? i + kb : L1;
+        int lo = i - kb > 1 ? i - kb : 1;
+        return (int64_t)(hi - lo + 1);
+      };
+      const int64_t c0 = cnt(a), c1 = cnt(b - 1);
+      total += (c0 + c1) * (int64_t)(b - a) / 2;
+      a = b;
+    }
+  }
+  return total;
+}
+
+}  // namespace crispy
+
+// ------------------------------------------------------------------ host side
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  template <class T> T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+struct ReadStore {
+  int32_t n = 0;
+  int64_t nbases = 0;
+  int32_t maxlen = 0;
+  int all_clean = 1;
+  DevBuf bases, off, len, pack, poff, clean;
+  std::vector<int32_t> h_len;
+  std::vector<uint8_t> h_clean;
+  int generation = 0;  // bumped on every upload (invalidates cached profiles)
+};
+
+struct KmerState {
+  // per-read profile for the current K
+  int K = 0;
+  int profile_gen = -1;
+  DevBuf bitmap, T, hinfo, rcnt, roff, rlist;
+  // result
+  DevBuf keys, keys_alt, dists, dists_alt, pairs, counter;
+  int64_t cap = 0;
+  int64_t count = 0;
+  bool valid = false;
+};
+
+struct NwState {
+  DevBuf in_pairs, keys, keys_alt, idx, idx_alt, dist, hist, cells, flags, sel, nsel;
+  DevBuf out_pairs, out_dists, scratch;
+  int64_t count = 0;
+  bool valid = false;
+};
+
+struct KStat {
+  double ms = 0;
+  int64_t launches = 0;
+  double units = 0;
+};
+
+struct PendingEvent {
+  std::string name;
+  hipEvent_t a, b;
+  double units;
+};
+
+struct crispy_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  ReadStore reads[2];
+  KmerState km;
+  NwState nw;
+  DevBuf temp;  // rocPRIM temporary storage
+  bool timing = false;
+  std::vector<PendingEvent> pending;
+  std::vector<hipEvent_t> pool;
+  std::map<std::string, KStat> stats;
+  int64_t dev_bytes = 0;
+};
+
+namespace crispy {
+
+int set_err(crispy_ctx *ctx, int code, const char *fmt, ...);
+int ensure(crispy_ctx *ctx, DevBuf &b, size_t bytes);
+void release(crispy_ctx *ctx, DevBuf &b);
+void timer_begin(crispy_ctx *ctx, const char *name, hipEvent_t *a);
+void timer_end(crispy_ctx *ctx, const char *name, hipEvent_t a, double units);
+
+}  // namespace crispy
+
+#define CRISPY_HIP(ctx, call)                                                                \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return crispy::set_err((ctx), CRISPY_ERR_HIP, "%s failed: %s (%s:%d)", #call,         \
+                             hipGetErrorString(e_), __FILE__, __LINE__);                    \
+  } while (0)
+
+#define CRISPY_TRY(expr)     \
+  do {                       \
+    int rc_ = (expr);        \
+    if (rc_ != CRISPY_OK)    \
+      return rc_;            \
+  } while (0)
+
+#define CRISPY_LAUNCHED(ctx) CRISPY_HIP(ctx, hipGetLastError())

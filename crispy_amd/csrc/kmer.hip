@@ -1,0 +1,561 @@
+// kmerDist on gfx950.
+//
+// The reference merges two sorted uint16 k-mer lists per pair
+// (kmerCPU.cpp:29-43, kmerKernel.cu:37-45).  That merge counts
+//     matches = I_hi + [hinfo_a == hinfo_b] * I_lo
+// where I_x = sum_v min(c_a[v], c_b[v]) over the codes of part x, "hi" are
+// the codes >= 0x8000 (only possible for K = 8, first base T or C), and
+// hinfo = (largest hi code, its multiplicity) or 0: the lists are sorted as
+// signed short (kmerMain.cpp:156-159, 275-279) and compared unsigned, so the
+// lo part is reached only when both hi parts run out together.  For K <= 7
+// every code is lo and hinfo is 0, so matches is the multiset intersection.
+//
+// sum_v min(c_a, c_b) = sum_v [c_a>=1][c_b>=1] + sum_v min(c_a-1, c_b-1)^+,
+// so each read is stored as
+//   * a presence bitmap of 4^K bits (padded to 16-word chunks), and
+//   * a short sorted list of (code << 16 | count-1) for codes seen twice or
+//     more ("repeats"),
+// and a 128x128 tile of pairs is an AND + popcount "GEMM" over the bitmaps
+// (v_bcnt_u32_b32 accumulates) plus a merge of the two repeat lists for
+// the few pairs where both have repeats.  The filter dist <= thr and the
+// compaction of passing pairs run in the same kernel; only passing pairs
+// leave the chip.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_select.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
+
+#include <algorithm>
+#include <cmath>
+
+#include "common.hpp"
+
+namespace crispy {
+
+// --------------------------------------------------------------- profiles
+// One block per read: codes of the T = L-K+1 k-mers (first base most
+// significant, as READ::formTuples kmerMain.cpp:117-154), a bitonic sort in
+// LDS, then runs -> bitmap bits, repeat entries and hinfo.
+template <int K>
+__global__ __launch_bounds__(256) void kmer_profile_kernel(
+    const uint32_t *__restrict__ pack, const int64_t *__restrict__ poff,
+    const int32_t *__restrict__ len, uint32_t *__restrict__ bitmap, int32_t *__restrict__ Tout,
+    uint32_t *__restrict__ hinfo, int32_t *__restrict__ rcnt, const int64_t *__restrict__ roff,
+    uint32_t *__restrict__ rlist) {
+  constexpr int WP = kmer_words(K);
+  __shared__ uint32_t codes[CRISPY_MAX_READ_LEN];
+  __shared__ uint32_t bm[WP];
+  __shared__ int scan[256];
+  const int r = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int L = len[r];
+  const int T = L - K + 1;  // >= 1, checked by the host
+  const uint32_t *s = pack + poff[r];
+  int P = 1;
+  while (P < T) P <<= 1;
+  for (int t = tid; t < P; t += 256) {
+    uint32_t c = 0xFFFFFFFFu;
+    if (t < T) {
+      c = 0;
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        const int p = t + q;
+        c = (c << 2) | ((s[p >> 4] >> (2 * (p & 15))) & 3u);
+      }
+    }
+    codes[t] = c;
+  }
+  for (int w = tid; w < WP; w += 256) bm[w] = 0;
+  __syncthreads();
+  for (int k = 2; k <= P; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int t = tid; t < P; t += 256) {
+        const int o = t ^ j;
+        if (o > t) {
+          const uint32_t a = codes[t], b = codes[o];
+          const bool up = (t & k) == 0;
+          if ((a > b) == up) {
+            codes[t] = b;
+            codes[o] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // runs: each thread owns the run starts inside its contiguous segment
+  const int seg = (T + 255) / 256;
+  const int lo = min(tid * seg, T), hi = min(lo + seg, T);
+  int nrep = 0;
+  for (int x = lo; x < hi; ++x) {
+    const uint32_t v = codes[x];
+    if (x == 0 || codes[x - 1] != v) {
+      int c = 1;
+      while (x + c < T && codes[x + c] == v) ++c;
+      atomicOr(&bm[v >> 5], 1u << (v & 31));
+      nrep += c >= 2;
+    }
+  }
+  scan[tid] = nrep;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const int add = tid >= off ? scan[tid - off] : 0;
+    __syncthreads();
+    scan[tid] += add;
+    __syncthreads();
+  }
+  int pos = scan[tid] - nrep;
+  uint32_t *out = rlist + roff[r];
+  for (int x = lo; x < hi; ++x) {
+    const uint32_t v = codes[x];
+    if (x == 0 || codes[x - 1] != v) {
+      int c = 1;
+      while (x + c < T && codes[x + c] == v) ++c;
+      if (c >= 2) out[pos++] = (v << 16) | (uint32_t)(c - 1);
+    }
+  }
+  if (tid == 255) rcnt[r] = scan[255];
+  if (tid == 0) {
+    Tout[r] = T;
+    const uint32_t v = codes[T - 1];
+    uint32_t h = 0;
+    if (v >= 0x8000u) {
+      int c = 1;
+      while (T - 1 - c >= 0 && codes[T - 1 - c] == v) ++c;
+      h = (v << 16) | (uint32_t)c;
+    }
+    hinfo[r] = h;
+  }
+  __syncthreads();
+  uint32_t *dst = bitmap + (int64_t)r * WP;
+  for (int w = tid; w < WP; w += 256) dst[w] = bm[w];
+}
+
+// sum_v min(e_a[v], e_b[v]) over two sorted repeat lists, split hi/lo.
+__device__ __forceinline__ void repeat_merge(const uint32_t *__restrict__ A, int na,
+                                             const uint32_t *__restrict__ B, int nb,
+                                             uint32_t &lo_sum, uint32_t &hi_sum) {
+  int x = 0, y = 0;
+  while (x < na && y < nb) {
+    const uint32_t ea = A[x], eb = B[y];
+    const uint32_t va = ea >> 16, vb = eb >> 16;
+    if (va == vb) {
+      const uint32_t m = min(ea & 0xFFFFu, eb & 0xFFFFu);
+      if (va & 0x8000u)
+        hi_sum += m;
+      else
+        lo_sum += m;
+      ++x;
+      ++y;
+    } else if (va < vb) {
+      ++x;
+    } else {
+      ++y;
+    }
+  }
+}
+
+// kmerMain.cpp:180 / kmerCPU.cpp:45
+__device__ __forceinline__ bool passes(float dist, float thr) {
+  const float diff = dist - thr;
+  return dist < thr || (double)fabsf(diff) < 0.00001;
+}
+
+__device__ __forceinline__ void tile_coords(int64_t t, int nb, int &bi, int &bj) {
+  // t = bi*nb - bi*(bi-1)/2 + (bj - bi), bi <= bj < nb
+  const double b = 2.0 * nb + 1.0;
+  int r = (int)((b - sqrt(b * b - 8.0 * (double)t)) * 0.5);
+  r = max(0, min(r, nb - 1));
+  auto start = [nb](int row) -> int64_t { return (int64_t)row * nb - (int64_t)row * (row - 1) / 2; };
+  while (r > 0 && start(r) > t) --r;
+  while (r + 1 < nb && start(r + 1) <= t) ++r;
+  bi = r;
+  bj = r + (int)(t - start(r));
+}
+
+// One 128x128 tile of read pairs per block of 256 threads; each thread owns
+// an 8x8 block of pairs: rows ty*4+{0..3} and 64+ty*4+{0..3}, columns
+// likewise with tx.  Bitmap words are staged in LDS 16 at a time, stored
+// word-major so each thread reads its 8 rows (or columns) with two 128-bit
+// LDS loads.  For K = 8 the hi half of the bitmap is processed first and
+// kept in a second accumulator set.
+template <int K>
+__global__ __launch_bounds__(256) void kmer_tile_kernel(
+    const uint32_t *__restrict__ bitmap, const int32_t *__restrict__ Tn,
+    const uint32_t *__restrict__ hinfo, const int32_t *__restrict__ rcnt,
+    const int64_t *__restrict__ roff, const uint32_t *__restrict__ rlist, int32_t n, int nb,
+    int64_t tile_begin, int64_t ntiles, int64_t per_xcd, float thr,
+    unsigned long long *__restrict__ counter, uint64_t *__restrict__ out_keys,
+    float *__restrict__ out_dists, int64_t cap) {
+  constexpr int WP = kmer_words(K);
+  constexpr int NCH = WP / kChunkWords;
+  constexpr bool kHi = K == 8;
+  __shared__ uint32_t As[kChunkWords][kTile];
+  __shared__ uint32_t Bs[kChunkWords][kTile];
+  __shared__ int32_t sT[2 * kTile];
+  __shared__ uint32_t sH[2 * kTile];
+  __shared__ int32_t sRc[2 * kTile];
+  __shared__ int64_t sRo[2 * kTile];
+
+  // Work-groups are dispatched round-robin over the 8 XCDs; give each XCD a
+  // contiguous run of tiles so tiles that share a row block share an L2.
+  const int64_t bid = blockIdx.x;
+  const int64_t local = (bid & 7) * per_xcd + (bid >> 3);
+  if (local >= ntiles) return;
+  int bi, bj;
+  tile_coords(tile_begin + local, nb, bi, bj);
+  const int row0 = bi * kTile, col0 = bj * kTile;
+  const int tid = threadIdx.x;
+  const int ty = tid >> 4, tx = tid & 15;
+
+  {
+    const int r = tid < kTile ? row0 + tid : col0 + (tid - kTile);
+    const bool ok = r < n;
+    sT[tid] = ok ? Tn[r] : 1;
+    sH[tid] = ok ? hinfo[r] : 0u;
+    sRc[tid] = ok ? rcnt[r] : 0;
+    sRo[tid] = ok ? roff[r] : 0;
+  }
+
+  uint32_t acc[8][8];
+  uint32_t acch[8][8];  // hi-half popcounts (K = 8 only)
+#pragma unroll
+  for (int x = 0; x < 8; ++x)
+#pragma unroll
+    for (int y = 0; y < 8; ++y) acc[x][y] = 0;
+  if constexpr (kHi) {
+#pragma unroll
+    for (int x = 0; x < 8; ++x)
+#pragma unroll
+      for (int y = 0; y < 8; ++y) acch[x][y] = 0;
+  }
+
+  const int lr = tid >> 1, lh = tid & 1;  // loader: row within the tile, half chunk
+  const bool aok = row0 + lr < n, bok = col0 + lr < n;
+  const uint32_t *ga = bitmap + (int64_t)(aok ? row0 + lr : 0) * WP + lh * 8;
+  const uint32_t *gb = bitmap + (int64_t)(bok ? col0 + lr : 0) * WP + lh * 8;
+  const uint4 z4 = make_uint4(0, 0, 0, 0);
+
+  auto chunk_of = [](int ch) { return kHi ? (ch + NCH / 2) % NCH : ch; };
+  int c0 = chunk_of(0);
+  uint4 pa0 = aok ? *(const uint4 *)(ga + c0 * kChunkWords) : z4;
+  uint4 pa1 = aok ? *(const uint4 *)(ga + c0 * kChunkWords + 4) : z4;
+  uint4 pb0 = bok ? *(const uint4 *)(gb + c0 * kChunkWords) : z4;
+  uint4 pb1 = bok ? *(const uint4 *)(gb + c0 * kChunkWords + 4) : z4;
+
+  for (int ch = 0; ch < NCH; ++ch) {
+    __syncthreads();
+    {
+      const int w = lh * 8;
+      As[w + 0][lr] = pa0.x; As[w + 1][lr] = pa0.y; As[w + 2][lr] = pa0.z; As[w + 3][lr] = pa0.w;
+      As[w + 4][lr] = pa1.x; As[w + 5][lr] = pa1.y; As[w + 6][lr] = pa1.z; As[w + 7][lr] = pa1.w;
+      Bs[w + 0][lr] = pb0.x; Bs[w + 1][lr] = pb0.y; Bs[w + 2][lr] = pb0.z; Bs[w + 3][lr] = pb0.w;
+      Bs[w + 4][lr] = pb1.x; Bs[w + 5][lr] = pb1.y; Bs[w + 6][lr] = pb1.z; Bs[w + 7][lr] = pb1.w;
+    }
+    __syncthreads();
+    if (ch + 1 < NCH) {  // prefetch the next chunk while this one is consumed
+      const int cn = chunk_of(ch + 1);
+      pa0 = aok ? *(const uint4 *)(ga + cn * kChunkWords) : z4;
+      pa1 = aok ? *(const uint4 *)(ga + cn * kChunkWords + 4) : z4;
+      pb0 = bok ? *(const uint4 *)(gb + cn * kChunkWords) : z4;
+      pb1 = bok ? *(const uint4 *)(gb + cn * kChunkWords + 4) : z4;
+    }
+#pragma unroll
+    for (int w = 0; w < kChunkWords; ++w) {
+      const uint4 ra0 = *(const uint4 *)&As[w][ty * 4];
+      const uint4 ra1 = *(const uint4 *)&As[w][64 + ty * 4];
+      const uint4 rb0 = *(const uint4 *)&Bs[w][tx * 4];
+      const uint4 rb1 = *(const uint4 *)&Bs[w][64 + tx * 4];
+      const uint32_t a[8] = {ra0.x, ra0.y, ra0.z, ra0.w, ra1.x, ra1.y, ra1.z, ra1.w};
+      const uint32_t b[8] = {rb0.x, rb0.y, rb0.z, rb0.w, rb1.x, rb1.y, rb1.z, rb1.w};
+#pragma unroll
+      for (int x = 0; x < 8; ++x)
+#pragma unroll
+        for (int y = 0; y < 8; ++y) acc[x][y] += __popc(a[x] & b[y]);
+    }
+    if constexpr (kHi) {
+      if (ch == NCH / 2 - 1) {
+#pragma unroll
+        for (int x = 0; x < 8; ++x)
+#pragma unroll
+          for (int y = 0; y < 8; ++y) {
+            acch[x][y] = acc[x][y];
+            acc[x][y] = 0;
+          }
+      }
+    }
+  }
+
+  // ---- epilogue: dist, filter, compaction
+  auto dist_of = [&](int x, int y, uint32_t pop_lo, uint32_t pop_hi) -> float {
+    const int li = x < 4 ? ty * 4 + x : 64 + ty * 4 + (x - 4);
+    const int lj = kTile + (y < 4 ? tx * 4 + y : 64 + tx * 4 + (y - 4));
+    uint32_t lo = pop_lo, hi = pop_hi;
+    const int na = sRc[li], nbr = sRc[lj];
+    if (na > 0 && nbr > 0) repeat_merge(rlist + sRo[li], na, rlist + sRo[lj], nbr, lo, hi);
+    uint32_t matches = lo;
+    if constexpr (kHi) matches = hi + (sH[li] == sH[lj] ? lo : 0u);
+    const int tmin = min(sT[li], sT[lj]);
+    return 1.0f - (float)matches / (float)tmin;
+  };
+  uint64_t passmask = 0;
+  int mycount = 0;
+#pragma unroll
+  for (int x = 0; x < 8; ++x) {
+    const int i = row0 + (x < 4 ? ty * 4 + x : 64 + ty * 4 + (x - 4));
+#pragma unroll
+    for (int y = 0; y < 8; ++y) {
+      const int j = col0 + (y < 4 ? tx * 4 + y : 64 + tx * 4 + (y - 4));
+      if (i < j && j < n) {
+        const float d = dist_of(x, y, acc[x][y], kHi ? acch[x][y] : 0u);
+        if (passes(d, thr)) {
+          passmask |= 1ull << (x * 8 + y);
+          ++mycount;
+        }
+      }
+    }
+  }
+  // wave-level exclusive scan of the per-thread counts, one atomic per wave
+  const int lane = tid & 63;
+  int incl = mycount;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int v = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += v;
+  }
+  const int total = __shfl(incl, 63, 64);
+  unsigned long long base = 0;
+  if (total > 0) {
+    if (lane == 63) base = atomicAdd(counter, (unsigned long long)total);
+    base = __shfl(base, 63, 64);
+  }
+  if (mycount == 0) return;
+  int64_t pos = (int64_t)base + (incl - mycount);
+#pragma unroll
+  for (int x = 0; x < 8; ++x) {
+    const int i = row0 + (x < 4 ? ty * 4 + x : 64 + ty * 4 + (x - 4));
+#pragma unroll
+    for (int y = 0; y < 8; ++y) {
+      if (passmask & (1ull << (x * 8 + y))) {
+        const int j = col0 + (y < 4 ? tx * 4 + y : 64 + tx * 4 + (y - 4));
+        if (pos < cap) {
+          out_keys[pos] = ((uint64_t)i << 20) | (uint64_t)j;
+          out_dists[pos] = dist_of(x, y, acc[x][y], kHi ? acch[x][y] : 0u);
+        }
+        ++pos;
+      }
+    }
+  }
+}
+
+__global__ void unpack_keys_kernel(const uint64_t *__restrict__ keys, int64_t n, int2 *__restrict__ pairs) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t v = keys[k];
+    pairs[k] = make_int2((int)(v >> 20), (int)(v & 0xFFFFFu));
+  }
+}
+
+// ------------------------------------------------------------ host driver
+template <int K>
+static int launch_profile(crispy_ctx *ctx, ReadStore &rs) {
+  KmerState &km = ctx->km;
+  hipEvent_t ev;
+  timer_begin(ctx, "kmer_profile", &ev);
+  kmer_profile_kernel<K><<<rs.n, 256, 0, ctx->stream>>>(
+      rs.pack.as<uint32_t>(), rs.poff.as<int64_t>(), rs.len.as<int32_t>(), km.bitmap.as<uint32_t>(),
+      km.T.as<int32_t>(), km.hinfo.as<uint32_t>(), km.rcnt.as<int32_t>(), km.roff.as<int64_t>(),
+      km.rlist.as<uint32_t>());
+  CRISPY_LAUNCHED(ctx);
+  timer_end(ctx, "kmer_profile", ev, rs.n);
+  return CRISPY_OK;
+}
+
+template <int K>
+static int launch_tiles(crispy_ctx *ctx, int32_t n, int nb, int64_t tb, int64_t nt, float thr) {
+  KmerState &km = ctx->km;
+  const int64_t per_xcd = (nt + 7) / 8;
+  const int64_t grid = per_xcd * 8;
+  // pairs evaluated (i < j) in the tile range, for the timing units
+  hipEvent_t ev;
+  timer_begin(ctx, "kmer_tile", &ev);
+  kmer_tile_kernel<K><<<(unsigned)grid, 256, 0, ctx->stream>>>(
+      km.bitmap.as<uint32_t>(), km.T.as<int32_t>(), km.hinfo.as<uint32_t>(), km.rcnt.as<int32_t>(),
+      km.roff.as<int64_t>(), km.rlist.as<uint32_t>(), n, nb, tb, nt, per_xcd, thr,
+      km.counter.as<unsigned long long>(), km.keys.as<uint64_t>(), km.dists.as<float>(), km.cap);
+  CRISPY_LAUNCHED(ctx);
+  double units = 0;
+  if (ctx->timing) {
+    // exact count of the i < j < n pairs inside the selected tiles
+    int64_t rs = 0;
+    for (int bi = 0; bi < nb; ++bi) {
+      const int64_t re = rs + (nb - bi);
+      const int64_t a = std::max(rs, tb), b = std::min(re, tb + nt);
+      const int64_t rh = std::min<int64_t>(kTile, n - (int64_t)bi * kTile);
+      for (int64_t t = a; t < b; ++t) {
+        const int bj = bi + (int)(t - rs);
+        const int64_t chh = std::min<int64_t>(kTile, n - (int64_t)bj * kTile);
+        units += bi == bj ? (double)rh * (rh - 1) / 2 : (double)rh * chh;
+      }
+      rs = re;
+    }
+  }
+  timer_end(ctx, "kmer_tile", ev, units);
+  return CRISPY_OK;
+}
+
+using ProfileFn = int (*)(crispy_ctx *, ReadStore &);
+using TileFn = int (*)(crispy_ctx *, int32_t, int, int64_t, int64_t, float);
+static const ProfileFn kProfile[9] = {nullptr, launch_profile<1>, launch_profile<2>,
+                                      launch_profile<3>, launch_profile<4>, launch_profile<5>,
+                                      launch_profile<6>, launch_profile<7>, launch_profile<8>};
+static const TileFn kTiles[9] = {nullptr, launch_tiles<1>, launch_tiles<2>, launch_tiles<3>,
+                                 launch_tiles<4>, launch_tiles<5>, launch_tiles<6>, launch_tiles<7>,
+                                 launch_tiles<8>};
+
+static int build_profiles(crispy_ctx *ctx, int K) {
+  ReadStore &rs = ctx->reads[CRISPY_READS_KMER];
+  KmerState &km = ctx->km;
+  if (km.K == K && km.profile_gen == rs.generation) return CRISPY_OK;
+  const int32_t n = rs.n;
+  std::vector<int64_t> roff(n + 1, 0);
+  for (int32_t r = 0; r < n; ++r) {
+    const int T = rs.h_len[r] - K + 1;
+    if (T < 1)
+      return set_err(ctx, CRISPY_ERR_ARG, "read %d is shorter than K=%d (length %d)", r, K, rs.h_len[r]);
+    roff[r + 1] = roff[r] + T / 2 + 1;
+  }
+  const int WP = kmer_words(K);
+  CRISPY_TRY(ensure(ctx, km.bitmap, sizeof(uint32_t) * (size_t)n * WP));
+  CRISPY_TRY(ensure(ctx, km.T, sizeof(int32_t) * (n + 1)));
+  CRISPY_TRY(ensure(ctx, km.hinfo, sizeof(uint32_t) * (n + 1)));
+  CRISPY_TRY(ensure(ctx, km.rcnt, sizeof(int32_t) * (n + 1)));
+  CRISPY_TRY(ensure(ctx, km.roff, sizeof(int64_t) * (n + 1)));
+  CRISPY_TRY(ensure(ctx, km.rlist, sizeof(uint32_t) * (roff[n] + 1)));
+  CRISPY_HIP(ctx, hipMemcpyAsync(km.roff.p, roff.data(), sizeof(int64_t) * (n + 1),
+                                 hipMemcpyHostToDevice, ctx->stream));
+  CRISPY_TRY(kProfile[K](ctx, rs));
+  // roff must stay alive until the copy has run
+  CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  km.K = K;
+  km.profile_gen = rs.generation;
+  return CRISPY_OK;
+}
+
+static int sort_keys(crispy_ctx *ctx, int64_t count) {
+  KmerState &km = ctx->km;
+  if (count <= 1) {
+    if (count == 1) {
+      CRISPY_HIP(ctx, hipMemcpyAsync(km.keys_alt.p, km.keys.p, 8, hipMemcpyDeviceToDevice, ctx->stream));
+      CRISPY_HIP(ctx, hipMemcpyAsync(km.dists_alt.p, km.dists.p, 4, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    return CRISPY_OK;
+  }
+  size_t tbytes = 0;
+  CRISPY_HIP(ctx, rocprim::radix_sort_pairs(nullptr, tbytes, km.keys.as<uint64_t>(),
+                                            km.keys_alt.as<uint64_t>(), km.dists.as<float>(),
+                                            km.dists_alt.as<float>(), (size_t)count, 0, 40,
+                                            ctx->stream));
+  CRISPY_TRY(ensure(ctx, ctx->temp, tbytes));
+  hipEvent_t ev;
+  timer_begin(ctx, "kmer_sort", &ev);
+  CRISPY_HIP(ctx, rocprim::radix_sort_pairs(ctx->temp.p, tbytes, km.keys.as<uint64_t>(),
+                                            km.keys_alt.as<uint64_t>(), km.dists.as<float>(),
+                                            km.dists_alt.as<float>(), (size_t)count, 0, 40,
+                                            ctx->stream));
+  timer_end(ctx, "kmer_sort", ev, (double)count);
+  return CRISPY_OK;
+}
+
+}  // namespace crispy
+
+using namespace crispy;
+
+extern "C" {
+
+int64_t crispy_kmer_num_tiles(int32_t n) {
+  const int64_t nb = (n + kTile - 1) / kTile;
+  return nb * (nb + 1) / 2;
+}
+
+int crispy_kmer_run(crispy_ctx *ctx, int K, float threshold, int64_t tile_begin, int64_t tile_end,
+                    int64_t *count) {
+  if (!ctx) return CRISPY_ERR_ARG;
+  if (K < 1 || K > 8) return set_err(ctx, CRISPY_ERR_ARG, "unsupported K=%d (1..8)", K);
+  CRISPY_HIP(ctx, hipSetDevice(ctx->device));
+  ReadStore &rs = ctx->reads[CRISPY_READS_KMER];
+  KmerState &km = ctx->km;
+  km.valid = false;
+  ctx->nw.valid = false;
+  const int32_t n = rs.n;
+  const int nb = (n + kTile - 1) / kTile;
+  const int64_t total_tiles = (int64_t)nb * (nb + 1) / 2;
+  if (tile_end < 0 || tile_end > total_tiles) tile_end = total_tiles;
+  if (tile_begin < 0 || tile_begin > tile_end)
+    return set_err(ctx, CRISPY_ERR_ARG, "bad tile range [%lld, %lld)", (long long)tile_begin,
+                   (long long)tile_end);
+  const int64_t nt = tile_end - tile_begin;
+  km.count = 0;
+  if (n < 2 || nt == 0) {
+    km.valid = true;
+    if (count) *count = 0;
+    return CRISPY_OK;
+  }
+  CRISPY_TRY(build_profiles(ctx, K));
+  if (km.cap == 0) km.cap = (int64_t)1 << 22;
+  CRISPY_TRY(ensure(ctx, km.counter, sizeof(unsigned long long)));
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    CRISPY_TRY(ensure(ctx, km.keys, sizeof(uint64_t) * km.cap));
+    CRISPY_TRY(ensure(ctx, km.dists, sizeof(float) * km.cap));
+    CRISPY_HIP(ctx, hipMemsetAsync(km.counter.p, 0, sizeof(unsigned long long), ctx->stream));
+    CRISPY_TRY(kTiles[K](ctx, n, nb, tile_begin, nt, threshold));
+    unsigned long long c = 0;
+    CRISPY_HIP(ctx, hipMemcpyAsync(&c, km.counter.p, sizeof c, hipMemcpyDeviceToHost, ctx->stream));
+    CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    km.count = (int64_t)c;
+    if (km.count <= km.cap) break;
+    km.cap = km.count + km.count / 8 + 1024;  // rerun with room for every passing pair
+  }
+  CRISPY_TRY(ensure(ctx, km.keys_alt, sizeof(uint64_t) * (km.count + 1)));
+  CRISPY_TRY(ensure(ctx, km.dists_alt, sizeof(float) * (km.count + 1)));
+  CRISPY_TRY(ensure(ctx, km.pairs, sizeof(int2) * (km.count + 1)));
+  CRISPY_TRY(sort_keys(ctx, km.count));
+  if (km.count > 0) {
+    const int blocks = (int)std::min<int64_t>((km.count + 255) / 256, 4096);
+    unpack_keys_kernel<<<blocks, 256, 0, ctx->stream>>>(km.keys_alt.as<uint64_t>(), km.count,
+                                                        km.pairs.as<int2>());
+    CRISPY_LAUNCHED(ctx);
+  }
+  CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  km.valid = true;
+  if (count) *count = km.count;
+  return CRISPY_OK;
+}
+
+int crispy_kmer_fetch(crispy_ctx *ctx, int64_t first, int64_t count, int32_t *pairs, float *dists) {
+  if (!ctx) return CRISPY_ERR_ARG;
+  KmerState &km = ctx->km;
+  if (!km.valid) return set_err(ctx, CRISPY_ERR_STATE, "no k-mer result");
+  if (first < 0 || count < 0 || first + count > km.count)
+    return set_err(ctx, CRISPY_ERR_ARG, "fetch range outside the result");
+  if (count == 0) return CRISPY_OK;
+  CRISPY_HIP(ctx, hipSetDevice(ctx->device));
+  if (pairs)
+    CRISPY_HIP(ctx, hipMemcpyAsync(pairs, km.pairs.as<int2>() + first, sizeof(int2) * count,
+                                   hipMemcpyDeviceToHost, ctx->stream));
+  if (dists)
+    CRISPY_HIP(ctx, hipMemcpyAsync(dists, km.dists_alt.as<float>() + first, sizeof(float) * count,
+                                   hipMemcpyDeviceToHost, ctx->stream));
+  CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return CRISPY_OK;
+}
+
+}  // extern "C"
+
+// Used by the NW driver for the fused path.
+namespace crispy {
+const int2 *kmer_result_pairs(crispy_ctx *ctx) { return ctx->km.pairs.as<int2>(); }
+}  // namespace crispy

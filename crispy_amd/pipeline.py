@@ -1,0 +1,79 @@
+"""Python mirrors of the reference's two drivers, file behaviour included.
+
+compute_kmer_dist  ~ computeKmerDist_CUDA (kmerMain.h:73, kmerCUDA.cu:14-126)
+compute_gen_dist   ~ computeGenDist_CUDA  (genMain.h:77,  genCUDA.cu:83-270)
+
+Both run on the GPU through the C ABI; the executables bin/kmerDist and
+bin/genDist implement the same logic in C++ for the command line.
+"""
+import numpy as np
+
+from .api import KMER, NW, Context, Reads
+
+CHUNK_PAIRS = 576 * 256 * 4   # NUM_PAIRS, genMain.h:24-27 (pairs per .kpair chunk)
+FLUSH_PAIRS = 576 * 256       # genCPU.cpp:339,484 (a file is written at >= this many)
+
+
+def compute_kmer_dist(ctx: Context, reads: Reads, K: int, threshold: float, pair_path: str,
+                      dist_path: str = None, block=1 << 20):
+    """Writes <pair_path> (int32 i, j) and optionally <dist_path> (float32)
+    for every pair i < j whose k-mer distance passes `threshold`, in (i, j)
+    order.  Returns the number of pairs."""
+    ctx.upload_reads(KMER, reads)
+    count = ctx.kmer_run(K, threshold)
+    with open(pair_path, "wb") as pf:
+        df = open(dist_path, "wb") if dist_path else None
+        try:
+            for first in range(0, count, block):
+                p, d = ctx.kmer_fetch(first, min(block, count - first), dists=df is not None)
+                p.tofile(pf)
+                if df is not None:
+                    d.tofile(df)
+        finally:
+            if df is not None:
+                df.close()
+    return count
+
+
+def compute_gen_dist(ctx: Context, reads: Reads, in_pair_path: str, pair_prefix: str,
+                     dist_prefix: str, threshold: float, band: int):
+    """Reads <in_pair_path> in chunks, keeps pairs whose NW distance passes
+    and writes <pair_prefix>_<f> (uint32 pairs) / <dist_prefix>_<f>
+    (float32), each ascending by distance (ties by (i, j)).  Returns
+    (number of files, number of pairs)."""
+    ctx.upload_reads(NW, reads)
+    acc_p, acc_d = [], []
+    nacc = 0
+    nfiles = 0
+    total = 0
+
+    def flush():
+        nonlocal nfiles, nacc, acc_p, acc_d
+        p = np.concatenate(acc_p) if acc_p else np.zeros((0, 2), np.int32)
+        d = np.concatenate(acc_d) if acc_d else np.zeros(0, np.float32)
+        p, d = ctx.sort_by_dist(p, d)
+        p.astype(np.uint32).tofile(f"{pair_prefix}_{nfiles}")
+        d.tofile(f"{dist_prefix}_{nfiles}")
+        nfiles += 1
+        acc_p, acc_d, nacc = [], [], 0
+
+    with open(in_pair_path, "rb") as f:
+        while True:
+            raw = np.fromfile(f, dtype=np.int32, count=2 * CHUNK_PAIRS)
+            eof = len(raw) < 2 * CHUNK_PAIRS
+            pairs = raw[: len(raw) // 2 * 2].reshape(-1, 2)
+            c = ctx.nw_run(pairs, band, threshold)
+            if c:
+                p, d = ctx.nw_fetch()
+                acc_p.append(p)
+                acc_d.append(d)
+                nacc += c
+            if nacc >= FLUSH_PAIRS:
+                total += nacc
+                flush()
+            if eof:
+                break
+    if nacc > 0:
+        total += nacc
+        flush()
+    return nfiles, total

@@ -1,0 +1,158 @@
+/*
+ * crispy_hip.h -- C ABI of the MI355X (gfx950) kmerDist / genDist library.
+ *
+ * Plain C types only (no HIP, torch or C++ types), so the reference's C++
+ * drivers, the drop-in executables bin/kmerDist and bin/genDist, Python
+ * (ctypes) and any other FFI can bind it.
+ *
+ * Replaces these reference seams:
+ *   computeKmerDist_CUDA   kmerMain.h:73, kmerCUDA.cu:14-126
+ *   launchKmerKernel       kmerKernel.h:41, kmerKernel.cu:53-58
+ *   computeGenDist_CUDA    genMain.h:77,  genCUDA.cu:83-270
+ *   launchGenKernel_band   genKernel.h:48 (and _full, :46)
+ *   writeVectorToFile_GPU  genCUDA.cu:17-81 (its thrust::sort_by_key)
+ *   readFile / thresholds  kmerMain.cpp:298-397,43-53; genMain.cpp:184-277,42-47
+ *
+ * Errors: every entry point returns CRISPY_OK (0) or a negative code and
+ * never exits the process (the reference calls exit(-1) from
+ * checkCudaErrors, kmerKernel.h:17-25).  crispy_ctx_error() returns the
+ * message of the last failure on that context.
+ *
+ * Threading: a context owns one HIP stream and its device buffers; calls on
+ * one context must not overlap.  Different contexts are independent (the
+ * reference's module-global textures, kmerKernel.cu:12 / genKernel.cu:11,
+ * made it non-reentrant).
+ */
+#ifndef CRISPY_HIP_H
+#define CRISPY_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CRISPY_OK 0
+#define CRISPY_ERR_ARG (-1)     /* invalid argument or input outside the reference's limits */
+#define CRISPY_ERR_HIP (-2)     /* HIP runtime error (includes "no device")                 */
+#define CRISPY_ERR_NOMEM (-3)   /* device or host allocation failed                         */
+#define CRISPY_ERR_STATE (-4)   /* call order (e.g. fetch before run)                       */
+#define CRISPY_ERR_IO (-5)      /* file could not be opened / read / written                */
+
+/* Limits of the reference (kmerMain.h:29-30, genMain.h:29-30). */
+#define CRISPY_MAX_READ_LEN 2048
+#define CRISPY_MAX_NUM_READS (1024 * 1024)
+
+/* Read-store slots: kmerDist and genDist trim reads differently
+ * (kmerMain.cpp:289-294 keeps [ATGCatgc], genMain.cpp:175-181 [nNATGCatgc]). */
+#define CRISPY_READS_KMER 0
+#define CRISPY_READS_NW 1
+
+typedef struct crispy_ctx crispy_ctx;
+
+/* ---------------------------------------------------------------- context */
+/* Opens `device` (HIP ordinal) and creates the context's stream. */
+int crispy_ctx_create(int device, crispy_ctx **out);
+void crispy_ctx_destroy(crispy_ctx *ctx);
+const char *crispy_ctx_error(const crispy_ctx *ctx);
+/* Blocks until the context's stream is idle. */
+int crispy_ctx_sync(crispy_ctx *ctx);
+/* Device bytes currently held by the context. */
+int64_t crispy_ctx_device_bytes(const crispy_ctx *ctx);
+
+/* ------------------------------------------------------------- host side */
+/* FASTA reader of the reference (kmerMain.cpp:298-397 for variant
+ * CRISPY_READS_KMER, genMain.cpp:184-277 for CRISPY_READS_NW): reads become
+ * concatenated bytes plus n+1 offsets, allocated with malloc and released
+ * with crispy_free.  Returns the number of reads, or CRISPY_ERR_IO /
+ * CRISPY_ERR_ARG (first line not '>').  *maxlen is the true maximum length. */
+int32_t crispy_read_fasta(const char *path, int variant, char **bases, int64_t **offsets,
+                          int32_t *maxlen);
+void crispy_free(void *p);
+
+/* Default thresholds used when -t is absent or outside [0, 1]
+ * (kmerMain.cpp:43-48, genMain.cpp:42-47). */
+float crispy_kmer_default_threshold(int32_t num_reads);
+float crispy_nw_default_threshold(int32_t num_reads);
+/* The filter predicate dist < thr || |dist - thr| < 1e-5 (kmerMain.cpp:180). */
+int crispy_pass(float dist, float threshold);
+/* In-band DP cells of one pair: #{1<=i<=L2, 1<=j<=L1 : |i-j| <= (L1+L2)/(2 band)},
+ * or L1*L2 for band <= 1 (full alignment). */
+int64_t crispy_band_cells(int32_t len1, int32_t len2, int32_t band);
+
+/* ----------------------------------------------------------- read store */
+/* Uploads n reads (raw bytes, offsets[n+1]) into `slot`, then packs them on
+ * the device: 2-bit codes A0 G1 T2 C3 (any other byte codes 0, as in
+ * kmerMain.cpp:137-152), 16 bases per 32-bit word, plus a per-read flag set
+ * when every byte is one of 'A','C','G','T'.  Replaces the tupleSet /
+ * texture upload (kmerCUDA.cu:35-72) and convertToBinary + texture upload
+ * (genMain.cpp:280-321, genCUDA.cu:94-144).  Reads stay resident until the
+ * slot is re-uploaded or the context is destroyed. */
+int crispy_reads_upload(crispy_ctx *ctx, int slot, const char *bases, const int64_t *offsets,
+                        int32_t n);
+
+/* ------------------------------------------------------------- kmerDist */
+/* All-pairs k-mer distance over the reads of slot CRISPY_READS_KMER:
+ * dist(i,j) = 1 - matches/min(T_i, T_j), T = L-K+1, matches exactly as the
+ * reference's merge over signed-sorted 16-bit codes (kmerCPU.cpp:29-43,
+ * kmerMain.cpp:156-159) -- including its K = 8 ordering -- and keeps pairs
+ * i < j that pass `threshold`.  The result stays on the device in (i, j)
+ * order; *count is the number of passing pairs.
+ * tile_begin/tile_end select a range of the 128x128 upper-triangular tiles
+ * (0 and -1 = all), which is how the work shards across devices.
+ * Requires 1 <= K <= 8 and every read length >= K. */
+int crispy_kmer_run(crispy_ctx *ctx, int K, float threshold, int64_t tile_begin,
+                    int64_t tile_end, int64_t *count);
+/* Number of 128x128 tiles for n reads (the shard index space). */
+int64_t crispy_kmer_num_tiles(int32_t num_reads);
+/* Copies result entries [first, first+count) to host: pairs as int32 (i, j),
+ * dists as float32 (either pointer may be NULL). */
+int crispy_kmer_fetch(crispy_ctx *ctx, int64_t first, int64_t count, int32_t *pairs,
+                      float *dists);
+
+/* -------------------------------------------------------------- genDist */
+/* Banded (band >= 2) or full (band <= 1) Needleman-Wunsch distance of each
+ * pair (i, j) of reads in slot CRISPY_READS_NW: read i on the column axis,
+ * read j on the row axis, scoring and tie rules of computeGenDist_CPU_band
+ * / _full (genCPU.cpp:290-464, :81-225); dist = ML/AL.  Keeps the pairs
+ * passing `threshold` in input order; the result stays on the device.
+ * pairs: host int32 (i, j) x npairs.  Replaces one chunk of
+ * computeGenDist_CUDA (genCUDA.cu:156-226). */
+int crispy_nw_run(crispy_ctx *ctx, const int32_t *pairs, int64_t npairs, int band,
+                  float threshold, int64_t *count);
+/* Fused form: the pairs are the device-resident result of the last
+ * crispy_kmer_run (no host round trip). */
+int crispy_nw_run_on_kmer(crispy_ctx *ctx, int band, float threshold, int64_t *count);
+int crispy_nw_fetch(crispy_ctx *ctx, int64_t first, int64_t count, int32_t *pairs,
+                    float *dists);
+/* Unfiltered distances for host pairs (every pair gets a distance). */
+int crispy_nw_dists(crispy_ctx *ctx, const int32_t *pairs, int64_t npairs, int band,
+                    float *dists);
+/* Sorts host arrays (pairs int32 x2, dists) ascending by (dist, i, j) on the
+ * device; the per-file order of writeVectorToFile_GPU (genCUDA.cu:39-45). */
+int crispy_sort_by_dist(crispy_ctx *ctx, int32_t *pairs, float *dists, int64_t n);
+
+/* --------------------------------------------------------------- timing */
+/* When enabled, every kernel launch is bracketed by HIP events on the
+ * context's stream.  Query returns the summed duration (ms), the number of
+ * launches and the algorithmic work units since the last reset:
+ *   "kmer_profile" reads, "kmer_tile" read pairs evaluated (i < j),
+ *   "nw_band_reg" and "nw_generic" in-band DP cells, "kmer_sort",
+ *   "nw_bucket", "nw_filter" elements. */
+int crispy_timing_enable(crispy_ctx *ctx, int enable);
+int crispy_timing_reset(crispy_ctx *ctx);
+int crispy_timing_query(crispy_ctx *ctx, const char *kernel, double *total_ms,
+                        int64_t *launches, double *work_units);
+
+/* ---------------------------------------------------------- self checks */
+/* Evaluates on the device the two float formulas the kernels use, for every
+ * integer operand pair: out_kmer[t*(maxv+1)+m] = 1 - m/t for 1<=t<=maxv,
+ * 0<=m<=maxv, and out_nw[a*(maxv+1)+m] = m/a likewise.  Lets a test compare
+ * the device rounding with the host's bit for bit. */
+int crispy_selftest_division(crispy_ctx *ctx, int32_t maxv, float *out_kmer, float *out_nw);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CRISPY_HIP_H */

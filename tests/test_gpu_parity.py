@@ -1,0 +1,386 @@
+"""GPU parity: the HIP path (through the C ABI) against the reference's
+golden outputs and the CPU oracle, bit for bit (integer counts and the
+float32 distances from the same formulas)."""
+import hashlib
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import crispy_amd as ca
+from crispy_amd import synth
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+CLEAN = os.path.join(GOLD, "V2Mice_sample1000_Clean")
+ADV = os.path.join(GOLD, "adversarial.fa")
+BIN = os.path.join(ROOT, "bin")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = ca.Context(0)
+    yield c
+    c.close()
+
+
+def canon(p, d):
+    p = np.asarray(p).astype(np.int64).reshape(-1, 2)
+    d = np.asarray(d, np.float32).view(np.uint32).astype(np.int64)
+    key = np.lexsort((p[:, 1], p[:, 0], d))
+    return np.stack([d[key], p[key, 0], p[key, 1]], 1)
+
+
+def digest(p, d):
+    c = canon(p, d)
+    rec = np.zeros(len(c), dtype=[("d", "<u4"), ("i", "<u4"), ("j", "<u4")])
+    rec["d"], rec["i"], rec["j"] = c[:, 0], c[:, 1], c[:, 2]
+    return hashlib.sha256(rec.tobytes()).hexdigest()
+
+
+def all_pairs(n):
+    i, j = np.triu_indices(n, 1)
+    return np.stack([i, j], 1).astype(np.int32)
+
+
+def sub_reads(path, n, variant):
+    r = ca.read_fasta(path, variant)
+    return ca.Reads(r.bases[:r.offsets[n]], r.offsets[:n + 1])
+
+
+def bits(d):
+    return np.asarray(d, np.float32).view(np.uint32)
+
+
+# ---------------------------------------------------------------- arithmetic
+def test_division_matches_host(ctx):
+    say("division table 2048")
+    k, w = ctx.selftest_division(2048)
+    m = np.arange(2049, dtype=np.float32)[None, :]
+    a = np.arange(2049, dtype=np.float32)[:, None]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ek = np.float32(1.0) - m / a
+        ew = m / a
+    ek[0, :] = 0
+    ew[0, :] = 0
+    assert np.array_equal(bits(k), bits(ek.astype(np.float32)))
+    assert np.array_equal(bits(w), bits(ew.astype(np.float32)))
+
+
+# ------------------------------------------------------------------ kmerDist
+@pytest.mark.parametrize("K", range(4, 9))
+def test_kmer_clean_sample_golden(ctx, K):
+    say("kmer golden K", K)
+    g = np.load(os.path.join(GOLD, "kmer_v2mice.npz"))
+    r = ca.read_fasta(CLEAN, ca.KMER)
+    ctx.upload_reads(ca.KMER, r)
+    c = ctx.kmer_run(K, ca.kmer_default_threshold(r.n))
+    p, d = ctx.kmer_fetch()
+    assert c == len(g[f"k{K}_pairs"])
+    assert np.array_equal(p, g[f"k{K}_pairs"])
+    assert np.array_equal(bits(d), bits(g[f"k{K}_dists"]))
+
+
+@pytest.mark.parametrize("K", range(1, 9))
+def test_kmer_all_pairs_golden(ctx, K):
+    say("kmer all pairs K", K)
+    g = np.load(os.path.join(GOLD, "kmer_sub120.npz"))
+    ctx.upload_reads(ca.KMER, sub_reads(CLEAN, 120, ca.KMER))
+    c = ctx.kmer_run(K, 1.0)
+    p, d = ctx.kmer_fetch()
+    assert c == 120 * 119 // 2
+    assert np.array_equal(p, all_pairs(120))
+    assert np.array_equal(bits(d), bits(g[f"k{K}_dists"]))
+
+
+@pytest.mark.parametrize("K", [3, 6, 8])
+def test_kmer_adversarial_golden(ctx, K):
+    say("kmer adversarial K", K)
+    g = np.load(os.path.join(GOLD, "adversarial.npz"))
+    ctx.upload_reads(ca.KMER, ca.read_fasta(ADV, ca.KMER))
+    ctx.kmer_run(K, 1.0)
+    p, d = ctx.kmer_fetch()
+    assert np.array_equal(p, g[f"k{K}_pairs"])
+    assert np.array_equal(bits(d), bits(g[f"k{K}_dists"]))
+
+
+def random_reads(rng, n, lo, hi, alphabet=b"ACGT", repeats=True):
+    seqs = []
+    alph = np.frombuffer(alphabet, np.uint8)
+    for k in range(n):
+        L = int(rng.integers(lo, hi + 1))
+        kind = k % 5
+        if kind == 0 and repeats:      # low complexity: tandem repeat
+            unit = alph[rng.integers(0, len(alph), int(rng.integers(1, 5)))]
+            s = np.resize(unit, L)
+        elif kind == 1:                # T/C rich (k = 8 hi codes)
+            s = np.frombuffer(b"TC", np.uint8)[rng.integers(0, 2, L)]
+        else:
+            s = alph[rng.integers(0, len(alph), L)]
+        seqs.append(s.tobytes())
+    # a family of near copies so that some pairs pass
+    base = seqs[2]
+    for k in range(3, n, 7):
+        t = bytearray(base[: len(seqs[k])] if len(seqs[k]) <= len(base) else base)
+        for _ in range(int(rng.integers(0, 6))):
+            t[int(rng.integers(0, len(t)))] = alph[int(rng.integers(0, len(alph)))]
+        seqs[k] = bytes(t)
+    return seqs
+
+
+@pytest.mark.parametrize("K", range(1, 9))
+def test_kmer_random_vs_oracle(ctx, K):
+    say("kmer random K", K)
+    rng = np.random.default_rng(1000 + K)
+    seqs = random_reads(rng, 333, max(K, 8), 300)
+    r = ca.Reads.from_list(seqs)
+    ro = O.Reads.from_list(seqs)
+    for thr in (0.3, 1.0):
+        ctx.upload_reads(ca.KMER, r)
+        ctx.kmer_run(K, thr)
+        p, d = ctx.kmer_fetch()
+        op, od = O.kmer_all(ro, K, thr, nthreads=1)
+        assert np.array_equal(p, op)
+        assert np.array_equal(bits(d), bits(od))
+
+
+def test_kmer_tile_ranges_partition(ctx):
+    say("kmer tile ranges")
+    seqs, _, _ = synth.make("reads1k")
+    r = ca.Reads.from_list(seqs)
+    ctx.upload_reads(ca.KMER, r)
+    ctx.kmer_run(6, 0.25)
+    fp, fd = ctx.kmer_fetch()
+    nt = ca.api.load_library().crispy_kmer_num_tiles(r.n)
+    parts = []
+    for t0, t1 in [(0, 5), (5, 17), (17, nt)]:
+        ctx.kmer_run(6, 0.25, tiles=(t0, t1))
+        parts.append(ctx.kmer_fetch())
+    p = np.concatenate([x[0] for x in parts])
+    d = np.concatenate([x[1] for x in parts])
+    assert np.array_equal(canon(p, d), canon(fp, fd))
+
+
+# ------------------------------------------------------------------- genDist
+@pytest.mark.parametrize("band", [10, 5, 2, 1])
+def test_nw_clean_sample_golden(ctx, band):
+    say("nw golden band", band)
+    gk = np.load(os.path.join(GOLD, "kmer_v2mice.npz"))
+    g = np.load(os.path.join(GOLD, "gen_v2mice.npz"))
+    r = ca.read_fasta(CLEAN, ca.NW)
+    ctx.upload_reads(ca.NW, r)
+    c = ctx.nw_run(gk["k6_pairs"], band, ca.nw_default_threshold(r.n))
+    p, d = ctx.nw_fetch()
+    assert c == len(g[f"b{band}_dists"])
+    assert np.array_equal(canon(p, d), canon(g[f"b{band}_pairs"], g[f"b{band}_dists"]))
+
+
+@pytest.mark.parametrize("band", [1, 2, 3, 5, 10, 20])
+def test_nw_all_pairs_golden(ctx, band):
+    say("nw all pairs band", band)
+    g = np.load(os.path.join(GOLD, "gen_sub60.npz"))
+    ctx.upload_reads(ca.NW, sub_reads(CLEAN, 60, ca.NW))
+    pairs = all_pairs(60)
+    d = ctx.nw_dists(pairs, band)
+    assert np.array_equal(canon(pairs, d), canon(g[f"b{band}_pairs"], g[f"b{band}_dists"]))
+
+
+@pytest.mark.parametrize("band", [1, 2, 5, 10])
+def test_nw_adversarial_golden(ctx, band):
+    say("nw adversarial band", band)
+    g = np.load(os.path.join(GOLD, "adversarial.npz"))
+    r = ca.read_fasta(ADV, ca.NW)
+    ctx.upload_reads(ca.NW, r)
+    pairs = all_pairs(r.n)
+    d = ctx.nw_dists(pairs, band)
+    assert np.array_equal(canon(pairs, d), canon(g[f"b{band}_pairs"], g[f"b{band}_dists"]))
+
+
+@pytest.mark.parametrize("band", [2, 3, 5, 7, 10, 16, 25])
+def test_nw_random_vs_oracle(ctx, band):
+    """Lengths 1..400 with every kind of length disparity, both orientations,
+    clean and non-ACGT reads: covers every register-kernel kB up to the
+    maximum and the generic kernel."""
+    say("nw random band", band)
+    rng = np.random.default_rng(2000 + band)
+    seqs = random_reads(rng, 120, 1, 400, repeats=True)
+    for k in range(0, 120, 13):  # some non-ACGT reads
+        s = bytearray(seqs[k])
+        if s:
+            s[int(rng.integers(0, len(s)))] = ord("N") if k % 2 else ord("a")
+        seqs[k] = bytes(s)
+    r = ca.Reads.from_list(seqs)
+    ro = O.Reads.from_list(seqs)
+    pairs = rng.integers(0, 120, (4000, 2)).astype(np.int32)
+    ctx.upload_reads(ca.NW, r)
+    d = ctx.nw_dists(pairs, band)
+    od, _, _ = O.nw_pairs(ro, pairs, band)
+    bad = np.nonzero(bits(d) != bits(od))[0]
+    assert len(bad) == 0, [(int(pairs[b, 0]), int(pairs[b, 1]), float(d[b]), float(od[b])) for b in bad[:5]]
+
+
+def test_nw_register_kernel_every_kb(ctx):
+    """Pairs whose kB = (L1+L2)/20 spans 0..32 at band 10 (lengths up to
+    329 each), near-identical and unrelated, against the oracle."""
+    say("nw every kB")
+    rng = np.random.default_rng(77)
+    seqs = []
+    for L in range(1, 330, 3):
+        seqs.append(rng.choice(np.frombuffer(b"ACGT", np.uint8), L).tobytes())
+    base = seqs[-1]
+    for L in range(5, 329, 11):
+        t = bytearray(base[:L])
+        for _ in range(L // 30):
+            t[int(rng.integers(0, L))] = ord("ACGT"[int(rng.integers(0, 4))])
+        seqs.append(bytes(t))
+    r = ca.Reads.from_list(seqs)
+    ro = O.Reads.from_list(seqs)
+    n = r.n
+    pairs = all_pairs(n)
+    pairs = np.concatenate([pairs, pairs[:, ::-1]])
+    ctx.upload_reads(ca.NW, r)
+    d = ctx.nw_dists(pairs, 10)
+    od, _, _ = O.nw_pairs(ro, pairs, 10)
+    assert np.array_equal(bits(d), bits(od))
+
+
+def test_sort_by_dist(ctx):
+    rng = np.random.default_rng(5)
+    n = 100000
+    p = rng.integers(0, 5000, (n, 2)).astype(np.int32)
+    d = (rng.integers(0, 50, n) / 64.0).astype(np.float32)
+    sp, sd = ctx.sort_by_dist(p, d)
+    assert np.array_equal(canon(sp, sd), canon(p, d))
+    c = canon(p, d)
+    assert np.array_equal(bits(sd).astype(np.int64), c[:, 0])
+    assert np.array_equal(sp[:, 0].astype(np.int64), c[:, 1])
+    assert np.array_equal(sp[:, 1].astype(np.int64), c[:, 2])
+
+
+# --------------------------------------------------------------- end to end
+def test_fused_reads1k_golden(ctx):
+    say("fused reads1k")
+    g = np.load(os.path.join(GOLD, "synth_reads1k.npz"))
+    seqs, _, _ = synth.make("reads1k")
+    r = ca.Reads.from_list(seqs)
+    ctx.upload_reads(ca.KMER, r)
+    ctx.upload_reads(ca.NW, r)
+    c = ctx.kmer_run(6, ca.kmer_default_threshold(r.n))
+    kp, kd = ctx.kmer_fetch()
+    assert np.array_equal(kp, g["k6_pairs"])
+    assert np.array_equal(bits(kd), bits(g["k6_dists"]))
+    ctx.nw_run_on_kmer(10, ca.nw_default_threshold(r.n))
+    p, d = ctx.nw_fetch()
+    assert int(g["b10_nfiles"]) == 1
+    assert len(d) == int(g["b10_f0_count"])
+    assert digest(p, d) == str(g["b10_f0_sha256"])
+
+
+def run(args, cwd, timeout=300):
+    r = subprocess.run(args, cwd=cwd, capture_output=True, text=True, timeout=timeout)
+    assert r.returncode == 0, (args, r.stdout[-2000:], r.stderr[-2000:])
+    return r
+
+
+def test_cli_clean_sample(tmp_path):
+    say("cli clean sample")
+    gk = np.load(os.path.join(GOLD, "kmer_v2mice.npz"))
+    g = np.load(os.path.join(GOLD, "gen_v2mice.npz"))
+    fa = tmp_path / "V2Mice_sample1000_Clean"
+    fa.write_bytes(open(CLEAN, "rb").read())
+    run([os.path.join(BIN, "kmerDist"), "-i", str(fa), "-k", "6", "-d"], tmp_path)
+    p = np.fromfile(str(fa) + ".kpair", np.int32).reshape(-1, 2)
+    d = np.fromfile(str(fa) + ".kdist", np.float32)
+    assert np.array_equal(p, gk["k6_pairs"]) and np.array_equal(bits(d), bits(gk["k6_dists"]))
+    run([os.path.join(BIN, "genDist"), "-i", str(fa), "-b", "10"], tmp_path)
+    assert not os.path.exists(str(fa) + ".npair_1")
+    np_ = np.fromfile(str(fa) + ".npair_0", np.uint32).reshape(-1, 2)
+    nd = np.fromfile(str(fa) + ".ndist_0", np.float32)
+    assert np.all(np.diff(nd) >= 0)
+    assert np.array_equal(canon(np_, nd), canon(g["b10_pairs"], g["b10_dists"]))
+
+
+def test_cli_file_split(tmp_path):
+    """719,400 input pairs: the file split follows the reference CPU path
+    (one file flushed after the first 589,824-pair chunk, one for the rest)."""
+    say("cli file split")
+    g = np.load(os.path.join(GOLD, "synth_multifile.npz"))
+    fa = str(tmp_path / "reads1200.fa")
+    synth.make("reads1k", fa, n=1200)
+    assert hashlib.sha256(open(fa, "rb").read()).hexdigest() == str(g["fasta_sha256"])
+    all_pairs(int(g["n"])).tofile(fa + ".kpair")
+    run([os.path.join(BIN, "genDist"), "-i", fa, "-b", "10", "-t", str(float(g["thr"]))], tmp_path)
+    for f in range(int(g["nfiles"])):
+        p = np.fromfile(f"{fa}.npair_{f}", np.uint32).reshape(-1, 2)
+        d = np.fromfile(f"{fa}.ndist_{f}", np.float32)
+        assert len(d) == int(g[f"f{f}_count"])
+        assert np.all(np.diff(d) >= 0)
+        assert digest(p, d) == str(g[f"f{f}_sha256"])
+    assert not os.path.exists(f"{fa}.npair_{int(g['nfiles'])}")
+
+
+def test_python_mirror_matches_cli(ctx, tmp_path):
+    say("python mirror")
+    fa = tmp_path / "c.fa"
+    fa.write_bytes(open(CLEAN, "rb").read())
+    run([os.path.join(BIN, "kmerDist"), "-i", str(fa), "-k", "6"], tmp_path)
+    run([os.path.join(BIN, "genDist"), "-i", str(fa), "-b", "10"], tmp_path)
+    r = ca.read_fasta(str(fa), ca.KMER)
+    n = ca.compute_kmer_dist(ctx, r, 6, ca.kmer_default_threshold(r.n), str(tmp_path / "py.kpair"))
+    assert open(tmp_path / "py.kpair", "rb").read() == open(str(fa) + ".kpair", "rb").read()
+    rg = ca.read_fasta(str(fa), ca.NW)
+    nf, tot = ca.compute_gen_dist(ctx, rg, str(fa) + ".kpair", str(tmp_path / "py.npair"),
+                                  str(tmp_path / "py.ndist"), ca.nw_default_threshold(rg.n), 10)
+    assert nf == 1 and tot > 0 and n > 0
+    for ext in ("npair_0", "ndist_0"):
+        assert open(tmp_path / f"py.{ext}", "rb").read() == open(f"{fa}.{ext}", "rb").read()
+
+
+def test_full_size_reads20k(ctx):
+    """The 20,000-read workload at full size: exact against the oracle on
+    a row subset, plus invariants of the whole output."""
+    say("reads20k full size")
+    seqs, _, cfg = synth.make("reads20k")
+    r = ca.Reads.from_list(seqs)
+    ro = O.Reads.from_list(seqs)
+    kthr = ca.kmer_default_threshold(r.n)
+    nthr = ca.nw_default_threshold(r.n)
+    ctx.upload_reads(ca.KMER, r)
+    ctx.upload_reads(ca.NW, r)
+    c = ctx.kmer_run(6, kthr)
+    p, d = ctx.kmer_fetch()
+    say("kmer pairs", c)
+    assert np.all(p[:, 0] < p[:, 1]) and p[:, 1].max() < r.n
+    key = p[:, 0].astype(np.int64) * r.n + p[:, 1]
+    assert np.all(np.diff(key) > 0)
+    assert all(O.passes(float(x), kthr) for x in d[:: max(1, len(d) // 5000)])
+    # rows [0, 40) and [19000, 19040) exactly
+    for r0, r1 in ((0, 40), (19000, 19040)):
+        op, od = O.kmer_all(ro, 6, kthr, nthreads=8, rows=(r0, r1))
+        sel = (p[:, 0] >= r0) & (p[:, 0] < r1)
+        assert np.array_equal(canon(p[sel], d[sel]), canon(op, od))
+    nc = ctx.nw_run_on_kmer(10, nthr)
+    np_, nd = ctx.nw_fetch()
+    say("nw pairs", nc)
+    # every kept pair came from the k-mer list and passes; a sample agrees with the oracle
+    rng = np.random.default_rng(9)
+    samp = rng.choice(len(p), size=min(3000, len(p)), replace=False)
+    od, _, _ = O.nw_pairs(ro, p[samp], 10)
+    full = ctx.nw_dists(p[samp], 10)
+    assert np.array_equal(bits(full), bits(od))
+    kept = {(int(a), int(b)): float(x) for (a, b), x in zip(np_, nd)}
+    for (a, b), x in zip(p[samp], od):
+        if O.passes(float(x), nthr):
+            assert kept[(int(a), int(b))] == float(x)
+        else:
+            assert (int(a), int(b)) not in kept
