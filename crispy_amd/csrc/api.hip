@@ -60,19 +60,37 @@ static hipEvent_t get_event(crispy_ctx *ctx) {
   return e;
 }
 
-void timer_begin(crispy_ctx *ctx, const char *, hipEvent_t *a) {
+void timer_begin(crispy_ctx *ctx, const char *, hipEvent_t *a, hipStream_t s) {
   *a = nullptr;
   if (!ctx->timing) return;
   *a = get_event(ctx);
-  if (*a) (void)hipEventRecord(*a, ctx->stream);
+  if (*a) (void)hipEventRecord(*a, s ? s : ctx->stream);
 }
 
-void timer_end(crispy_ctx *ctx, const char *name, hipEvent_t a, double units) {
+void timer_end(crispy_ctx *ctx, const char *name, hipEvent_t a, double units, hipStream_t s) {
   if (!ctx->timing || !a) return;
   hipEvent_t b = get_event(ctx);
   if (!b) return;
-  (void)hipEventRecord(b, ctx->stream);
+  (void)hipEventRecord(b, s ? s : ctx->stream);
   ctx->pending.push_back(PendingEvent{name, a, b, units});
+}
+
+// Work queued on `stream` so far happens before anything later queued on the
+// workers.
+int fork_workers(crispy_ctx *ctx) {
+  CRISPY_HIP(ctx, hipEventRecord(ctx->fork_ev, ctx->stream));
+  for (int w = 0; w < kNumWorkers; ++w) CRISPY_HIP(ctx, hipStreamWaitEvent(ctx->workers[w], ctx->fork_ev, 0));
+  return CRISPY_OK;
+}
+
+// Everything queued on the workers happens before anything later queued on
+// `stream`.
+int join_workers(crispy_ctx *ctx) {
+  for (int w = 0; w < kNumWorkers; ++w) {
+    CRISPY_HIP(ctx, hipEventRecord(ctx->join_ev[w], ctx->workers[w]));
+    CRISPY_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->join_ev[w], 0));
+  }
+  return CRISPY_OK;
 }
 
 static void drain_timers(crispy_ctx *ctx) {
@@ -156,8 +174,13 @@ int crispy_ctx_create(int device, crispy_ctx **out) {
   if (device < 0 || device >= ndev) return CRISPY_ERR_ARG;
   crispy_ctx *ctx = new crispy_ctx();
   ctx->device = device;
-  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess) {
-    delete ctx;
+  bool ok = hipSetDevice(device) == hipSuccess && hipStreamCreate(&ctx->stream) == hipSuccess &&
+            hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming) == hipSuccess;
+  for (int w = 0; ok && w < kNumWorkers; ++w)
+    ok = hipStreamCreate(&ctx->workers[w]) == hipSuccess &&
+         hipEventCreateWithFlags(&ctx->join_ev[w], hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    crispy_ctx_destroy(ctx);
     return CRISPY_ERR_HIP;
   }
   *out = ctx;
@@ -167,11 +190,13 @@ int crispy_ctx_create(int device, crispy_ctx **out) {
 void crispy_ctx_destroy(crispy_ctx *ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
-  (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (int w = 0; w < kNumWorkers; ++w)
+    if (ctx->workers[w]) (void)hipStreamSynchronize(ctx->workers[w]);
   drain_timers(ctx);
   for (auto e : ctx->pool) (void)hipEventDestroy(e);
   DevBuf *bufs[] = {&ctx->km.bitmap, &ctx->km.T, &ctx->km.hinfo, &ctx->km.rcnt, &ctx->km.roff,
-                    &ctx->km.rlist, &ctx->km.keys, &ctx->km.keys_alt, &ctx->km.dists,
+                    &ctx->km.rlist, &ctx->km.excess, &ctx->km.keys, &ctx->km.keys_alt, &ctx->km.dists,
                     &ctx->km.dists_alt, &ctx->km.pairs, &ctx->km.counter, &ctx->nw.in_pairs,
                     &ctx->nw.keys, &ctx->nw.keys_alt, &ctx->nw.idx, &ctx->nw.idx_alt,
                     &ctx->nw.dist, &ctx->nw.hist, &ctx->nw.cells, &ctx->nw.flags, &ctx->nw.sel,
@@ -182,7 +207,12 @@ void crispy_ctx_destroy(crispy_ctx *ctx) {
     DevBuf *rb[] = {&r.bases, &r.off, &r.len, &r.pack, &r.poff, &r.clean};
     for (DevBuf *b : rb) release(ctx, *b);
   }
-  (void)hipStreamDestroy(ctx->stream);
+  for (int w = 0; w < kNumWorkers; ++w) {
+    if (ctx->workers[w]) (void)hipStreamDestroy(ctx->workers[w]);
+    if (ctx->join_ev[w]) (void)hipEventDestroy(ctx->join_ev[w]);
+  }
+  if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
 

@@ -92,7 +92,7 @@ struct KmerState {
   // per-read profile for the current K
   int K = 0;
   int profile_gen = -1;
-  DevBuf bitmap, T, hinfo, rcnt, roff, rlist;
+  DevBuf bitmap, T, hinfo, rcnt, roff, rlist, excess;
   // result
   DevBuf keys, keys_alt, dists, dists_alt, pairs, counter;
   int64_t cap = 0;
@@ -119,9 +119,16 @@ struct PendingEvent {
   double units;
 };
 
+constexpr int kNumWorkers = 4;
+
 struct crispy_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // Side streams (default priority) for independent launches of one phase;
+  // forked from and joined back into `stream` with events.
+  hipStream_t workers[kNumWorkers] = {};
+  hipEvent_t fork_ev = nullptr;
+  hipEvent_t join_ev[kNumWorkers] = {};
   std::string err;
   ReadStore reads[2];
   KmerState km;
@@ -139,8 +146,10 @@ namespace crispy {
 int set_err(crispy_ctx *ctx, int code, const char *fmt, ...);
 int ensure(crispy_ctx *ctx, DevBuf &b, size_t bytes);
 void release(crispy_ctx *ctx, DevBuf &b);
-void timer_begin(crispy_ctx *ctx, const char *name, hipEvent_t *a);
-void timer_end(crispy_ctx *ctx, const char *name, hipEvent_t a, double units);
+void timer_begin(crispy_ctx *ctx, const char *name, hipEvent_t *a, hipStream_t s = nullptr);
+void timer_end(crispy_ctx *ctx, const char *name, hipEvent_t a, double units, hipStream_t s = nullptr);
+int fork_workers(crispy_ctx *ctx);
+int join_workers(crispy_ctx *ctx);
 
 }  // namespace crispy
 

@@ -44,11 +44,12 @@ __global__ __launch_bounds__(256) void kmer_profile_kernel(
     const uint32_t *__restrict__ pack, const int64_t *__restrict__ poff,
     const int32_t *__restrict__ len, uint32_t *__restrict__ bitmap, int32_t *__restrict__ Tout,
     uint32_t *__restrict__ hinfo, int32_t *__restrict__ rcnt, const int64_t *__restrict__ roff,
-    uint32_t *__restrict__ rlist) {
+    uint32_t *__restrict__ rlist, int32_t *__restrict__ excess) {
   constexpr int WP = kmer_words(K);
   __shared__ uint32_t codes[CRISPY_MAX_READ_LEN];
   __shared__ uint32_t bm[WP];
   __shared__ int scan[256];
+  __shared__ int s_distinct;
   const int r = blockIdx.x;
   const int tid = threadIdx.x;
   const int L = len[r];
@@ -69,6 +70,7 @@ __global__ __launch_bounds__(256) void kmer_profile_kernel(
     codes[t] = c;
   }
   for (int w = tid; w < WP; w += 256) bm[w] = 0;
+  if (tid == 0) s_distinct = 0;
   __syncthreads();
   for (int k = 2; k <= P; k <<= 1) {
     for (int j = k >> 1; j > 0; j >>= 1) {
@@ -89,7 +91,7 @@ __global__ __launch_bounds__(256) void kmer_profile_kernel(
   // runs: each thread owns the run starts inside its contiguous segment
   const int seg = (T + 255) / 256;
   const int lo = min(tid * seg, T), hi = min(lo + seg, T);
-  int nrep = 0;
+  int nrep = 0, nrun = 0;
   for (int x = lo; x < hi; ++x) {
     const uint32_t v = codes[x];
     if (x == 0 || codes[x - 1] != v) {
@@ -97,8 +99,10 @@ __global__ __launch_bounds__(256) void kmer_profile_kernel(
       while (x + c < T && codes[x + c] == v) ++c;
       atomicOr(&bm[v >> 5], 1u << (v & 31));
       nrep += c >= 2;
+      ++nrun;
     }
   }
+  if (nrun) atomicAdd(&s_distinct, nrun);
   scan[tid] = nrep;
   __syncthreads();
   for (int off = 1; off < 256; off <<= 1) {
@@ -120,6 +124,7 @@ __global__ __launch_bounds__(256) void kmer_profile_kernel(
   if (tid == 255) rcnt[r] = scan[255];
   if (tid == 0) {
     Tout[r] = T;
+    excess[r] = T - s_distinct;  // sum over codes of (count - 1)
     const uint32_t v = codes[T - 1];
     uint32_t h = 0;
     if (v >= 0x8000u) {
@@ -158,6 +163,14 @@ __device__ __forceinline__ void repeat_merge(const uint32_t *__restrict__ A, int
   }
 }
 
+// acc + popcount(x) as one v_bcnt_u32_b32 (the compiler otherwise splits
+// the sum into a popcount and an add tree: 3 instead of 2 ops per word).
+__device__ __forceinline__ uint32_t bcnt_acc(uint32_t x, uint32_t acc) {
+  uint32_t r;
+  asm("v_bcnt_u32_b32 %0, %1, %2" : "=v"(r) : "v"(x), "v"(acc));
+  return r;
+}
+
 // kmerMain.cpp:180 / kmerCPU.cpp:45
 __device__ __forceinline__ bool passes(float dist, float thr) {
   const float diff = dist - thr;
@@ -182,12 +195,18 @@ __device__ __forceinline__ void tile_coords(int64_t t, int nb, int &bi, int &bj)
 // word-major so each thread reads its 8 rows (or columns) with two 128-bit
 // LDS loads.  For K = 8 the hi half of the bitmap is processed first and
 // kept in a second accumulator set.
+// Epilogue: matches <= popcount + min(excess_i, excess_j) (excess = sum of
+// count-1 over a read's codes), so a pair with
+//   ub < T_min * (1 - thr - 2e-5) - 1
+// cannot pass (the margin covers the fp32 roundings); only the others pay
+// for the repeat-list merge and the division.
 template <int K>
 __global__ __launch_bounds__(256) void kmer_tile_kernel(
     const uint32_t *__restrict__ bitmap, const int32_t *__restrict__ Tn,
     const uint32_t *__restrict__ hinfo, const int32_t *__restrict__ rcnt,
-    const int64_t *__restrict__ roff, const uint32_t *__restrict__ rlist, int32_t n, int nb,
-    int64_t tile_begin, int64_t ntiles, int64_t per_xcd, float thr,
+    const int64_t *__restrict__ roff, const uint32_t *__restrict__ rlist,
+    const int32_t *__restrict__ excess, int32_t n, int nb, int64_t tile_begin, int64_t ntiles,
+    int64_t per_xcd, float thr, float cut,
     unsigned long long *__restrict__ counter, uint64_t *__restrict__ out_keys,
     float *__restrict__ out_dists, int64_t cap) {
   constexpr int WP = kmer_words(K);
@@ -199,6 +218,7 @@ __global__ __launch_bounds__(256) void kmer_tile_kernel(
   __shared__ uint32_t sH[2 * kTile];
   __shared__ int32_t sRc[2 * kTile];
   __shared__ int64_t sRo[2 * kTile];
+  __shared__ int32_t sE[2 * kTile];
 
   // Work-groups are dispatched round-robin over the 8 XCDs; give each XCD a
   // contiguous run of tiles so tiles that share a row block share an L2.
@@ -218,6 +238,7 @@ __global__ __launch_bounds__(256) void kmer_tile_kernel(
     sH[tid] = ok ? hinfo[r] : 0u;
     sRc[tid] = ok ? rcnt[r] : 0;
     sRo[tid] = ok ? roff[r] : 0;
+    sE[tid] = ok ? excess[r] : 0;
   }
 
   uint32_t acc[8][8];
@@ -274,7 +295,7 @@ __global__ __launch_bounds__(256) void kmer_tile_kernel(
 #pragma unroll
       for (int x = 0; x < 8; ++x)
 #pragma unroll
-        for (int y = 0; y < 8; ++y) acc[x][y] += __popc(a[x] & b[y]);
+        for (int y = 0; y < 8; ++y) acc[x][y] = bcnt_acc(a[x] & b[y], acc[x][y]);
     }
     if constexpr (kHi) {
       if (ch == NCH / 2 - 1) {
@@ -310,6 +331,10 @@ __global__ __launch_bounds__(256) void kmer_tile_kernel(
     for (int y = 0; y < 8; ++y) {
       const int j = col0 + (y < 4 ? tx * 4 + y : 64 + tx * 4 + (y - 4));
       if (i < j && j < n) {
+        const int li = x < 4 ? ty * 4 + x : 64 + ty * 4 + (x - 4);
+        const int lj = kTile + (y < 4 ? tx * 4 + y : 64 + tx * 4 + (y - 4));
+        const uint32_t ub = acc[x][y] + (kHi ? acch[x][y] : 0u) + (uint32_t)min(sE[li], sE[lj]);
+        if ((float)ub < (float)min(sT[li], sT[lj]) * cut - 1.0f) continue;
         const float d = dist_of(x, y, acc[x][y], kHi ? acch[x][y] : 0u);
         if (passes(d, thr)) {
           passmask |= 1ull << (x * 8 + y);
@@ -368,7 +393,7 @@ static int launch_profile(crispy_ctx *ctx, ReadStore &rs) {
   kmer_profile_kernel<K><<<rs.n, 256, 0, ctx->stream>>>(
       rs.pack.as<uint32_t>(), rs.poff.as<int64_t>(), rs.len.as<int32_t>(), km.bitmap.as<uint32_t>(),
       km.T.as<int32_t>(), km.hinfo.as<uint32_t>(), km.rcnt.as<int32_t>(), km.roff.as<int64_t>(),
-      km.rlist.as<uint32_t>());
+      km.rlist.as<uint32_t>(), km.excess.as<int32_t>());
   CRISPY_LAUNCHED(ctx);
   timer_end(ctx, "kmer_profile", ev, rs.n);
   return CRISPY_OK;
@@ -384,7 +409,8 @@ static int launch_tiles(crispy_ctx *ctx, int32_t n, int nb, int64_t tb, int64_t 
   timer_begin(ctx, "kmer_tile", &ev);
   kmer_tile_kernel<K><<<(unsigned)grid, 256, 0, ctx->stream>>>(
       km.bitmap.as<uint32_t>(), km.T.as<int32_t>(), km.hinfo.as<uint32_t>(), km.rcnt.as<int32_t>(),
-      km.roff.as<int64_t>(), km.rlist.as<uint32_t>(), n, nb, tb, nt, per_xcd, thr,
+      km.roff.as<int64_t>(), km.rlist.as<uint32_t>(), km.excess.as<int32_t>(), n, nb, tb, nt,
+      per_xcd, thr, 1.0f - thr - 2e-5f,
       km.counter.as<unsigned long long>(), km.keys.as<uint64_t>(), km.dists.as<float>(), km.cap);
   CRISPY_LAUNCHED(ctx);
   double units = 0;
@@ -419,7 +445,7 @@ static const TileFn kTiles[9] = {nullptr, launch_tiles<1>, launch_tiles<2>, laun
 static int build_profiles(crispy_ctx *ctx, int K) {
   ReadStore &rs = ctx->reads[CRISPY_READS_KMER];
   KmerState &km = ctx->km;
-  if (km.K == K && km.profile_gen == rs.generation) return CRISPY_OK;
+  // Rebuilt on every run: the per-read profiles are part of the stage's work.
   const int32_t n = rs.n;
   std::vector<int64_t> roff(n + 1, 0);
   for (int32_t r = 0; r < n; ++r) {
@@ -433,6 +459,7 @@ static int build_profiles(crispy_ctx *ctx, int K) {
   CRISPY_TRY(ensure(ctx, km.T, sizeof(int32_t) * (n + 1)));
   CRISPY_TRY(ensure(ctx, km.hinfo, sizeof(uint32_t) * (n + 1)));
   CRISPY_TRY(ensure(ctx, km.rcnt, sizeof(int32_t) * (n + 1)));
+  CRISPY_TRY(ensure(ctx, km.excess, sizeof(int32_t) * (n + 1)));
   CRISPY_TRY(ensure(ctx, km.roff, sizeof(int64_t) * (n + 1)));
   CRISPY_TRY(ensure(ctx, km.rlist, sizeof(uint32_t) * (roff[n] + 1)));
   CRISPY_HIP(ctx, hipMemcpyAsync(km.roff.p, roff.data(), sizeof(int64_t) * (n + 1),

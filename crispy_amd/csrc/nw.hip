@@ -48,11 +48,21 @@ const int2 *kmer_result_pairs(crispy_ctx *ctx);
 constexpr uint32_t kGenericKey = 0xFFFFFFFFu;
 
 // ---------------------------------------------------------------- bucketing
-__global__ void nw_bucket_kernel(const int2 *__restrict__ pairs, int64_t np,
-                                 const int32_t *__restrict__ len, const uint8_t *__restrict__ clean,
-                                 int band, uint32_t *__restrict__ keys, uint32_t *__restrict__ idx,
-                                 unsigned int *__restrict__ hist,
-                                 unsigned long long *__restrict__ cells) {
+// Sort key per pair: (kB << 24 | L2 << 12 | L1) for the register kernel,
+// all ones for the generic kernel.  Per-bucket pair and cell counts are
+// reduced in LDS first (a handful of buckets would serialise global atomics).
+__global__ __launch_bounds__(256) void nw_bucket_kernel(
+    const int2 *__restrict__ pairs, int64_t np, const int32_t *__restrict__ len,
+    const uint8_t *__restrict__ clean, int band, uint32_t *__restrict__ keys,
+    uint32_t *__restrict__ idx, unsigned int *__restrict__ hist,
+    unsigned long long *__restrict__ cells) {
+  __shared__ unsigned int s_hist[kNumBuckets];
+  __shared__ unsigned long long s_cells[kNumBuckets];
+  for (int b = threadIdx.x; b < kNumBuckets; b += blockDim.x) {
+    s_hist[b] = 0;
+    s_cells[b] = 0;
+  }
+  __syncthreads();
   for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np;
        p += (int64_t)gridDim.x * blockDim.x) {
     const int2 pr = pairs[p];
@@ -72,8 +82,15 @@ __global__ void nw_bucket_kernel(const int2 *__restrict__ pairs, int64_t np,
     }
     keys[p] = key;
     idx[p] = (uint32_t)p;
-    atomicAdd(&hist[bucket], 1u);
-    atomicAdd(&cells[bucket], (unsigned long long)nc);
+    atomicAdd(&s_hist[bucket], 1u);
+    atomicAdd(&s_cells[bucket], (unsigned long long)nc);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < kNumBuckets; b += blockDim.x) {
+    if (s_hist[b]) {
+      atomicAdd(&hist[b], s_hist[b]);
+      atomicAdd(&cells[b], s_cells[b]);
+    }
   }
 }
 
@@ -117,10 +134,20 @@ __device__ __forceinline__ void nw_row(const int i, int (&Mt)[2 * KB + 1], int (
     const uint32_t lad = LA[d] + (s << 16);
     const uint32_t lal = d < W - 1 ? LA[d + 1] + 1u : 0u;
     const uint32_t la = cd ? lad : (cu ? LAu : lal);
-    Mh = M + (cd ? kGO : (cu ? kGE : 0));
+    // pen_h (low half) and 4 + pen_v (high half) of the winner in one select;
+    // the sign-extending half reads fold into SDWA operands of the adds.
+    constexpr uint32_t kPenDiag = ((uint32_t)(4 + kGO) << 16) | (uint16_t)kGO;
+    constexpr uint32_t kPenUp = ((uint32_t)4 << 16) | (uint16_t)kGE;
+    constexpr uint32_t kPenLeft = ((uint32_t)(4 + kGE) << 16) | 0u;
+    const uint32_t pk = cd ? kPenDiag : (cu ? kPenUp : kPenLeft);
+    Mh = M + (int)(int16_t)(pk & 0xFFFFu);
     LAu = la;
     Mt[d] = M;
-    Mv[d] = M + (cd ? 4 + kGO : (cu ? 4 : 4 + kGE));
+    Mv[d] = M + ((int)pk >> 16);
+    // Materialise Mv here: left to itself the compiler sinks this select
+    // into the next row and keeps both comparison masks of every cell alive
+    // (SGPR pairs, spilled to VGPR lanes).
+    asm volatile("" : "+v"(Mv[d]));
     LA[d] = la;
     if (MODE != kRowFull) {
       const bool c = d == dl;
@@ -405,13 +432,13 @@ static inline int grid_for(int64_t n) { return (int)std::max<int64_t>(1, std::mi
 
 // ------------------------------------------------------------ host driver
 template <int KB>
-static void launch_reg(crispy_ctx *ctx, const ReadStore &rs, const int2 *pairs, const uint32_t *order,
+static void launch_reg(hipStream_t s, const ReadStore &rs, const int2 *pairs, const uint32_t *order,
                        int32_t count, float *out) {
-  nw_band_reg_kernel<KB><<<(count + 63) / 64, 64, 0, ctx->stream>>>(
+  nw_band_reg_kernel<KB><<<(count + 63) / 64, 64, 0, s>>>(
       rs.pack.as<uint32_t>(), rs.poff.as<int64_t>(), rs.len.as<int32_t>(), pairs, order, count, out);
 }
 
-using RegFn = void (*)(crispy_ctx *, const ReadStore &, const int2 *, const uint32_t *, int32_t, float *);
+using RegFn = void (*)(hipStream_t, const ReadStore &, const int2 *, const uint32_t *, int32_t, float *);
 
 template <int... KBs>
 struct RegTable {
@@ -445,7 +472,7 @@ static int nw_core(crispy_ctx *ctx, const int2 *d_pairs, int64_t np, int band) {
   CRISPY_HIP(ctx, hipMemsetAsync(nw.cells.p, 0, sizeof(unsigned long long) * kNumBuckets, ctx->stream));
   hipEvent_t ev;
   timer_begin(ctx, "nw_bucket", &ev);
-  nw_bucket_kernel<<<grid_for(np), 256, 0, ctx->stream>>>(
+  nw_bucket_kernel<<<(int)std::min<int64_t>((np + 255) / 256, 1024), 256, 0, ctx->stream>>>(
       d_pairs, np, rs.len.as<int32_t>(), rs.clean.as<uint8_t>(), band, nw.keys.as<uint32_t>(),
       nw.idx.as<uint32_t>(), nw.hist.as<unsigned int>(), nw.cells.as<unsigned long long>());
   CRISPY_LAUNCHED(ctx);
@@ -465,28 +492,45 @@ static int nw_core(crispy_ctx *ctx, const int2 *d_pairs, int64_t np, int band) {
   CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
   const uint32_t *order = nw.idx_alt.as<uint32_t>();
   float *out = nw.dist.as<float>();
+  // Buckets are independent: fan them out over the worker streams, largest
+  // first, so small buckets do not leave the chip idle between launches.
+  int64_t start[kNumBuckets];
   int64_t pos = 0;
-  for (int kb = 0; kb <= kKbRegMax; ++kb) {
-    if (!hist[kb]) continue;
-    timer_begin(ctx, "nw_band_reg", &ev);
-    RegFns::fns[kb](ctx, rs, d_pairs, order + pos, (int32_t)hist[kb], out);
-    CRISPY_LAUNCHED(ctx);
-    timer_end(ctx, "nw_band_reg", ev, (double)cells[kb]);
-    pos += hist[kb];
+  std::vector<int> todo;
+  double total_cells = 0;
+  for (int b = 0; b < kNumBuckets; ++b) {
+    start[b] = pos;
+    pos += hist[b];
+    if (hist[b]) todo.push_back(b);
+    total_cells += (double)cells[b];
   }
+  std::sort(todo.begin(), todo.end(), [&](int a, int b) { return cells[a] > cells[b]; });
   const int64_t ng = hist[kGenericBucket];
-  if (ng > 0) {
-    const int64_t lanes = std::min<int64_t>(ng, 32768);
-    const int blocks = (int)((lanes + 255) / 256);
-    const int64_t stride = (int64_t)blocks * 256;
-    CRISPY_TRY(ensure(ctx, nw.scratch, sizeof(int2) * (size_t)stride * (rs.maxlen + 1)));
-    timer_begin(ctx, "nw_generic", &ev);
-    nw_generic_kernel<<<blocks, 256, 0, ctx->stream>>>(
-        rs.bases.as<uint8_t>(), rs.off.as<int64_t>(), rs.len.as<int32_t>(), d_pairs, order + pos,
-        (int32_t)ng, band, nw.scratch.as<int2>(), out);
-    CRISPY_LAUNCHED(ctx);
-    timer_end(ctx, "nw_generic", ev, (double)cells[kGenericBucket]);
+  const int gblocks = (int)((std::min<int64_t>(ng, 32768) + 255) / 256);
+  if (ng > 0)  // before the fork: ensure() may free and reallocate
+    CRISPY_TRY(ensure(ctx, nw.scratch, sizeof(int2) * (size_t)gblocks * 256 * (rs.maxlen + 1)));
+  hipEvent_t phase;
+  timer_begin(ctx, "nw_dp_phase", &phase);
+  CRISPY_TRY(fork_workers(ctx));
+  int wk = 0;
+  for (int b : todo) {
+    hipStream_t s = ctx->workers[wk++ % kNumWorkers];
+    if (b == kGenericBucket) {
+      timer_begin(ctx, "nw_generic", &ev, s);
+      nw_generic_kernel<<<gblocks, 256, 0, s>>>(rs.bases.as<uint8_t>(), rs.off.as<int64_t>(),
+                                               rs.len.as<int32_t>(), d_pairs, order + start[b],
+                                               (int32_t)ng, band, nw.scratch.as<int2>(), out);
+      CRISPY_LAUNCHED(ctx);
+      timer_end(ctx, "nw_generic", ev, (double)cells[b], s);
+    } else {
+      timer_begin(ctx, "nw_band_reg", &ev, s);
+      RegFns::fns[b](s, rs, d_pairs, order + start[b], (int32_t)hist[b], out);
+      CRISPY_LAUNCHED(ctx);
+      timer_end(ctx, "nw_band_reg", ev, (double)cells[b], s);
+    }
   }
+  CRISPY_TRY(join_workers(ctx));
+  timer_end(ctx, "nw_dp_phase", phase, total_cells);
   return CRISPY_OK;
 }
 
