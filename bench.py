@@ -95,6 +95,18 @@ def band_cells_np(L1, L2, kb):
     return total
 
 
+def measured_traffic(workload):
+    """HBM bytes per nw_band_reg dispatch from the committed PMC summary of
+    this workload (two rocprofv3 --pmc runs, scaled by a copy kernel of known
+    size), or None when there is none."""
+    import re
+    path = os.path.join(ROOT, "profiles", f"kmer_nw_step_{workload}_traffic.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"nw_band_reg traffic bytes per dispatch: (\d+)", open(path).read())
+    return int(m.group(1)) if m else None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,7 +115,20 @@ def main():
     ap.add_argument("--workload", default="reads20k")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-budget", type=float, default=24.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a 1 GiB read + 1 GiB write elementwise kernel (3x), to scale "
+                         "the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(1 << 30, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.add(a, 1, out=b)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": 1 << 30, "bytes_written": 1 << 30,
+                          "dispatches": 3}))
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py measures one GPU: the path shards by k-mer tile range with no "
@@ -169,7 +194,9 @@ def main():
                     "frac": ach / VALU_PEAK_TOPS,
                     "work": f"{NW_OPS_PER_CELL} int ops x in-band DP cells ({cells:.4e}/step)",
                     "phase_ms": phase_ms, "launches_per_step": kern["nw_band_reg"]["launches"] / args.steps,
-                    "avg_launch_ms": kern["nw_band_reg"]["avg_ms"], "traffic": None}
+                    "avg_launch_ms": kern["nw_band_reg"]["avg_ms"],
+                    "traffic": measured_traffic(args.workload),
+                    "traffic_source": f"profiles/kmer_nw_step_{args.workload}_traffic.md"}
     else:
         roofline = {"kernel": "kmer_tile", "bound": "valu", "achieved": kmer_tile_tops,
                     "peak": VALU_PEAK_TOPS, "unit": "Tops/s (int32 lane-ops)",
