@@ -27,7 +27,7 @@ METRIC = "k-mer pairs/s + banded-NW GCUPS, 1 GPU; served jobs/s with p50/p99 lat
 VALU_PEAK_TOPS = 39.3    # 256 CU x 64 lanes x 2.4 GHz int32 lane-ops/s
 NW_OPS_PER_CELL = 13     # nominal int ops per in-band DP cell
 KMER_KERNELS = ("kmer_profile", "kmer_tile", "kmer_sort")
-NW_KERNELS = ("nw_bucket", "nw_band_reg", "nw_generic", "nw_filter")
+NW_KERNELS = ("nw_bucket", "nw_band_reg", "nw_band_sliced", "nw_generic", "nw_filter")
 # the NW launches of one step run concurrently on side streams; this event
 # pair on the main stream brackets all of them (fork to join)
 NW_PHASE = "nw_dp_phase"

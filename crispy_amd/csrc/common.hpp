@@ -23,12 +23,15 @@ constexpr int kMatch = 5;
 constexpr int kNeg = -1000000;
 
 // Largest band half-width kB = (L1+L2)/(2 band) handled by the register
-// resident NW kernel (one template instance per kB).  Pairs above it, full
-// alignments (band <= 1) and pairs with non-ACGT bytes run the generic
-// kernel.
+// resident NW kernel (one template instance per kB).  Full alignments
+// (band <= 1), pairs with non-ACGT bytes and kB > kKbSliceMax run the
+// generic kernel.
 constexpr int kKbRegMax = 32;
-constexpr int kNumBuckets = kKbRegMax + 2;  // [0, kKbRegMax] + generic
-constexpr int kGenericBucket = kKbRegMax + 1;
+// Wider bands up to kKbSliceMax run the lane-sliced kernel (W = 2kB+1 band
+// cells split over up to 64 lanes of 24 cells each).
+constexpr int kKbSliceMax = (64 * 24 - 1) / 2;
+constexpr int kNumBuckets = kKbSliceMax + 2;  // [0, kKbSliceMax] + generic
+constexpr int kGenericBucket = kKbSliceMax + 1;
 
 // k-mer tile geometry.
 constexpr int kTile = 128;        // reads per tile side

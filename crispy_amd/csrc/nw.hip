@@ -48,8 +48,8 @@ const int2 *kmer_result_pairs(crispy_ctx *ctx);
 constexpr uint32_t kGenericKey = 0xFFFFFFFFu;
 
 // ---------------------------------------------------------------- bucketing
-// Sort key per pair: (kB << 24 | L2 << 12 | L1) for the register kernel,
-// all ones for the generic kernel.  Per-bucket pair and cell counts are
+// Sort key per pair: (kB << 22 | L2 << 11 | L1), lengths clamped to 2047,
+// for the register and sliced kernels; all ones for the generic kernel.  Per-bucket pair and cell counts are
 // reduced in LDS first (a handful of buckets would serialise global atomics).
 __global__ __launch_bounds__(256) void nw_bucket_kernel(
     const int2 *__restrict__ pairs, int64_t np, const int32_t *__restrict__ len,
@@ -73,8 +73,8 @@ __global__ __launch_bounds__(256) void nw_bucket_kernel(
     if (band >= 2) {
       const int kb = (L1 + L2) / (2 * band);
       nc = band_cells(L1, L2, kb);
-      if (kb <= kKbRegMax && clean[pr.x] && clean[pr.y]) {
-        key = ((uint32_t)kb << 24) | ((uint32_t)L2 << 12) | (uint32_t)L1;
+      if (kb <= kKbSliceMax && clean[pr.x] && clean[pr.y]) {
+        key = ((uint32_t)kb << 22) | ((uint32_t)min(L2, 2047) << 11) | (uint32_t)min(L1, 2047);
         bucket = kb;
       }
     } else {
@@ -197,14 +197,17 @@ __global__ __launch_bounds__(64, nw_reg_waves<KB>()) void nw_band_reg_kernel(
   const uint32_t *s2 = pack + poff[pr.y];
   const int nw1 = (L1 + 15) >> 4, nw2 = (L2 + 15) >> 4;
 
-  // wave-uniform row bounds (pairs are sorted by (kB, L2, L1))
-  const int last = min(base + 63, count - 1);
-  const int L2max = __builtin_amdgcn_readfirstlane(len[pairs[order[last]].y]);
-  const int L2min = __builtin_amdgcn_readfirstlane(len[pairs[order[base]].y]);
-  int l1m = L1;
+  // wave-uniform row bounds (pairs are sorted by (kB, L2, L1), so these are close)
+  int l1m = L1, l2lo = L2, l2hi = L2;
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) l1m = min(l1m, __shfl_xor(l1m, off, 64));
+  for (int off = 32; off > 0; off >>= 1) {
+    l1m = min(l1m, __shfl_xor(l1m, off, 64));
+    l2lo = min(l2lo, __shfl_xor(l2lo, off, 64));
+    l2hi = max(l2hi, __shfl_xor(l2hi, off, 64));
+  }
   const int L1min = __builtin_amdgcn_readfirstlane(l1m);
+  const int L2min = __builtin_amdgcn_readfirstlane(l2lo);
+  const int L2max = __builtin_amdgcn_readfirstlane(l2hi);
 
   int Mt[W], Mv[W];
   uint32_t LA[W];
@@ -297,6 +300,243 @@ __global__ __launch_bounds__(64, nw_reg_waves<KB>()) void nw_band_reg_kernel(
   const int AL = (int)(la & 0xFFFFu) + j;
   const int ML = AL - (int)(la >> 16);
   if (active) out[pidx] = (float)ML / (float)AL;
+}
+
+// -------------------------------------------------------- sliced kernel
+// Wide bands (kB > kKbRegMax): G = ceil(W / 24) lanes per pair, lane g owns
+// band positions [24g, 24g + 24) of every row in VGPRs (same per-cell state
+// and arithmetic as nw_band_reg_kernel).  Lane g processes row t - g at step
+// t, one row behind lane g - 1, so per step it needs only
+//   * "up" for its first cell: lane g-1's last cell of the same row, sent at
+//     the end of the previous step, and
+//   * "left" for its last cell: lane g+1's first cell of the previous row,
+//     which lane g+1 computes first in this step and sends mid-step.
+// Positions >= W in the last lane are dead; the last real cell (KEND in the
+// last lane, KEND = (W - 1) mod 24) sees an out-of-band left neighbour.
+template <int KEND, int MODE>
+__device__ __forceinline__ void nw_slice_cell(const int k, const int i, const uint32_t s,
+                                              int &Mt_k, int &Mv_k, uint32_t &LA_k,
+                                              const int leftM, const uint32_t leftLA, int &Mh,
+                                              uint32_t &LAu, const int k0, const int kc,
+                                              int &capM, uint32_t &capLA) {
+  const int diag = Mt_k + (int)(9u * s);
+  int M = max(max(diag, Mh), leftM);
+  const bool cd = M == diag;
+  const bool cu = M == Mh;
+  uint32_t la = cd ? LA_k + (s << 16) : (cu ? LAu : leftLA + 1u);
+  constexpr uint32_t kPenDiag = ((uint32_t)(4 + kGO) << 16) | (uint16_t)kGO;
+  constexpr uint32_t kPenUp = ((uint32_t)4 << 16) | (uint16_t)kGE;
+  constexpr uint32_t kPenLeft = ((uint32_t)(4 + kGE) << 16) | 0u;
+  const uint32_t pk = cd ? kPenDiag : (cu ? kPenUp : kPenLeft);
+  int mh = M + (int)(int16_t)(pk & 0xFFFFu);
+  int mv = M + ((int)pk >> 16);
+  if (MODE == kRowTop) {  // boundary cell (i, 0): M 0, ML 0, AL i
+    const bool b = k == k0;
+    M = b ? 4 * i : M;
+    mv = b ? 4 * i + 4 : mv;
+    la = b ? (((uint32_t)i << 16) | (uint32_t)i) : la;
+    mh = b ? 4 * i : mh;
+  }
+  Mt_k = M;
+  Mv_k = mv;
+  asm volatile("" : "+v"(Mv_k));
+  LA_k = la;
+  Mh = mh;
+  LAu = la;
+  if (MODE != kRowFull) {
+    const bool c = k == kc;
+    capM = c ? M : capM;
+    capLA = c ? la : capLA;
+  }
+}
+
+constexpr int kSliceCells = 24;
+
+template <int KEND>
+__global__ __launch_bounds__(64) void nw_band_sliced_kernel(
+    const uint32_t *__restrict__ pack, const int64_t *__restrict__ poff,
+    const int32_t *__restrict__ len, const int2 *__restrict__ pairs,
+    const uint32_t *__restrict__ order, int32_t count, int KB, int G, float *__restrict__ out) {
+  constexpr int C = kSliceCells;
+  constexpr int NMW = (C + 15) / 16;
+  constexpr int NB = NMW + 1;
+  __shared__ int red[64][6];
+  const int W = 2 * KB + 1;
+  const int lane = threadIdx.x;
+  const int groups = 64 / G;
+  const int grp = lane / G;
+  const int g = lane - grp * G;
+  const int slot = blockIdx.x * groups + grp;
+  const bool has_pair = grp < groups && slot < count;
+  const uint32_t pidx = order[has_pair ? slot : count - 1];
+  const int2 pr = pairs[pidx];
+  const int L1 = len[pr.x];
+  const int L2 = has_pair ? len[pr.y] : 0;
+  const uint32_t *s1 = pack + poff[pr.x];
+  const uint32_t *s2 = pack + poff[pr.y];
+  const int nw1 = (L1 + 15) >> 4;
+  const bool is_end = g == G - 1;
+  const int src_up = g > 0 ? lane - 1 : lane;
+  const int src_left = g < G - 1 ? lane + 1 : lane;
+  const int base_d = g * C;
+
+  int tm = has_pair ? L2 + g : 0;
+  int l1m = has_pair ? L1 : 0x7fffffff;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    tm = max(tm, __shfl_xor(tm, off, 64));
+    l1m = min(l1m, __shfl_xor(l1m, off, 64));
+  }
+  const int tmax = __builtin_amdgcn_readfirstlane(tm);
+  const int L1min = __builtin_amdgcn_readfirstlane(l1m);
+
+  int Mt[C], Mv[C];
+  uint32_t LA[C];
+#pragma unroll
+  for (int k = 0; k < C; ++k) {  // row 0 within the band; dead cells out of band
+    const int j = base_d + k - KB;
+    const bool live = !(is_end && k > KEND) && j >= 0;
+    Mt[k] = live ? 0 : kNeg;
+    Mv[k] = live ? 4 : kNeg;
+    LA[k] = live ? ((uint32_t)j << 16) : 0u;
+  }
+  // read-1 window: cell k of lane g at step t is base t - g - kB - 1 + 24g + k
+  const int p0 = base_d - g - KB;
+  uint32_t B[NB];
+#pragma unroll
+  for (int k = 0; k < NB; ++k) B[k] = get16(s1, p0 + 16 * k, nw1);
+  int qn = p0 + 16 * NB;
+  uint32_t F = get16(s1, qn, nw1);
+  int fcnt = 16;
+
+  int upM = kNeg;
+  uint32_t upLA = 0;
+  int bIM = kNeg, bIj = 0, bJM = kNeg, bJr = 0;
+  uint32_t bILA = 0, bJLA = 0;
+
+  auto step = [&](const int t, auto mode_tag) {
+    constexpr int MODE = decltype(mode_tag)::value;
+    const int i = t - g;
+    const bool active = has_pair && i >= 1 && i <= L2;
+    const int im1 = i - 1;
+    const uint32_t w2 = (im1 >= 0 && im1 < L2) ? s2[im1 >> 4] : 0u;
+    const uint32_t bb = ((w2 >> (2 * (im1 & 15))) & 3u) * 0x55555555u;
+    uint32_t m[NMW];
+#pragma unroll
+    for (int k = 0; k < NMW; ++k) {
+      const uint32_t x = B[k] ^ bb;
+      m[k] = ~(x | (x >> 1)) & 0x55555555u;
+    }
+    const int k0 = KB - i - base_d;      // the boundary cell (i, 0) in this lane
+    const int kc = L1 - i + KB - base_d;  // the last-column cell (i, L1) in this lane
+    int capM = kNeg;
+    uint32_t capLA = 0;
+    int Mh = upM;
+    uint32_t LAu = upLA;
+    if (active) {
+      const bool last0 = KEND == 0 && is_end;
+      nw_slice_cell<KEND, MODE>(0, i, m[0] & 1u, Mt[0], Mv[0], LA[0], last0 ? kNeg : Mv[1], LA[1],
+                                Mh, LAu, k0, kc, capM, capLA);
+    }
+    // lane g+1's first cell of row i-1 is this lane's "left" for its last cell
+    const int xM = __shfl(Mv[0], src_left, 64);
+    const uint32_t xLA = (uint32_t)__shfl((int)LA[0], src_left, 64);
+    if (active) {
+#pragma unroll
+      for (int k = 1; k < C; ++k) {
+        const uint32_t s = (m[k >> 4] >> (2 * (k & 15))) & 1u;
+        int lM;
+        uint32_t lLA;
+        if (k == C - 1) {
+          lM = is_end ? kNeg : xM;
+          lLA = xLA;
+        } else {
+          lM = (k == KEND && is_end) ? kNeg : Mv[k + 1];
+          lLA = LA[k + 1];
+        }
+        nw_slice_cell<KEND, MODE>(k, i, s, Mt[k], Mv[k], LA[k], lM, lLA, Mh, LAu, k0, kc, capM,
+                                  capLA);
+      }
+      if (MODE != kRowFull && kc >= 0 && kc < C && i >= L1 - KB) {
+        const int Mr = capM - 4 * i;
+        if (Mr >= bJM) {
+          bJM = Mr;
+          bJr = i;
+          bJLA = capLA;
+        }
+      }
+      if (i == L2) {
+#pragma unroll
+        for (int k = 0; k < C; ++k) {
+          const int j = i - KB + base_d + k;
+          if (j >= 1 && j <= L1 && base_d + k < W) {
+            const int Mr = Mt[k] - 4 * i;
+            if (Mr >= bIM) {
+              bIM = Mr;
+              bIj = j;
+              bILA = LA[k];
+            }
+          }
+        }
+      }
+    }
+    // this lane's last cell of row i is lane g+1's "up" at the next step
+    const int nM = __shfl(Mh, src_up, 64);
+    const uint32_t nLA = (uint32_t)__shfl((int)LAu, src_up, 64);
+    upM = g > 0 ? nM : kNeg;
+    upLA = nLA;
+#pragma unroll
+    for (int k = 0; k < NB - 1; ++k) B[k] = __builtin_amdgcn_alignbit(B[k + 1], B[k], 2);
+    B[NB - 1] = __builtin_amdgcn_alignbit(F, B[NB - 1], 2);
+    F >>= 2;
+    if (--fcnt == 0) {
+      qn += 16;
+      F = get16(s1, qn, nw1);
+      fcnt = 16;
+    }
+  };
+  const int top_end = min(KB + G - 1, tmax);
+  const int mid_end = max(top_end, min(tmax, L1min - KB - 1));
+  int t = 1;
+  for (; t <= top_end; ++t) step(t, std::integral_constant<int, kRowTop>{});
+  for (; t <= mid_end; ++t) step(t, std::integral_constant<int, kRowFull>{});
+  for (; t <= tmax; ++t) step(t, std::integral_constant<int, kRowCapture>{});
+
+  red[lane][0] = bIM;
+  red[lane][1] = bIj;
+  red[lane][2] = (int)bILA;
+  red[lane][3] = bJM;
+  red[lane][4] = bJr;
+  red[lane][5] = (int)bJLA;
+  __syncthreads();
+  if (has_pair && g == 0) {
+    int iM = kNeg, ij = 0, jM = kNeg, jr = 0;
+    uint32_t iLA = 0, jLA = 0;
+    for (int q = lane; q < lane + G; ++q) {  // last maximum: larger j (row) wins ties
+      if (red[q][0] > iM || (red[q][0] == iM && red[q][1] > ij)) {
+        iM = red[q][0];
+        ij = red[q][1];
+        iLA = (uint32_t)red[q][2];
+      }
+      if (red[q][3] > jM || (red[q][3] == jM && red[q][4] > jr)) {
+        jM = red[q][3];
+        jr = red[q][4];
+        jLA = (uint32_t)red[q][5];
+      }
+    }
+    uint32_t la;
+    int j;
+    if (iM > jM) {
+      la = iLA;
+      j = ij;
+    } else {
+      la = jLA;
+      j = L1;
+    }
+    const int AL = (int)(la & 0xFFFFu) + j;
+    const int ML = AL - (int)(la >> 16);
+    out[pidx] = (float)ML / (float)AL;
+  }
 }
 
 // --------------------------------------------------------- generic kernel
@@ -455,6 +695,25 @@ struct MakeTable<0, KBs...> {
 };
 using RegFns = MakeTable<kKbRegMax + 1>::type;
 
+template <int KEND>
+static void launch_sliced(hipStream_t s, const ReadStore &rs, const int2 *pairs, const uint32_t *order,
+                          int32_t count, int kb, float *out) {
+  const int W = 2 * kb + 1;
+  const int G = (W + kSliceCells - 1) / kSliceCells;
+  const int groups = 64 / G;
+  nw_band_sliced_kernel<KEND><<<(count + groups - 1) / groups, 64, 0, s>>>(
+      rs.pack.as<uint32_t>(), rs.poff.as<int64_t>(), rs.len.as<int32_t>(), pairs, order, count, kb, G,
+      out);
+}
+
+using SliceFn = void (*)(hipStream_t, const ReadStore &, const int2 *, const uint32_t *, int32_t, int,
+                         float *);
+// KEND = (W - 1) mod 24 = 2kB mod 24 is even
+static const SliceFn kSliceFns[kSliceCells / 2] = {
+    launch_sliced<0>,  launch_sliced<2>,  launch_sliced<4>,  launch_sliced<6>,
+    launch_sliced<8>,  launch_sliced<10>, launch_sliced<12>, launch_sliced<14>,
+    launch_sliced<16>, launch_sliced<18>, launch_sliced<20>, launch_sliced<22>};
+
 // Distances of np device pairs into nw.dist (indexed like the pairs).
 static int nw_core(crispy_ctx *ctx, const int2 *d_pairs, int64_t np, int band) {
   ReadStore &rs = ctx->reads[CRISPY_READS_NW];
@@ -522,11 +781,16 @@ static int nw_core(crispy_ctx *ctx, const int2 *d_pairs, int64_t np, int band) {
                                                (int32_t)ng, band, nw.scratch.as<int2>(), out);
       CRISPY_LAUNCHED(ctx);
       timer_end(ctx, "nw_generic", ev, (double)cells[b], s);
-    } else {
+    } else if (b <= kKbRegMax) {
       timer_begin(ctx, "nw_band_reg", &ev, s);
       RegFns::fns[b](s, rs, d_pairs, order + start[b], (int32_t)hist[b], out);
       CRISPY_LAUNCHED(ctx);
       timer_end(ctx, "nw_band_reg", ev, (double)cells[b], s);
+    } else {
+      timer_begin(ctx, "nw_band_sliced", &ev, s);
+      kSliceFns[((2 * b) % kSliceCells) / 2](s, rs, d_pairs, order + start[b], (int32_t)hist[b], b, out);
+      CRISPY_LAUNCHED(ctx);
+      timer_end(ctx, "nw_band_sliced", ev, (double)cells[b], s);
     }
   }
   CRISPY_TRY(join_workers(ctx));

@@ -254,6 +254,38 @@ def test_nw_register_kernel_every_kb(ctx):
     assert np.array_equal(bits(d), bits(od))
 
 
+def test_nw_sliced_kernel_shapes(ctx):
+    """Band 2 and 3 on reads of 60..2048 bp: kB from 33 to beyond 767 covers
+    every lane-sliced instance (kB mod 12), group sizes 3..64 lanes, and the
+    generic kernel above; near copies and unrelated reads, both orders."""
+    say("nw sliced shapes")
+    rng = np.random.default_rng(31)
+    alph = np.frombuffer(b"ACGT", np.uint8)
+    base = rng.choice(alph, 2048).tobytes()
+    seqs = []
+    for L in list(range(60, 2049, 53)) + [2048, 1535, 1534, 767, 768]:
+        t = bytearray(base[:L])
+        for _ in range(L // 25):
+            t[int(rng.integers(0, L))] = int(alph[int(rng.integers(0, 4))])
+        if int(rng.integers(0, 2)):
+            del t[int(rng.integers(0, L)):int(rng.integers(0, L)) + 3]
+        seqs.append(bytes(t) if len(t) > 0 else b"ACGT")
+    for L in range(100, 2049, 211):
+        seqs.append(rng.choice(alph, L).tobytes())
+    r = ca.Reads.from_list(seqs)
+    ro = O.Reads.from_list(seqs)
+    pairs = all_pairs(r.n)
+    sel = rng.choice(len(pairs), size=min(700, len(pairs)), replace=False)
+    pairs = np.concatenate([pairs[sel], pairs[sel[:200], ::-1]])
+    ctx.upload_reads(ca.NW, r)
+    for band in (2, 3):
+        d = ctx.nw_dists(pairs, band)
+        od, _, _ = O.nw_pairs(ro, pairs, band)
+        bad = np.nonzero(bits(d) != bits(od))[0]
+        assert len(bad) == 0, (band, [(int(pairs[b, 0]), int(pairs[b, 1]), float(d[b]), float(od[b]))
+                                      for b in bad[:5]])
+
+
 def test_sort_by_dist(ctx):
     rng = np.random.default_rng(5)
     n = 100000
