@@ -26,7 +26,7 @@ sys.path.insert(0, ROOT)
 METRIC = "k-mer pairs/s + banded-NW GCUPS, 1 GPU; served jobs/s with p50/p99 latency"
 VALU_PEAK_TOPS = 39.3    # 256 CU x 64 lanes x 2.4 GHz int32 lane-ops/s
 NW_OPS_PER_CELL = 13     # nominal int ops per in-band DP cell
-KMER_KERNELS = ("kmer_profile", "kmer_tile", "kmer_sort")
+KMER_KERNELS = ("kmer_profile", "kmer_tile", "kmer_cand", "kmer_sort")
 NW_KERNELS = ("nw_bucket", "nw_band_reg", "nw_band_sliced", "nw_generic", "nw_filter")
 # the NW launches of one step run concurrently on side streams; this event
 # pair on the main stream brackets all of them (fork to join)
@@ -113,6 +113,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="reads20k")
+    ap.add_argument("--k", type=int, default=None, help="override the workload's k")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-budget", type=float, default=24.0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -143,7 +144,7 @@ def main():
     seqs, _, cfg = synth.make(args.workload)
     reads = ca.Reads.from_list(seqs)
     n = reads.n
-    K, band = cfg["K"], cfg["band"]
+    K, band = (args.k or cfg["K"]), cfg["band"]
     kthr, nthr = ca.kmer_default_threshold(n), ca.nw_default_threshold(n)
     log(f"workload {args.workload}: n={n} K={K} band={band} kthr={kthr:.4f} nthr={nthr:.4f} "
         f"(generated in {time.perf_counter() - t0:.1f}s)")

@@ -196,7 +196,7 @@ void crispy_ctx_destroy(crispy_ctx *ctx) {
   drain_timers(ctx);
   for (auto e : ctx->pool) (void)hipEventDestroy(e);
   DevBuf *bufs[] = {&ctx->km.bitmap, &ctx->km.T, &ctx->km.hinfo, &ctx->km.rcnt, &ctx->km.roff,
-                    &ctx->km.rlist, &ctx->km.excess, &ctx->km.keys, &ctx->km.keys_alt, &ctx->km.dists,
+                    &ctx->km.rlist, &ctx->km.excess, &ctx->km.loff, &ctx->km.lists, &ctx->km.cand, &ctx->km.keys, &ctx->km.keys_alt, &ctx->km.dists,
                     &ctx->km.dists_alt, &ctx->km.pairs, &ctx->km.counter, &ctx->nw.in_pairs,
                     &ctx->nw.keys, &ctx->nw.keys_alt, &ctx->nw.idx, &ctx->nw.idx_alt,
                     &ctx->nw.dist, &ctx->nw.hist, &ctx->nw.cells, &ctx->nw.flags, &ctx->nw.sel,

@@ -96,6 +96,10 @@ struct KmerState {
   int K = 0;
   int profile_gen = -1;
   DevBuf bitmap, T, hinfo, rcnt, roff, rlist, excess;
+  // K = 7, 8: signed-sorted K-mer lists and the k = 6 candidate pairs
+  DevBuf loff, lists, cand;
+  int64_t cand_cap = 0;
+  int64_t candidates = 0;
   // result
   DevBuf keys, keys_alt, dists, dists_alt, pairs, counter;
   int64_t cap = 0;

@@ -200,13 +200,13 @@ __device__ __forceinline__ void tile_coords(int64_t t, int nb, int &bi, int &bj)
 //   ub < T_min * (1 - thr - 2e-5) - 1
 // cannot pass (the margin covers the fp32 roundings); only the others pay
 // for the repeat-list merge and the division.
-template <int K>
+template <int K, bool CAND>
 __global__ __launch_bounds__(256) void kmer_tile_kernel(
     const uint32_t *__restrict__ bitmap, const int32_t *__restrict__ Tn,
     const uint32_t *__restrict__ hinfo, const int32_t *__restrict__ rcnt,
     const int64_t *__restrict__ roff, const uint32_t *__restrict__ rlist,
     const int32_t *__restrict__ excess, int32_t n, int nb, int64_t tile_begin, int64_t ntiles,
-    int64_t per_xcd, float thr, float cut,
+    int64_t per_xcd, float thr, float cut, int tshift,
     unsigned long long *__restrict__ counter, uint64_t *__restrict__ out_keys,
     float *__restrict__ out_dists, int64_t cap) {
   constexpr int WP = kmer_words(K);
@@ -334,9 +334,8 @@ __global__ __launch_bounds__(256) void kmer_tile_kernel(
         const int li = x < 4 ? ty * 4 + x : 64 + ty * 4 + (x - 4);
         const int lj = kTile + (y < 4 ? tx * 4 + y : 64 + tx * 4 + (y - 4));
         const uint32_t ub = acc[x][y] + (kHi ? acch[x][y] : 0u) + (uint32_t)min(sE[li], sE[lj]);
-        if ((float)ub < (float)min(sT[li], sT[lj]) * cut - 1.0f) continue;
-        const float d = dist_of(x, y, acc[x][y], kHi ? acch[x][y] : 0u);
-        if (passes(d, thr)) {
+        if ((float)ub < (float)(min(sT[li], sT[lj]) - tshift) * cut - 1.0f) continue;
+        if (CAND || passes(dist_of(x, y, acc[x][y], kHi ? acch[x][y] : 0u), thr)) {
           passmask |= 1ull << (x * 8 + y);
           ++mycount;
         }
@@ -368,7 +367,7 @@ __global__ __launch_bounds__(256) void kmer_tile_kernel(
         const int j = col0 + (y < 4 ? tx * 4 + y : 64 + tx * 4 + (y - 4));
         if (pos < cap) {
           out_keys[pos] = ((uint64_t)i << 20) | (uint64_t)j;
-          out_dists[pos] = dist_of(x, y, acc[x][y], kHi ? acch[x][y] : 0u);
+          if (!CAND) out_dists[pos] = dist_of(x, y, acc[x][y], kHi ? acch[x][y] : 0u);
         }
         ++pos;
       }
@@ -381,6 +380,112 @@ __global__ void unpack_keys_kernel(const uint64_t *__restrict__ keys, int64_t n,
        k += (int64_t)gridDim.x * blockDim.x) {
     const uint64_t v = keys[k];
     pairs[k] = make_int2((int)(v >> 20), (int)(v & 0xFFFFFu));
+  }
+}
+
+// ------------------------------------------------------ K = 7, 8 (exact)
+// Every shared K-mer occurrence (K > 6) starts with a shared 6-mer at the same
+// position, so matches_K <= I_K <= I_6 <= popcount_6 + min(excess_6): the
+// k = 6 bitmap tiles, bounded against T_K = T_6 - (K - 6) and the K
+// threshold, give a candidate superset (kmer_tile_kernel<6, true>).  The
+// candidates then run the reference's merge (kmerCPU.cpp:29-43) over their
+// K-mer lists sorted as signed short (kmerMain.cpp:156-159) -- its K = 8
+// ordering included -- for the exact count.
+
+// One block per read: the T = L-K+1 codes in signed-short order.
+template <int K>
+__global__ __launch_bounds__(256) void kmer_list_kernel(const uint32_t *__restrict__ pack,
+                                                        const int64_t *__restrict__ poff,
+                                                        const int32_t *__restrict__ len,
+                                                        const int64_t *__restrict__ loff,
+                                                        uint16_t *__restrict__ lists) {
+  __shared__ uint32_t keys[CRISPY_MAX_READ_LEN];
+  const int r = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int T = len[r] - K + 1;
+  const uint32_t *s = pack + poff[r];
+  int P = 1;
+  while (P < T) P <<= 1;
+  for (int t = tid; t < P; t += 256) {
+    uint32_t c = 0xFFFFFFFFu;
+    if (t < T) {
+      c = 0;
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        const int p = t + q;
+        c = (c << 2) | ((s[p >> 4] >> (2 * (p & 15))) & 3u);
+      }
+      c ^= 0x8000u;  // signed-short order of the 16-bit code = unsigned order of code ^ 0x8000
+    }
+    keys[t] = c;
+  }
+  __syncthreads();
+  for (int k = 2; k <= P; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int t = tid; t < P; t += 256) {
+        const int o = t ^ j;
+        if (o > t) {
+          const uint32_t a = keys[t], b = keys[o];
+          const bool up = (t & k) == 0;
+          if ((a > b) == up) {
+            keys[t] = b;
+            keys[o] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  uint16_t *dst = lists + loff[r];
+  for (int t = tid; t < T; t += 256) dst[t] = (uint16_t)(keys[t] ^ 0x8000u);
+}
+
+__global__ __launch_bounds__(256) void kmer_cand_kernel(
+    const uint64_t *__restrict__ cand, int64_t ncand, const uint16_t *__restrict__ lists,
+    const int64_t *__restrict__ loff, int K, const int32_t *__restrict__ len, float thr,
+    unsigned long long *__restrict__ counter, uint64_t *__restrict__ out_keys,
+    float *__restrict__ out_dists, int64_t cap) {
+  const int lane = threadIdx.x & 63;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  // every lane of a wave runs the same number of outer iterations (shuffles)
+  const int64_t base0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63);
+  for (int64_t b = base0; b < ncand; b += stride) {
+    const int64_t c = b + lane;
+    bool pass = false;
+    float dist = 0.f;
+    uint64_t key = 0;
+    if (c < ncand) {
+      key = cand[c];
+      const int i = (int)(key >> 20), j = (int)(key & 0xFFFFFu);
+      const uint16_t *A = lists + loff[i];
+      const uint16_t *Bl = lists + loff[j];
+      const unsigned short l1 = (unsigned short)(len[i] - K + 1), l2 = (unsigned short)(len[j] - K + 1);
+      unsigned short matches = 0;
+      int k = 0, l = 0;
+      while (k < l1 && l < l2) {
+        const unsigned short x = A[k], y = Bl[l];
+        matches = (unsigned short)(matches + (x == y));
+        k += x <= y;
+        l += x >= y;
+      }
+      const unsigned short mn = l1 < l2 ? l1 : l2;
+      dist = 1.0f - (float)matches / (float)mn;
+      pass = passes(dist, thr);
+    }
+    const uint64_t ball = __ballot(pass);
+    const int total = __popcll(ball);
+    unsigned long long basep = 0;
+    if (total > 0) {
+      if (lane == 0) basep = atomicAdd(counter, (unsigned long long)total);
+      basep = __shfl(basep, 0, 64);
+    }
+    if (pass) {
+      const int64_t pos = (int64_t)basep + __popcll(ball & ((1ull << lane) - 1));
+      if (pos < cap) {
+        out_keys[pos] = key;
+        out_dists[pos] = dist;
+      }
+    }
   }
 }
 
@@ -399,23 +504,22 @@ static int launch_profile(crispy_ctx *ctx, ReadStore &rs) {
   return CRISPY_OK;
 }
 
-template <int K>
-static int launch_tiles(crispy_ctx *ctx, int32_t n, int nb, int64_t tb, int64_t nt, float thr) {
+template <int K, bool CAND>
+static int launch_tiles(crispy_ctx *ctx, int32_t n, int nb, int64_t tb, int64_t nt, float thr,
+                        int tshift, uint64_t *out_keys, float *out_dists, int64_t cap) {
   KmerState &km = ctx->km;
   const int64_t per_xcd = (nt + 7) / 8;
   const int64_t grid = per_xcd * 8;
-  // pairs evaluated (i < j) in the tile range, for the timing units
   hipEvent_t ev;
   timer_begin(ctx, "kmer_tile", &ev);
-  kmer_tile_kernel<K><<<(unsigned)grid, 256, 0, ctx->stream>>>(
+  kmer_tile_kernel<K, CAND><<<(unsigned)grid, 256, 0, ctx->stream>>>(
       km.bitmap.as<uint32_t>(), km.T.as<int32_t>(), km.hinfo.as<uint32_t>(), km.rcnt.as<int32_t>(),
       km.roff.as<int64_t>(), km.rlist.as<uint32_t>(), km.excess.as<int32_t>(), n, nb, tb, nt,
-      per_xcd, thr, 1.0f - thr - 2e-5f,
-      km.counter.as<unsigned long long>(), km.keys.as<uint64_t>(), km.dists.as<float>(), km.cap);
+      per_xcd, thr, 1.0f - thr - 2e-5f, tshift, km.counter.as<unsigned long long>(), out_keys,
+      out_dists, cap);
   CRISPY_LAUNCHED(ctx);
-  double units = 0;
+  double units = 0;  // read pairs i < j < n inside the selected tiles
   if (ctx->timing) {
-    // exact count of the i < j < n pairs inside the selected tiles
     int64_t rs = 0;
     for (int bi = 0; bi < nb; ++bi) {
       const int64_t re = rs + (nb - bi);
@@ -434,27 +538,33 @@ static int launch_tiles(crispy_ctx *ctx, int32_t n, int nb, int64_t tb, int64_t 
 }
 
 using ProfileFn = int (*)(crispy_ctx *, ReadStore &);
-using TileFn = int (*)(crispy_ctx *, int32_t, int, int64_t, int64_t, float);
-static const ProfileFn kProfile[9] = {nullptr, launch_profile<1>, launch_profile<2>,
+using TileFn = int (*)(crispy_ctx *, int32_t, int, int64_t, int64_t, float, int, uint64_t *, float *,
+                       int64_t);
+static const ProfileFn kProfile[7] = {nullptr,           launch_profile<1>, launch_profile<2>,
                                       launch_profile<3>, launch_profile<4>, launch_profile<5>,
-                                      launch_profile<6>, launch_profile<7>, launch_profile<8>};
-static const TileFn kTiles[9] = {nullptr, launch_tiles<1>, launch_tiles<2>, launch_tiles<3>,
-                                 launch_tiles<4>, launch_tiles<5>, launch_tiles<6>, launch_tiles<7>,
-                                 launch_tiles<8>};
+                                      launch_profile<6>};
+static const TileFn kTiles[7] = {nullptr,
+                                 launch_tiles<1, false>,
+                                 launch_tiles<2, false>,
+                                 launch_tiles<3, false>,
+                                 launch_tiles<4, false>,
+                                 launch_tiles<5, false>,
+                                 launch_tiles<6, false>};
 
-static int build_profiles(crispy_ctx *ctx, int K) {
+// Profiles of the bitmap word size KP (KP = K for K <= 6, 6 above); every
+// read must hold at least one K-mer.
+static int build_profiles(crispy_ctx *ctx, int KP, int K) {
   ReadStore &rs = ctx->reads[CRISPY_READS_KMER];
   KmerState &km = ctx->km;
   // Rebuilt on every run: the per-read profiles are part of the stage's work.
   const int32_t n = rs.n;
   std::vector<int64_t> roff(n + 1, 0);
   for (int32_t r = 0; r < n; ++r) {
-    const int T = rs.h_len[r] - K + 1;
-    if (T < 1)
+    if (rs.h_len[r] - K + 1 < 1)
       return set_err(ctx, CRISPY_ERR_ARG, "read %d is shorter than K=%d (length %d)", r, K, rs.h_len[r]);
-    roff[r + 1] = roff[r] + T / 2 + 1;
+    roff[r + 1] = roff[r] + (rs.h_len[r] - KP + 1) / 2 + 1;
   }
-  const int WP = kmer_words(K);
+  const int WP = kmer_words(KP);
   CRISPY_TRY(ensure(ctx, km.bitmap, sizeof(uint32_t) * (size_t)n * WP));
   CRISPY_TRY(ensure(ctx, km.T, sizeof(int32_t) * (n + 1)));
   CRISPY_TRY(ensure(ctx, km.hinfo, sizeof(uint32_t) * (n + 1)));
@@ -464,11 +574,54 @@ static int build_profiles(crispy_ctx *ctx, int K) {
   CRISPY_TRY(ensure(ctx, km.rlist, sizeof(uint32_t) * (roff[n] + 1)));
   CRISPY_HIP(ctx, hipMemcpyAsync(km.roff.p, roff.data(), sizeof(int64_t) * (n + 1),
                                  hipMemcpyHostToDevice, ctx->stream));
-  CRISPY_TRY(kProfile[K](ctx, rs));
+  CRISPY_TRY(kProfile[KP](ctx, rs));
+  if (K > 6) {  // the K-mer lists for the exact merge of the candidates
+    std::vector<int64_t> loff(n + 1, 0);
+    for (int32_t r = 0; r < n; ++r) loff[r + 1] = loff[r] + (rs.h_len[r] - K + 1);
+    CRISPY_TRY(ensure(ctx, km.loff, sizeof(int64_t) * (n + 1)));
+    CRISPY_TRY(ensure(ctx, km.lists, sizeof(uint16_t) * (loff[n] + 8)));
+    CRISPY_HIP(ctx, hipMemcpyAsync(km.loff.p, loff.data(), sizeof(int64_t) * (n + 1),
+                                   hipMemcpyHostToDevice, ctx->stream));
+    hipEvent_t ev;
+    timer_begin(ctx, "kmer_profile", &ev);
+    if (K == 7)
+      kmer_list_kernel<7><<<n, 256, 0, ctx->stream>>>(rs.pack.as<uint32_t>(), rs.poff.as<int64_t>(),
+                                                     rs.len.as<int32_t>(), km.loff.as<int64_t>(),
+                                                     km.lists.as<uint16_t>());
+    else
+      kmer_list_kernel<8><<<n, 256, 0, ctx->stream>>>(rs.pack.as<uint32_t>(), rs.poff.as<int64_t>(),
+                                                     rs.len.as<int32_t>(), km.loff.as<int64_t>(),
+                                                     km.lists.as<uint16_t>());
+    CRISPY_LAUNCHED(ctx);
+    timer_end(ctx, "kmer_profile", ev, n);
+    CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));  // loff is a local
+  }
   // roff must stay alive until the copy has run
   CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
   km.K = K;
   km.profile_gen = rs.generation;
+  return CRISPY_OK;
+}
+
+// Runs `launch` (which appends to km.counter) and grows the output once if
+// it overflowed `cap`; returns the final count.
+template <class F>
+static int run_counted(crispy_ctx *ctx, int64_t &cap, DevBuf &keys, DevBuf *dists, int64_t *count,
+                       F &&launch) {
+  KmerState &km = ctx->km;
+  CRISPY_TRY(ensure(ctx, km.counter, sizeof(unsigned long long)));
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    CRISPY_TRY(ensure(ctx, keys, sizeof(uint64_t) * cap));
+    if (dists) CRISPY_TRY(ensure(ctx, *dists, sizeof(float) * cap));
+    CRISPY_HIP(ctx, hipMemsetAsync(km.counter.p, 0, sizeof(unsigned long long), ctx->stream));
+    CRISPY_TRY(launch());
+    unsigned long long c = 0;
+    CRISPY_HIP(ctx, hipMemcpyAsync(&c, km.counter.p, sizeof c, hipMemcpyDeviceToHost, ctx->stream));
+    CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *count = (int64_t)c;
+    if (*count <= cap) break;
+    cap = *count + *count / 8 + 1024;  // rerun with room for every entry
+  }
   return CRISPY_OK;
 }
 
@@ -531,20 +684,35 @@ int crispy_kmer_run(crispy_ctx *ctx, int K, float threshold, int64_t tile_begin,
     if (count) *count = 0;
     return CRISPY_OK;
   }
-  CRISPY_TRY(build_profiles(ctx, K));
+  const int KP = K <= 6 ? K : 6;
+  CRISPY_TRY(build_profiles(ctx, KP, K));
   if (km.cap == 0) km.cap = (int64_t)1 << 22;
-  CRISPY_TRY(ensure(ctx, km.counter, sizeof(unsigned long long)));
-  for (int attempt = 0; attempt < 2; ++attempt) {
-    CRISPY_TRY(ensure(ctx, km.keys, sizeof(uint64_t) * km.cap));
-    CRISPY_TRY(ensure(ctx, km.dists, sizeof(float) * km.cap));
-    CRISPY_HIP(ctx, hipMemsetAsync(km.counter.p, 0, sizeof(unsigned long long), ctx->stream));
-    CRISPY_TRY(kTiles[K](ctx, n, nb, tile_begin, nt, threshold));
-    unsigned long long c = 0;
-    CRISPY_HIP(ctx, hipMemcpyAsync(&c, km.counter.p, sizeof c, hipMemcpyDeviceToHost, ctx->stream));
-    CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    km.count = (int64_t)c;
-    if (km.count <= km.cap) break;
-    km.cap = km.count + km.count / 8 + 1024;  // rerun with room for every passing pair
+  if (K <= 6) {
+    CRISPY_TRY(run_counted(ctx, km.cap, km.keys, &km.dists, &km.count, [&]() {
+      return kTiles[K](ctx, n, nb, tile_begin, nt, threshold, 0, km.keys.as<uint64_t>(),
+                       km.dists.as<float>(), km.cap);
+    }));
+  } else {
+    if (km.cand_cap == 0) km.cand_cap = (int64_t)1 << 22;
+    int64_t ncand = 0;
+    CRISPY_TRY(run_counted(ctx, km.cand_cap, km.cand, nullptr, &ncand, [&]() {
+      return launch_tiles<6, true>(ctx, n, nb, tile_begin, nt, threshold, K - 6,
+                                   km.cand.as<uint64_t>(), nullptr, km.cand_cap);
+    }));
+    km.candidates = ncand;
+    CRISPY_TRY(run_counted(ctx, km.cap, km.keys, &km.dists, &km.count, [&]() -> int {
+      if (ncand == 0) return CRISPY_OK;
+      hipEvent_t ev;
+      timer_begin(ctx, "kmer_cand", &ev);
+      const int blocks = (int)std::min<int64_t>((ncand + 255) / 256, 8192);
+      kmer_cand_kernel<<<blocks, 256, 0, ctx->stream>>>(
+          km.cand.as<uint64_t>(), ncand, km.lists.as<uint16_t>(), km.loff.as<int64_t>(), K,
+          rs.len.as<int32_t>(), threshold, km.counter.as<unsigned long long>(), km.keys.as<uint64_t>(),
+          km.dists.as<float>(), km.cap);
+      CRISPY_LAUNCHED(ctx);
+      timer_end(ctx, "kmer_cand", ev, (double)ncand);
+      return CRISPY_OK;
+    }));
   }
   CRISPY_TRY(ensure(ctx, km.keys_alt, sizeof(uint64_t) * (km.count + 1)));
   CRISPY_TRY(ensure(ctx, km.dists_alt, sizeof(float) * (km.count + 1)));
