@@ -47,6 +47,7 @@ def load_library():
         "crispy_ctx_error": ([_vp], ctypes.c_char_p),
         "crispy_ctx_sync": ([_vp], ctypes.c_int),
         "crispy_ctx_device_bytes": ([_vp], ctypes.c_int64),
+        "crispy_ctx_trim": ([_vp], ctypes.c_int),
         "crispy_read_fasta": ([ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(_vp),
                                ctypes.POINTER(_vp), _i32p], ctypes.c_int32),
         "crispy_free": ([_vp], None),
@@ -70,6 +71,21 @@ def load_library():
         "crispy_timing_reset": ([_vp], ctypes.c_int),
         "crispy_timing_query": ([_vp, ctypes.c_char_p, _f64p, _i64p, _f64p], ctypes.c_int),
         "crispy_selftest_division": ([_vp, ctypes.c_int32, _f32p, _f32p], ctypes.c_int),
+        "crispy_server_create": ([ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.POINTER(_vp)],
+                                 ctypes.c_int),
+        "crispy_server_destroy": ([_vp], None),
+        "crispy_server_job_bytes": ([ctypes.c_char_p, _i64p, ctypes.c_int32, ctypes.c_int,
+                                     ctypes.c_int], ctypes.c_int64),
+        "crispy_server_submit": ([_vp, ctypes.c_char_p, _i64p, ctypes.c_char_p, _i64p,
+                                  ctypes.c_int32, ctypes.c_int, ctypes.c_float, ctypes.c_int,
+                                  ctypes.c_float], ctypes.c_int64),
+        "crispy_server_wait": ([_vp, ctypes.c_int64, _i64p, _i64p, _f64p, _f64p], ctypes.c_int),
+        "crispy_server_job_times": ([_vp, ctypes.c_int64, _f64p, _f64p, _f64p, _i64p, _i64p],
+                                    ctypes.c_int),
+        "crispy_server_job_error": ([_vp, ctypes.c_int64], ctypes.c_char_p),
+        "crispy_server_fetch": ([_vp, ctypes.c_int64, _i32p, _f32p], ctypes.c_int),
+        "crispy_server_status": ([_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                  _i64p, _i64p], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -249,8 +265,91 @@ class Context:
     def device_bytes(self):
         return int(self._L.crispy_ctx_device_bytes(self._h))
 
+    def trim(self):
+        self._check(self._L.crispy_ctx_trim(self._h))
+
     def selftest_division(self, maxv):
         a = np.zeros((maxv + 1) * (maxv + 1), np.float32)
         b = np.zeros_like(a)
         self._check(self._L.crispy_selftest_division(self._h, maxv, _p(a, _f32p), _p(b, _f32p)))
         return a.reshape(maxv + 1, maxv + 1), b.reshape(maxv + 1, maxv + 1)
+
+
+def job_bytes(reads: Reads, K, band):
+    """Device bytes the server charges a job at admission."""
+    return int(load_library().crispy_server_job_bytes(reads.bases, _p(reads.offsets, _i64p),
+                                                      reads.n, K, band))
+
+
+class Server:
+    """Job server: `slots` contexts behind a FIFO queue with HBM-budget
+    admission; one job = k-mer filter + banded NW on the passing pairs."""
+
+    def __init__(self, device=0, slots=4, hbm_budget=64 << 30):
+        self._L = load_library()
+        h = _vp()
+        rc = self._L.crispy_server_create(device, slots, int(hbm_budget), ctypes.byref(h))
+        if rc != 0:
+            raise CrispyError(rc, f"cannot start a server on HIP device {device}")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.crispy_server_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, reads: Reads, K, kmer_threshold, band, nw_threshold, nw_reads=None):
+        nb = nw_reads.bases if nw_reads is not None else None
+        no = _p(nw_reads.offsets, _i64p) if nw_reads is not None else None
+        jid = self._L.crispy_server_submit(self._h, reads.bases, _p(reads.offsets, _i64p), nb, no,
+                                           reads.n, K, kmer_threshold, band, nw_threshold)
+        if jid < 0:
+            raise CrispyError(int(jid), "job rejected")
+        return int(jid)
+
+    def _job_check(self, jid, rc):
+        if rc != 0:
+            msg = self._L.crispy_server_job_error(self._h, jid)
+            raise CrispyError(rc, (msg or b"").decode(errors="replace") or f"job {jid} failed")
+
+    def wait(self, jid):
+        """Blocks until job `jid` is done; returns a dict of its counts and times (ms)."""
+        kp, npairs = ctypes.c_int64(), ctypes.c_int64()
+        q, r = ctypes.c_double(), ctypes.c_double()
+        rc = self._L.crispy_server_wait(self._h, jid, ctypes.byref(kp), ctypes.byref(npairs),
+                                        ctypes.byref(q), ctypes.byref(r))
+        self._job_check(jid, rc)
+        s, b, e = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        est, held = ctypes.c_int64(), ctypes.c_int64()
+        self._job_check(jid, self._L.crispy_server_job_times(
+            self._h, jid, ctypes.byref(s), ctypes.byref(b), ctypes.byref(e), ctypes.byref(est),
+            ctypes.byref(held)))
+        return dict(kmer_pairs=kp.value, nw_pairs=npairs.value, queue_ms=q.value, run_ms=r.value,
+                    submit_ms=s.value, start_ms=b.value, end_ms=e.value, est_bytes=est.value,
+                    held_bytes=held.value)
+
+    def fetch(self, jid):
+        """Waits for job `jid`, returns (pairs (n, 2) int32, dists float32) and forgets it."""
+        info = self.wait(jid)
+        n = info["nw_pairs"]
+        p = np.zeros((n, 2), np.int32)
+        d = np.zeros(n, np.float32)
+        self._job_check(jid, self._L.crispy_server_fetch(self._h, jid, _p(p, _i32p), _p(d, _f32p)))
+        return p, d
+
+    def status(self):
+        r, q, b, pk = ctypes.c_int(), ctypes.c_int(), ctypes.c_int64(), ctypes.c_int64()
+        rc = self._L.crispy_server_status(self._h, ctypes.byref(r), ctypes.byref(q), ctypes.byref(b),
+                                          ctypes.byref(pk))
+        if rc != 0:
+            raise CrispyError(rc, "status")
+        return dict(running=r.value, queued=q.value, reserved_bytes=b.value,
+                    peak_reserved_bytes=pk.value)

@@ -1,6 +1,7 @@
 """Builds the gfx950 library and the drop-in executables in-tree:
 
-  crispy_amd/lib/libcrispy_hip.so   HIP kernels + the C ABI (include/crispy_hip.h)
+  crispy_amd/lib/libcrispy_hip.so   HIP kernels + the C ABI (include/crispy_hip.h),
+                                    including the job server
   bin/kmerDist, bin/genDist         the reference's command lines over that ABI
 
 Usage: python -m crispy_amd.build   (or crispy_amd.build.build()).
@@ -25,7 +26,7 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["api.hip", "kmer.hip", "nw.hip"]
-HOST_SOURCES = ["host.cpp"]
+HOST_SOURCES = ["host.cpp", "serve.cpp"]
 CLI = {"kmerDist": "cli_kmer.cpp", "genDist": "cli_gen.cpp"}
 
 HIP_FLAGS = [
@@ -60,14 +61,14 @@ def build(verbose=False, jobs=None, save_temps=False):
         objs.append(obj)
     for src in HOST_SOURCES:
         obj = os.path.join(OBJDIR, src + ".o")
-        cmds.append(["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-I", INC, "-c",
+        cmds.append(["g++", "-O2", "-std=c++17", "-fPIC", "-pthread", "-Wall", "-I", INC, "-c",
                      os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     jobs = jobs or min(8, os.cpu_count() or 1)
     with ThreadPoolExecutor(jobs) as ex:
         list(ex.map(lambda c: _run(c, verbose), cmds))
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs,
+    _run([HIPCC, "-shared", "-pthread", f"--offload-arch={ARCH}", "-o", tmp, *objs,
           f"-Wl,-rpath,{ROCM}/lib"], verbose)
     os.replace(tmp, LIB)
     for exe, src in CLI.items():

@@ -187,26 +187,35 @@ int crispy_ctx_create(int device, crispy_ctx **out) {
   return CRISPY_OK;
 }
 
-void crispy_ctx_destroy(crispy_ctx *ctx) {
-  if (!ctx) return;
-  (void)hipSetDevice(ctx->device);
-  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  for (int w = 0; w < kNumWorkers; ++w)
-    if (ctx->workers[w]) (void)hipStreamSynchronize(ctx->workers[w]);
-  drain_timers(ctx);
-  for (auto e : ctx->pool) (void)hipEventDestroy(e);
-  DevBuf *bufs[] = {&ctx->km.bitmap, &ctx->km.T, &ctx->km.hinfo, &ctx->km.rcnt, &ctx->km.roff,
-                    &ctx->km.rlist, &ctx->km.excess, &ctx->km.loff, &ctx->km.lists, &ctx->km.cand, &ctx->km.keys, &ctx->km.keys_alt, &ctx->km.dists,
-                    &ctx->km.dists_alt, &ctx->km.pairs, &ctx->km.counter, &ctx->nw.in_pairs,
-                    &ctx->nw.keys, &ctx->nw.keys_alt, &ctx->nw.idx, &ctx->nw.idx_alt,
-                    &ctx->nw.dist, &ctx->nw.hist, &ctx->nw.cells, &ctx->nw.flags, &ctx->nw.sel,
-                    &ctx->nw.nsel, &ctx->nw.out_pairs, &ctx->nw.out_dists, &ctx->nw.scratch,
-                    &ctx->temp};
+static void release_all(crispy_ctx *ctx) {
+  DevBuf *bufs[] = {&ctx->km.bitmap,   &ctx->km.T,         &ctx->km.hinfo,     &ctx->km.rcnt,
+                    &ctx->km.roff,     &ctx->km.rlist,     &ctx->km.excess,    &ctx->km.loff,
+                    &ctx->km.lists,    &ctx->km.cand,      &ctx->km.keys,      &ctx->km.keys_alt,
+                    &ctx->km.dists,    &ctx->km.dists_alt, &ctx->km.pairs,     &ctx->km.counter,
+                    &ctx->nw.in_pairs, &ctx->nw.keys,      &ctx->nw.keys_alt,  &ctx->nw.idx,
+                    &ctx->nw.idx_alt,  &ctx->nw.dist,      &ctx->nw.hist,      &ctx->nw.cells,
+                    &ctx->nw.flags,    &ctx->nw.sel,       &ctx->nw.nsel,      &ctx->nw.out_pairs,
+                    &ctx->nw.out_dists, &ctx->nw.scratch,  &ctx->temp};
   for (DevBuf *b : bufs) release(ctx, *b);
   for (auto &r : ctx->reads) {
     DevBuf *rb[] = {&r.bases, &r.off, &r.len, &r.pack, &r.poff, &r.clean};
     for (DevBuf *b : rb) release(ctx, *b);
   }
+}
+
+static void sync_all(crispy_ctx *ctx) {
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (int w = 0; w < kNumWorkers; ++w)
+    if (ctx->workers[w]) (void)hipStreamSynchronize(ctx->workers[w]);
+}
+
+void crispy_ctx_destroy(crispy_ctx *ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  sync_all(ctx);
+  drain_timers(ctx);
+  for (auto e : ctx->pool) (void)hipEventDestroy(e);
+  release_all(ctx);
   for (int w = 0; w < kNumWorkers; ++w) {
     if (ctx->workers[w]) (void)hipStreamDestroy(ctx->workers[w]);
     if (ctx->join_ev[w]) (void)hipEventDestroy(ctx->join_ev[w]);
@@ -214,6 +223,21 @@ void crispy_ctx_destroy(crispy_ctx *ctx) {
   if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
+}
+
+int crispy_ctx_trim(crispy_ctx *ctx) {
+  if (!ctx) return CRISPY_ERR_ARG;
+  CRISPY_HIP(ctx, hipSetDevice(ctx->device));
+  sync_all(ctx);
+  release_all(ctx);
+  for (auto &r : ctx->reads) {
+    const int g = r.generation;
+    r = ReadStore();
+    r.generation = g + 1;
+  }
+  ctx->km = KmerState();
+  ctx->nw = NwState();
+  return CRISPY_OK;
 }
 
 const char *crispy_ctx_error(const crispy_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }

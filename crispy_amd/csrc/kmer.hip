@@ -686,14 +686,16 @@ int crispy_kmer_run(crispy_ctx *ctx, int K, float threshold, int64_t tile_begin,
   }
   const int KP = K <= 6 ? K : 6;
   CRISPY_TRY(build_profiles(ctx, KP, K));
-  if (km.cap == 0) km.cap = (int64_t)1 << 22;
+  // first guess at the result size: the pairs in the range, at most 4 Mi
+  const int64_t guess = std::min<int64_t>((int64_t)1 << 22, std::max<int64_t>(4096, nt * kTile * kTile));
+  km.cap = std::max(km.cap, guess);
   if (K <= 6) {
     CRISPY_TRY(run_counted(ctx, km.cap, km.keys, &km.dists, &km.count, [&]() {
       return kTiles[K](ctx, n, nb, tile_begin, nt, threshold, 0, km.keys.as<uint64_t>(),
                        km.dists.as<float>(), km.cap);
     }));
   } else {
-    if (km.cand_cap == 0) km.cand_cap = (int64_t)1 << 22;
+    km.cand_cap = std::max(km.cand_cap, guess);
     int64_t ncand = 0;
     CRISPY_TRY(run_counted(ctx, km.cand_cap, km.cand, nullptr, &ncand, [&]() {
       return launch_tiles<6, true>(ctx, n, nb, tile_begin, nt, threshold, K - 6,

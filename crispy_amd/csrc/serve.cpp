@@ -1,0 +1,332 @@
+// Served form: a job queue in front of `slots` worker threads, each owning a
+// context (its own streams and resident device buffers).  A job (one read
+// set through kmerDist -> genDist) is admitted when a slot is free and its
+// device-memory estimate fits the HBM budget next to the running jobs; jobs
+// admitted while others run overlap on the GPU.  Results are kept on the host
+// until fetched.  Host code only: it drives the library through its C ABI.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "crispy_hip.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Job {
+  int64_t id = 0;
+  std::string kbases, nbases;
+  std::vector<int64_t> koff, noff;
+  bool same_reads = true;
+  int32_t n = 0;
+  int K = 6, band = 10;
+  float kthr = 0.f, nthr = 0.f;
+  int64_t est = 0;
+  int64_t held = 0;  // device bytes the slot's context held after the job
+  bool done = false;
+  int status = CRISPY_OK;
+  std::string error;
+  int64_t kmer_pairs = 0;
+  std::vector<int32_t> pairs;
+  std::vector<float> dists;
+  Clock::time_point t_submit, t_start, t_end;
+};
+
+}  // namespace
+
+struct crispy_server {
+  int device = 0;
+  int64_t budget = 0;
+  // Device bytes charged to the slots: an idle slot is charged what its
+  // context holds, a busy one max(held, job estimate).  `reserved` is the sum.
+  int64_t reserved = 0;
+  int64_t peak_reserved = 0;
+  std::vector<int64_t> charge;
+  int running = 0;
+  bool stop = false;
+  int64_t next_id = 0;
+  Clock::time_point t0 = Clock::now();
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Job *> queue;
+  std::map<int64_t, std::unique_ptr<Job>> jobs;
+  std::vector<crispy_ctx *> ctxs;
+  std::vector<std::thread> workers;
+};
+
+namespace {
+
+void run_job(crispy_ctx *ctx, Job &job) {
+  int rc = crispy_reads_upload(ctx, CRISPY_READS_KMER, job.kbases.data(), job.koff.data(), job.n);
+  if (rc == CRISPY_OK) {
+    rc = job.same_reads
+             ? crispy_reads_upload(ctx, CRISPY_READS_NW, job.kbases.data(), job.koff.data(), job.n)
+             : crispy_reads_upload(ctx, CRISPY_READS_NW, job.nbases.data(), job.noff.data(), job.n);
+  }
+  if (rc == CRISPY_OK) rc = crispy_kmer_run(ctx, job.K, job.kthr, 0, -1, &job.kmer_pairs);
+  int64_t c = 0;
+  if (rc == CRISPY_OK) rc = crispy_nw_run_on_kmer(ctx, job.band, job.nthr, &c);
+  if (rc == CRISPY_OK) {
+    job.pairs.resize(2 * c);
+    job.dists.resize(c);
+    rc = crispy_nw_fetch(ctx, 0, c, job.pairs.data(), job.dists.data());
+  }
+  job.status = rc;
+  if (rc != CRISPY_OK) job.error = crispy_ctx_error(ctx);
+}
+
+void worker_loop(crispy_server *srv, int slot) {
+  crispy_ctx *ctx = srv->ctxs[slot];
+  for (;;) {
+    Job *job = nullptr;
+    {
+      std::unique_lock<std::mutex> lk(srv->mu);
+      for (;;) {
+        if (srv->stop && srv->queue.empty()) return;
+        if (!srv->queue.empty()) {
+          // FIFO: the head job starts when it fits the budget next to what the
+          // other slots hold
+          const int64_t held = srv->charge[slot];
+          const int64_t need = std::max(held, srv->queue.front()->est);
+          if (srv->reserved - held + need <= srv->budget) {
+            job = srv->queue.front();
+            srv->queue.pop_front();
+            srv->reserved += need - held;
+            srv->peak_reserved = std::max(srv->peak_reserved, srv->reserved);
+            srv->charge[slot] = need;
+            srv->running += 1;
+            job->t_start = Clock::now();
+            break;
+          }
+          if (held > 0) {
+            // give this idle slot's buffers back, then look again
+            lk.unlock();
+            crispy_ctx_trim(ctx);
+            lk.lock();
+            srv->reserved -= held - crispy_ctx_device_bytes(ctx);
+            srv->charge[slot] = crispy_ctx_device_bytes(ctx);
+            srv->cv_work.notify_all();
+            continue;
+          }
+        }
+        srv->cv_work.wait(lk);
+      }
+    }
+    run_job(ctx, *job);
+    {
+      std::lock_guard<std::mutex> lk(srv->mu);
+      job->t_end = Clock::now();
+      job->done = true;
+      const int64_t held = crispy_ctx_device_bytes(ctx);
+      job->held = held;
+      srv->reserved += held - srv->charge[slot];
+      srv->peak_reserved = std::max(srv->peak_reserved, srv->reserved);
+      srv->charge[slot] = held;
+      srv->running -= 1;
+      // the input copies are no longer needed
+      std::string().swap(job->kbases);
+      std::string().swap(job->nbases);
+    }
+    srv->cv_done.notify_all();
+    srv->cv_work.notify_all();
+  }
+}
+
+// Widest half band kB = (L1 + L2) / (2 band) the register and sliced NW
+// kernels take (64 lanes x 24 cells per pair); wider pairs use the generic one.
+constexpr int64_t kSlicedMaxHalfBand = (64 * 24 - 1) / 2;
+
+int64_t words_per_read(int K) {
+  const int KP = K <= 6 ? K : 6;
+  const int64_t w = (int64_t(1) << (2 * KP)) / 32;
+  return w < 16 ? 16 : w;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Device bytes a context holds for a job: both read slots (bytes, 2-bit
+// packing, per-read arrays), the k-mer profiles (bitmap, repeat lists; the
+// K-mer lists and candidate buffer for K > 6), the k-mer result buffers at
+// their first size (the pairs in the tile range, at most 4 Mi entries), the
+// sort/NW/result buffers for up to 256 passing k-mer pairs per read, and the
+// per-lane DP scratch of the generic NW kernel when some pair needs it (full
+// alignment, a non-ACGT byte, or a band wider than the sliced kernel's).  A job
+// with more passing pairs grows its buffers past the estimate.
+int64_t crispy_server_job_bytes(const char *bases, const int64_t *offsets, int32_t n, int K,
+                                int band) {
+  if (n <= 0 || !bases || !offsets) return 0;
+  const int64_t nbases = offsets[n] - offsets[0];
+  int64_t maxlen = 0;
+  for (int32_t r = 0; r < n; ++r) maxlen = std::max(maxlen, offsets[r + 1] - offsets[r]);
+  bool clean = true;
+  for (int64_t q = offsets[0]; q < offsets[n] && clean; ++q) {
+    const char c = bases[q];
+    clean = c == 'A' || c == 'C' || c == 'G' || c == 'T';
+  }
+  const int64_t nb = (n + 127) / 128;
+  const int64_t tiles = nb * (nb + 1) / 2;
+  const int64_t guess = std::min<int64_t>(int64_t(1) << 22, std::max<int64_t>(4096, tiles * 128 * 128));
+  const int64_t all_pairs = (int64_t)n * (n - 1) / 2;
+  const int64_t pairs = std::min<int64_t>(all_pairs, 256 * (int64_t)n);
+  const int64_t reads = 2 * (nbases + nbases / 4 + 48 * (int64_t)n);
+  int64_t prof = (int64_t)n * (4 * words_per_read(K) + 32) + 4 * nbases;
+  if (K > 6) prof += 2 * nbases + 8 * (int64_t)n + 8 * std::max(guess, pairs);
+  int64_t est = reads + prof + 12 * guess + 72 * pairs + (int64_t)(8 << 20);
+  const bool generic = band <= 1 || !clean || (band > 0 && maxlen / band > kSlicedMaxHalfBand);
+  if (generic) est += 8 * std::min<int64_t>(pairs, 32768) * (maxlen + 1);
+  return est;
+}
+
+int crispy_server_create(int device, int slots, int64_t hbm_budget_bytes, crispy_server **out) {
+  if (!out || slots < 1 || slots > 64 || hbm_budget_bytes <= 0) return CRISPY_ERR_ARG;
+  *out = nullptr;
+  auto *srv = new crispy_server();
+  srv->device = device;
+  srv->budget = hbm_budget_bytes;
+  srv->charge.assign(slots, 0);
+  for (int s = 0; s < slots; ++s) {
+    crispy_ctx *c = nullptr;
+    const int rc = crispy_ctx_create(device, &c);
+    if (rc != CRISPY_OK) {
+      for (auto *x : srv->ctxs) crispy_ctx_destroy(x);
+      delete srv;
+      return rc;
+    }
+    srv->ctxs.push_back(c);
+  }
+  for (int s = 0; s < slots; ++s) srv->workers.emplace_back(worker_loop, srv, s);
+  *out = srv;
+  return CRISPY_OK;
+}
+
+void crispy_server_destroy(crispy_server *srv) {
+  if (!srv) return;
+  {
+    std::lock_guard<std::mutex> lk(srv->mu);
+    srv->stop = true;
+  }
+  srv->cv_work.notify_all();
+  for (auto &t : srv->workers) t.join();
+  for (auto *c : srv->ctxs) crispy_ctx_destroy(c);
+  delete srv;
+}
+
+int64_t crispy_server_submit(crispy_server *srv, const char *kmer_bases, const int64_t *kmer_offsets,
+                             const char *nw_bases, const int64_t *nw_offsets, int32_t n, int K,
+                             float kmer_threshold, int band, float nw_threshold) {
+  if (!srv || n < 0 || n > CRISPY_MAX_NUM_READS || (n > 0 && (!kmer_bases || !kmer_offsets)) ||
+      K < 1 || K > 8)
+    return CRISPY_ERR_ARG;
+  auto job = std::make_unique<Job>();
+  job->n = n;
+  job->K = K;
+  job->kthr = kmer_threshold;
+  job->band = band;
+  job->nthr = nw_threshold;
+  job->same_reads = nw_bases == nullptr;
+  if (n == 0) {
+    job->koff.assign(1, 0);
+    job->noff.assign(1, 0);
+  } else {
+    if (kmer_offsets[0] != 0 || (!job->same_reads && (!nw_offsets || nw_offsets[0] != 0)))
+      return CRISPY_ERR_ARG;
+    job->koff.assign(kmer_offsets, kmer_offsets + n + 1);
+    job->kbases.assign(kmer_bases, (size_t)job->koff[n]);
+    if (!job->same_reads) {
+      job->noff.assign(nw_offsets, nw_offsets + n + 1);
+      job->nbases.assign(nw_bases, (size_t)job->noff[n]);
+    }
+  }
+  job->est = crispy_server_job_bytes(job->kbases.data(), job->koff.data(), n, K, band);
+  std::lock_guard<std::mutex> lk(srv->mu);
+  if (job->est > srv->budget) return CRISPY_ERR_NOMEM;  // would never be admitted
+  job->id = srv->next_id++;
+  job->t_submit = Clock::now();
+  Job *raw = job.get();
+  srv->jobs[raw->id] = std::move(job);
+  srv->queue.push_back(raw);
+  srv->cv_work.notify_all();
+  return raw->id;
+}
+
+int crispy_server_wait(crispy_server *srv, int64_t job_id, int64_t *kmer_pairs, int64_t *nw_pairs,
+                       double *queue_ms, double *run_ms) {
+  if (!srv) return CRISPY_ERR_ARG;
+  std::unique_lock<std::mutex> lk(srv->mu);
+  auto it = srv->jobs.find(job_id);
+  if (it == srv->jobs.end()) return CRISPY_ERR_ARG;
+  Job *job = it->second.get();
+  srv->cv_done.wait(lk, [&] { return job->done; });
+  if (kmer_pairs) *kmer_pairs = job->kmer_pairs;
+  if (nw_pairs) *nw_pairs = (int64_t)job->dists.size();
+  using ms = std::chrono::duration<double, std::milli>;
+  if (queue_ms) *queue_ms = ms(job->t_start - job->t_submit).count();
+  if (run_ms) *run_ms = ms(job->t_end - job->t_start).count();
+  return job->status;
+}
+
+// Submission, start and end of a finished job in ms since the server was
+// created.
+int crispy_server_job_times(crispy_server *srv, int64_t job_id, double *submit_ms, double *start_ms,
+                            double *end_ms, int64_t *est_bytes, int64_t *held_bytes) {
+  if (!srv) return CRISPY_ERR_ARG;
+  std::lock_guard<std::mutex> lk(srv->mu);
+  auto it = srv->jobs.find(job_id);
+  if (it == srv->jobs.end() || !it->second->done) return CRISPY_ERR_STATE;
+  using ms = std::chrono::duration<double, std::milli>;
+  const auto t0 = srv->t0;
+  if (submit_ms) *submit_ms = ms(it->second->t_submit - t0).count();
+  if (start_ms) *start_ms = ms(it->second->t_start - t0).count();
+  if (end_ms) *end_ms = ms(it->second->t_end - t0).count();
+  if (est_bytes) *est_bytes = it->second->est;
+  if (held_bytes) *held_bytes = it->second->held;
+  return CRISPY_OK;
+}
+
+const char *crispy_server_job_error(crispy_server *srv, int64_t job_id) {
+  if (!srv) return "";
+  std::lock_guard<std::mutex> lk(srv->mu);
+  auto it = srv->jobs.find(job_id);
+  if (it == srv->jobs.end() || !it->second->done) return "";
+  return it->second->error.c_str();
+}
+
+int crispy_server_fetch(crispy_server *srv, int64_t job_id, int32_t *pairs, float *dists) {
+  if (!srv) return CRISPY_ERR_ARG;
+  std::unique_lock<std::mutex> lk(srv->mu);
+  auto it = srv->jobs.find(job_id);
+  if (it == srv->jobs.end()) return CRISPY_ERR_ARG;
+  Job *job = it->second.get();
+  srv->cv_done.wait(lk, [&] { return job->done; });
+  const int status = job->status;
+  if (status == CRISPY_OK) {
+    if (pairs && !job->pairs.empty()) std::memcpy(pairs, job->pairs.data(), job->pairs.size() * sizeof(int32_t));
+    if (dists && !job->dists.empty()) std::memcpy(dists, job->dists.data(), job->dists.size() * sizeof(float));
+  }
+  srv->jobs.erase(it);
+  return status;
+}
+
+int crispy_server_status(crispy_server *srv, int *running, int *queued, int64_t *reserved_bytes,
+                         int64_t *peak_reserved_bytes) {
+  if (!srv) return CRISPY_ERR_ARG;
+  std::lock_guard<std::mutex> lk(srv->mu);
+  if (running) *running = srv->running;
+  if (queued) *queued = (int)srv->queue.size();
+  if (reserved_bytes) *reserved_bytes = srv->reserved;
+  if (peak_reserved_bytes) *peak_reserved_bytes = srv->peak_reserved;
+  return CRISPY_OK;
+}
+
+}  // extern "C"

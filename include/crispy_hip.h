@@ -59,6 +59,10 @@ const char *crispy_ctx_error(const crispy_ctx *ctx);
 int crispy_ctx_sync(crispy_ctx *ctx);
 /* Device bytes currently held by the context. */
 int64_t crispy_ctx_device_bytes(const crispy_ctx *ctx);
+/* Frees every device buffer of the context (reads, profiles, results); the
+ * streams stay.  Buffers otherwise only grow, so a long-lived context holds
+ * the largest job it has run. */
+int crispy_ctx_trim(crispy_ctx *ctx);
 
 /* ------------------------------------------------------------- host side */
 /* FASTA reader of the reference (kmerMain.cpp:298-397 for variant
@@ -150,6 +154,50 @@ int crispy_timing_query(crispy_ctx *ctx, const char *kernel, double *total_ms,
  * 0<=m<=maxv, and out_nw[a*(maxv+1)+m] = m/a likewise.  Lets a test compare
  * the device rounding with the host's bit for bit. */
 int crispy_selftest_division(crispy_ctx *ctx, int32_t maxv, float *out_kmer, float *out_nw);
+
+/* --------------------------------------------------------------- serving */
+/* No counterpart in the reference, which runs one sample per process
+ * (crispy.sh:19-20: kmerDist, then genDist on its pairs).  A server takes a
+ * stream of such jobs: each job is one read set through the k-mer filter and
+ * the banded NW on the passing pairs (the fused crispy_kmer_run +
+ * crispy_nw_run_on_kmer), with the same results as running it alone.
+ * `slots` worker threads each own a context; the head of the FIFO queue starts
+ * as soon as a slot is free and its device-byte estimate
+ * (crispy_server_job_bytes) fits `hbm_budget_bytes` together with what the
+ * other slots hold, so admitted jobs overlap on the device. */
+typedef struct crispy_server crispy_server;
+
+int crispy_server_create(int device, int slots, int64_t hbm_budget_bytes, crispy_server **out);
+/* Finishes the queued jobs, then stops the workers and frees the contexts. */
+void crispy_server_destroy(crispy_server *srv);
+/* Device bytes admission charges for a job (an estimate of what its context
+ * will hold, from the read count, bases, longest read, K and band). */
+int64_t crispy_server_job_bytes(const char *bases, const int64_t *offsets, int32_t n, int K,
+                                int band);
+/* Copies the reads (as for crispy_reads_upload; nw_bases NULL = the same reads
+ * for both stages) and queues the job.  Returns its id (>= 0) or an error code;
+ * CRISPY_ERR_NOMEM when the job alone exceeds the budget. */
+int64_t crispy_server_submit(crispy_server *srv, const char *kmer_bases, const int64_t *kmer_offsets,
+                             const char *nw_bases, const int64_t *nw_offsets, int32_t n, int K,
+                             float kmer_threshold, int band, float nw_threshold);
+/* Blocks until the job has finished; returns its status and its pair counts,
+ * time queued and time running (ms). */
+int crispy_server_wait(crispy_server *srv, int64_t job_id, int64_t *kmer_pairs, int64_t *nw_pairs,
+                       double *queue_ms, double *run_ms);
+/* Submission, start and end times (ms since crispy_server_create) of a
+ * finished job, its admission estimate and the bytes its slot held after it. */
+int crispy_server_job_times(crispy_server *srv, int64_t job_id, double *submit_ms, double *start_ms,
+                            double *end_ms, int64_t *est_bytes, int64_t *held_bytes);
+/* Error message of a finished job that failed ("" otherwise). */
+const char *crispy_server_job_error(crispy_server *srv, int64_t job_id);
+/* Waits for the job, copies its NW-passing pairs (int32 x2, ascending (i, j))
+ * and distances out and forgets the job. */
+int crispy_server_fetch(crispy_server *srv, int64_t job_id, int32_t *pairs, float *dists);
+/* Jobs running and queued, device bytes charged to the slots now and at most
+ * so far (idle slots are charged what their contexts hold and give it back
+ * when the queue head would not fit otherwise). */
+int crispy_server_status(crispy_server *srv, int *running, int *queued, int64_t *reserved_bytes,
+                         int64_t *peak_reserved_bytes);
 
 #ifdef __cplusplus
 }

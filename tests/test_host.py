@@ -120,3 +120,23 @@ def test_cli_error_paths(tmp_path):
     assert run_bin([gd, "-h"], tmp_path).returncode == 0
     assert run_bin([gd, "-i", str(fa), "-c"], tmp_path).returncode == 2
     assert run_bin([gd, "-i", str(fa)], tmp_path).returncode == 255  # no .kpair
+
+
+def test_server_job_estimate():
+    """Admission estimate: grows with reads, pairs, K > 6 lists, and adds the
+    generic-kernel scratch for full alignment or non-ACGT reads."""
+    rng = np.random.default_rng(5)
+    seqs = [bytes(rng.choice(np.frombuffer(b"ACGT", np.uint8), int(L)).tobytes())
+            for L in rng.integers(150, 550, 3000)]
+    small = ca.Reads.from_list(seqs[:300])
+    big = ca.Reads.from_list(seqs)
+    assert ca.job_bytes(ca.Reads(b"", np.zeros(1, np.int64)), 6, 10) == 0
+    e_small, e_big = ca.job_bytes(small, 6, 10), ca.job_bytes(big, 6, 10)
+    assert 0 < e_small < e_big
+    # the bitmap alone: 4^6 bits per read
+    assert e_big > big.n * 512 + 2 * len(big.bases)
+    assert ca.job_bytes(big, 8, 10) > e_big
+    full = ca.job_bytes(big, 6, 1) - e_big
+    assert full == 8 * min(256 * big.n, 32768) * (max(np.diff(big.offsets)) + 1)
+    dirty = ca.Reads.from_list([seqs[0][:100] + b"N" + seqs[0][101:]] + seqs[1:])
+    assert ca.job_bytes(dirty, 6, 10) - ca.job_bytes(big, 6, 10) == full
