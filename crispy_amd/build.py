@@ -25,7 +25,8 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["api.hip", "kmer.hip", "nw.hip"]
+HIP_SOURCES = ["api.hip", "kmer.hip", "nw.hip", "nw_fam_reg0.hip", "nw_fam_reg1.hip",
+               "nw_fam_reg2.hip", "nw_fam_reg3.hip", "nw_fam_sliced.hip"]
 HOST_SOURCES = ["host.cpp", "serve.cpp"]
 CLI = {"kmerDist": "cli_kmer.cpp", "genDist": "cli_gen.cpp"}
 

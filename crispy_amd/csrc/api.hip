@@ -40,6 +40,24 @@ int ensure(crispy_ctx *ctx, DevBuf &b, size_t bytes) {
   return CRISPY_OK;
 }
 
+int ensure_host(crispy_ctx *ctx, size_t bytes) {
+  if (ctx->h_stage_bytes >= bytes) return CRISPY_OK;
+  if (ctx->h_stage) {
+    CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    CRISPY_HIP(ctx, hipHostFree(ctx->h_stage));
+    ctx->h_stage = nullptr;
+    ctx->h_stage_bytes = 0;
+  }
+  bytes = std::max<size_t>(bytes, 64 << 10);
+  if (hipHostMalloc(&ctx->h_stage, bytes, hipHostMallocDefault) != hipSuccess) {
+    ctx->h_stage = nullptr;
+    (void)hipGetLastError();
+    return set_err(ctx, CRISPY_ERR_NOMEM, "hipHostMalloc(%zu) failed", bytes);
+  }
+  ctx->h_stage_bytes = bytes;
+  return CRISPY_OK;
+}
+
 void release(crispy_ctx *ctx, DevBuf &b) {
   if (b.p) {
     (void)hipFree(b.p);
@@ -195,7 +213,7 @@ static void release_all(crispy_ctx *ctx) {
                     &ctx->nw.in_pairs, &ctx->nw.keys,      &ctx->nw.keys_alt,  &ctx->nw.idx,
                     &ctx->nw.idx_alt,  &ctx->nw.dist,      &ctx->nw.hist,      &ctx->nw.cells,
                     &ctx->nw.flags,    &ctx->nw.sel,       &ctx->nw.nsel,      &ctx->nw.out_pairs,
-                    &ctx->nw.out_dists, &ctx->nw.scratch,  &ctx->temp};
+                    &ctx->nw.out_dists, &ctx->nw.scratch,  &ctx->nw.tasks,     &ctx->temp};
   for (DevBuf *b : bufs) release(ctx, *b);
   for (auto &r : ctx->reads) {
     DevBuf *rb[] = {&r.bases, &r.off, &r.len, &r.pack, &r.poff, &r.clean};
@@ -216,6 +234,7 @@ void crispy_ctx_destroy(crispy_ctx *ctx) {
   drain_timers(ctx);
   for (auto e : ctx->pool) (void)hipEventDestroy(e);
   release_all(ctx);
+  if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
   for (int w = 0; w < kNumWorkers; ++w) {
     if (ctx->workers[w]) (void)hipStreamDestroy(ctx->workers[w]);
     if (ctx->join_ev[w]) (void)hipEventDestroy(ctx->join_ev[w]);

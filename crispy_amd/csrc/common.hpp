@@ -110,6 +110,7 @@ struct KmerState {
 struct NwState {
   DevBuf in_pairs, keys, keys_alt, idx, idx_alt, dist, hist, cells, flags, sel, nsel;
   DevBuf out_pairs, out_dists, scratch;
+  DevBuf tasks;  // bucket table + wave tasks of the fused DP launches
   int64_t count = 0;
   bool valid = false;
 };
@@ -141,6 +142,8 @@ struct crispy_ctx {
   KmerState km;
   NwState nw;
   DevBuf temp;  // rocPRIM temporary storage
+  void *h_stage = nullptr;  // pinned host staging for small async uploads
+  size_t h_stage_bytes = 0;
   bool timing = false;
   std::vector<PendingEvent> pending;
   std::vector<hipEvent_t> pool;
@@ -153,6 +156,9 @@ namespace crispy {
 int set_err(crispy_ctx *ctx, int code, const char *fmt, ...);
 int ensure(crispy_ctx *ctx, DevBuf &b, size_t bytes);
 void release(crispy_ctx *ctx, DevBuf &b);
+// Grow-only pinned host staging buffer (ctx->h_stage); the main stream is
+// synchronised before an old buffer is freed.
+int ensure_host(crispy_ctx *ctx, size_t bytes);
 void timer_begin(crispy_ctx *ctx, const char *name, hipEvent_t *a, hipStream_t s = nullptr);
 void timer_end(crispy_ctx *ctx, const char *name, hipEvent_t a, double units, hipStream_t s = nullptr);
 int fork_workers(crispy_ctx *ctx);
