@@ -11,15 +11,16 @@
 // every code is lo and hinfo is 0, so matches is the multiset intersection.
 //
 // sum_v min(c_a, c_b) = sum_v [c_a>=1][c_b>=1] + sum_v min(c_a-1, c_b-1)^+,
-// so each read is stored as
+// so for K <= 6 each read is stored as
 //   * a presence bitmap of 4^K bits (padded to 16-word chunks), and
 //   * a short sorted list of (code << 16 | count-1) for codes seen twice or
 //     more ("repeats"),
 // and a 128x128 tile of pairs is an AND + popcount "GEMM" over the bitmaps
 // (v_bcnt_u32_b32 accumulates) plus a merge of the two repeat lists for
-// the few pairs where both have repeats.  The filter dist <= thr and the
-// compaction of passing pairs run in the same kernel; only passing pairs
-// leave the chip.
+// the pairs the popcount bound does not settle.  The filter dist <= thr and
+// the compaction of passing pairs run in the same kernel; only passing pairs
+// leave the chip.  K = 7, 8 use the k = 6 tiles as a candidate filter and the
+// reference's merge on the candidates (kmer_cand_kernel).
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -139,28 +140,20 @@ __global__ __launch_bounds__(256) void kmer_profile_kernel(
   for (int w = tid; w < WP; w += 256) dst[w] = bm[w];
 }
 
-// sum_v min(e_a[v], e_b[v]) over two sorted repeat lists, split hi/lo.
-__device__ __forceinline__ void repeat_merge(const uint32_t *__restrict__ A, int na,
-                                             const uint32_t *__restrict__ B, int nb,
-                                             uint32_t &lo_sum, uint32_t &hi_sum) {
+// sum_v min(e_a[v], e_b[v]) over two repeat lists sorted by code (entries
+// code << 16 | count - 1).
+template <class P>
+__device__ __forceinline__ uint32_t repeat_min_sum(P A, int na, P B, int nb) {
+  uint32_t sum = 0;
   int x = 0, y = 0;
   while (x < na && y < nb) {
     const uint32_t ea = A[x], eb = B[y];
     const uint32_t va = ea >> 16, vb = eb >> 16;
-    if (va == vb) {
-      const uint32_t m = min(ea & 0xFFFFu, eb & 0xFFFFu);
-      if (va & 0x8000u)
-        hi_sum += m;
-      else
-        lo_sum += m;
-      ++x;
-      ++y;
-    } else if (va < vb) {
-      ++x;
-    } else {
-      ++y;
-    }
+    if (va == vb) sum += min(ea & 0xFFFFu, eb & 0xFFFFu);
+    x += va <= vb;
+    y += va >= vb;
   }
+  return sum;
 }
 
 // acc + popcount(x) as one v_bcnt_u32_b32 (the compiler otherwise splits
@@ -193,32 +186,37 @@ __device__ __forceinline__ void tile_coords(int64_t t, int nb, int &bi, int &bj)
 // an 8x8 block of pairs: rows ty*4+{0..3} and 64+ty*4+{0..3}, columns
 // likewise with tx.  Bitmap words are staged in LDS 16 at a time, stored
 // word-major so each thread reads its 8 rows (or columns) with two 128-bit
-// LDS loads.  For K = 8 the hi half of the bitmap is processed first and
-// kept in a second accumulator set.
+// LDS loads.
 // Epilogue: matches <= popcount + min(excess_i, excess_j) (excess = sum of
 // count-1 over a read's codes), so a pair with
 //   ub < T_min * (1 - thr - 2e-5) - 1
 // cannot pass (the margin covers the fp32 roundings); only the others pay
-// for the repeat-list merge and the division.
+// for the repeat-list merge and the division.  The tile's 256 repeat lists
+// are copied into the (then free) chunk buffers when they fit, so the
+// merges read LDS instead of L2; a passing pair's match count replaces its
+// popcount for the output pass.
+// CAND: K = 7, 8 candidates -- no merge, the bound alone (see below).
 template <int K, bool CAND>
-__global__ __launch_bounds__(256) void kmer_tile_kernel(
+__global__ __launch_bounds__(256, 4) void kmer_tile_kernel(
     const uint32_t *__restrict__ bitmap, const int32_t *__restrict__ Tn,
-    const uint32_t *__restrict__ hinfo, const int32_t *__restrict__ rcnt,
-    const int64_t *__restrict__ roff, const uint32_t *__restrict__ rlist,
-    const int32_t *__restrict__ excess, int32_t n, int nb, int64_t tile_begin, int64_t ntiles,
-    int64_t per_xcd, float thr, float cut, int tshift,
+    const int32_t *__restrict__ rcnt, const int64_t *__restrict__ roff,
+    const uint32_t *__restrict__ rlist, const int32_t *__restrict__ excess, int32_t n, int nb,
+    int64_t tile_begin, int64_t ntiles, int64_t per_xcd, float thr, float cut, int tshift,
     unsigned long long *__restrict__ counter, uint64_t *__restrict__ out_keys,
     float *__restrict__ out_dists, int64_t cap) {
+  static_assert(K >= 1 && K <= 6, "bitmap tiles hold k <= 6");
   constexpr int WP = kmer_words(K);
   constexpr int NCH = WP / kChunkWords;
-  constexpr bool kHi = K == 8;
-  __shared__ uint32_t As[kChunkWords][kTile];
-  __shared__ uint32_t Bs[kChunkWords][kTile];
+  constexpr int kStageWords = 2 * kChunkWords * kTile;
+  __shared__ uint32_t smem[kStageWords];  // As | Bs in the main loop, then repeat lists
+  uint32_t(*As)[kTile] = reinterpret_cast<uint32_t(*)[kTile]>(smem);
+  uint32_t(*Bs)[kTile] = reinterpret_cast<uint32_t(*)[kTile]>(smem + kChunkWords * kTile);
   __shared__ int32_t sT[2 * kTile];
-  __shared__ uint32_t sH[2 * kTile];
   __shared__ int32_t sRc[2 * kTile];
   __shared__ int64_t sRo[2 * kTile];
   __shared__ int32_t sE[2 * kTile];
+  __shared__ int32_t sLo[2 * kTile];
+  __shared__ int32_t sWave[4];
 
   // Work-groups are dispatched round-robin over the 8 XCDs; give each XCD a
   // contiguous run of tiles so tiles that share a row block share an L2.
@@ -235,24 +233,16 @@ __global__ __launch_bounds__(256) void kmer_tile_kernel(
     const int r = tid < kTile ? row0 + tid : col0 + (tid - kTile);
     const bool ok = r < n;
     sT[tid] = ok ? Tn[r] : 1;
-    sH[tid] = ok ? hinfo[r] : 0u;
     sRc[tid] = ok ? rcnt[r] : 0;
     sRo[tid] = ok ? roff[r] : 0;
     sE[tid] = ok ? excess[r] : 0;
   }
 
   uint32_t acc[8][8];
-  uint32_t acch[8][8];  // hi-half popcounts (K = 8 only)
 #pragma unroll
   for (int x = 0; x < 8; ++x)
 #pragma unroll
     for (int y = 0; y < 8; ++y) acc[x][y] = 0;
-  if constexpr (kHi) {
-#pragma unroll
-    for (int x = 0; x < 8; ++x)
-#pragma unroll
-      for (int y = 0; y < 8; ++y) acch[x][y] = 0;
-  }
 
   const int lr = tid >> 1, lh = tid & 1;  // loader: row within the tile, half chunk
   const bool aok = row0 + lr < n, bok = col0 + lr < n;
@@ -260,12 +250,10 @@ __global__ __launch_bounds__(256) void kmer_tile_kernel(
   const uint32_t *gb = bitmap + (int64_t)(bok ? col0 + lr : 0) * WP + lh * 8;
   const uint4 z4 = make_uint4(0, 0, 0, 0);
 
-  auto chunk_of = [](int ch) { return kHi ? (ch + NCH / 2) % NCH : ch; };
-  int c0 = chunk_of(0);
-  uint4 pa0 = aok ? *(const uint4 *)(ga + c0 * kChunkWords) : z4;
-  uint4 pa1 = aok ? *(const uint4 *)(ga + c0 * kChunkWords + 4) : z4;
-  uint4 pb0 = bok ? *(const uint4 *)(gb + c0 * kChunkWords) : z4;
-  uint4 pb1 = bok ? *(const uint4 *)(gb + c0 * kChunkWords + 4) : z4;
+  uint4 pa0 = aok ? *(const uint4 *)(ga) : z4;
+  uint4 pa1 = aok ? *(const uint4 *)(ga + 4) : z4;
+  uint4 pb0 = bok ? *(const uint4 *)(gb) : z4;
+  uint4 pb1 = bok ? *(const uint4 *)(gb + 4) : z4;
 
   for (int ch = 0; ch < NCH; ++ch) {
     __syncthreads();
@@ -278,11 +266,11 @@ __global__ __launch_bounds__(256) void kmer_tile_kernel(
     }
     __syncthreads();
     if (ch + 1 < NCH) {  // prefetch the next chunk while this one is consumed
-      const int cn = chunk_of(ch + 1);
-      pa0 = aok ? *(const uint4 *)(ga + cn * kChunkWords) : z4;
-      pa1 = aok ? *(const uint4 *)(ga + cn * kChunkWords + 4) : z4;
-      pb0 = bok ? *(const uint4 *)(gb + cn * kChunkWords) : z4;
-      pb1 = bok ? *(const uint4 *)(gb + cn * kChunkWords + 4) : z4;
+      const int cn = (ch + 1) * kChunkWords;
+      pa0 = aok ? *(const uint4 *)(ga + cn) : z4;
+      pa1 = aok ? *(const uint4 *)(ga + cn + 4) : z4;
+      pb0 = bok ? *(const uint4 *)(gb + cn) : z4;
+      pb1 = bok ? *(const uint4 *)(gb + cn + 4) : z4;
     }
 #pragma unroll
     for (int w = 0; w < kChunkWords; ++w) {
@@ -297,45 +285,64 @@ __global__ __launch_bounds__(256) void kmer_tile_kernel(
 #pragma unroll
         for (int y = 0; y < 8; ++y) acc[x][y] = bcnt_acc(a[x] & b[y], acc[x][y]);
     }
-    if constexpr (kHi) {
-      if (ch == NCH / 2 - 1) {
-#pragma unroll
-        for (int x = 0; x < 8; ++x)
-#pragma unroll
-          for (int y = 0; y < 8; ++y) {
-            acch[x][y] = acc[x][y];
-            acc[x][y] = 0;
-          }
-      }
-    }
   }
 
-  // ---- epilogue: dist, filter, compaction
-  auto dist_of = [&](int x, int y, uint32_t pop_lo, uint32_t pop_hi) -> float {
-    const int li = x < 4 ? ty * 4 + x : 64 + ty * 4 + (x - 4);
-    const int lj = kTile + (y < 4 ? tx * 4 + y : 64 + tx * 4 + (y - 4));
-    uint32_t lo = pop_lo, hi = pop_hi;
+  // ---- stage the repeat lists in LDS when all 256 fit
+  bool staged = false;
+  if (!CAND) {
+    const int own = sRc[tid];
+    int incl = own;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int v = __shfl_up(incl, off, 64);
+      if ((tid & 63) >= off) incl += v;
+    }
+    if ((tid & 63) == 63) sWave[tid >> 6] = incl;
+    __syncthreads();  // also: every thread is done with As / Bs
+    int before = 0;
+    for (int w = 0; w < (tid >> 6); ++w) before += sWave[w];
+    staged = sWave[0] + sWave[1] + sWave[2] + sWave[3] <= kStageWords;
+    if (staged) {
+      const int o = before + incl - own;
+      sLo[tid] = o;
+      const uint32_t *src = rlist + sRo[tid];
+      for (int e = 0; e < own; ++e) smem[o + e] = src[e];
+    }
+    __syncthreads();
+  }
+  auto matches_of = [&](int li, int lj, uint32_t pop) -> uint32_t {
     const int na = sRc[li], nbr = sRc[lj];
-    if (na > 0 && nbr > 0) repeat_merge(rlist + sRo[li], na, rlist + sRo[lj], nbr, lo, hi);
-    uint32_t matches = lo;
-    if constexpr (kHi) matches = hi + (sH[li] == sH[lj] ? lo : 0u);
-    const int tmin = min(sT[li], sT[lj]);
-    return 1.0f - (float)matches / (float)tmin;
+    if (na > 0 && nbr > 0) {
+      if (staged)
+        pop += repeat_min_sum(smem + sLo[li], na, smem + sLo[lj], nbr);
+      else
+        pop += repeat_min_sum(rlist + sRo[li], na, rlist + sRo[lj], nbr);
+    }
+    return pop;
   };
+
+  // ---- epilogue: bound, exact count, filter, compaction
   uint64_t passmask = 0;
   int mycount = 0;
 #pragma unroll
   for (int x = 0; x < 8; ++x) {
-    const int i = row0 + (x < 4 ? ty * 4 + x : 64 + ty * 4 + (x - 4));
+    const int li = x < 4 ? ty * 4 + x : 64 + ty * 4 + (x - 4);
+    const int i = row0 + li;
 #pragma unroll
     for (int y = 0; y < 8; ++y) {
-      const int j = col0 + (y < 4 ? tx * 4 + y : 64 + tx * 4 + (y - 4));
+      const int lj = kTile + (y < 4 ? tx * 4 + y : 64 + tx * 4 + (y - 4));
+      const int j = col0 + lj - kTile;
       if (i < j && j < n) {
-        const int li = x < 4 ? ty * 4 + x : 64 + ty * 4 + (x - 4);
-        const int lj = kTile + (y < 4 ? tx * 4 + y : 64 + tx * 4 + (y - 4));
-        const uint32_t ub = acc[x][y] + (kHi ? acch[x][y] : 0u) + (uint32_t)min(sE[li], sE[lj]);
-        if ((float)ub < (float)(min(sT[li], sT[lj]) - tshift) * cut - 1.0f) continue;
-        if (CAND || passes(dist_of(x, y, acc[x][y], kHi ? acch[x][y] : 0u), thr)) {
+        const int tmin = min(sT[li], sT[lj]);
+        const uint32_t ub = acc[x][y] + (uint32_t)min(sE[li], sE[lj]);
+        if ((float)ub < (float)(tmin - tshift) * cut - 1.0f) continue;
+        bool keep = CAND;
+        if (!CAND) {
+          const uint32_t m = matches_of(li, lj, acc[x][y]);
+          keep = passes(1.0f - (float)m / (float)tmin, thr);
+          acc[x][y] = m;
+        }
+        if (keep) {
           passmask |= 1ull << (x * 8 + y);
           ++mycount;
         }
@@ -360,14 +367,16 @@ __global__ __launch_bounds__(256) void kmer_tile_kernel(
   int64_t pos = (int64_t)base + (incl - mycount);
 #pragma unroll
   for (int x = 0; x < 8; ++x) {
-    const int i = row0 + (x < 4 ? ty * 4 + x : 64 + ty * 4 + (x - 4));
+    const int li = x < 4 ? ty * 4 + x : 64 + ty * 4 + (x - 4);
+    const int i = row0 + li;
 #pragma unroll
     for (int y = 0; y < 8; ++y) {
       if (passmask & (1ull << (x * 8 + y))) {
-        const int j = col0 + (y < 4 ? tx * 4 + y : 64 + tx * 4 + (y - 4));
+        const int lj = kTile + (y < 4 ? tx * 4 + y : 64 + tx * 4 + (y - 4));
+        const int j = col0 + lj - kTile;
         if (pos < cap) {
           out_keys[pos] = ((uint64_t)i << 20) | (uint64_t)j;
-          if (!CAND) out_dists[pos] = dist_of(x, y, acc[x][y], kHi ? acch[x][y] : 0u);
+          if (!CAND) out_dists[pos] = 1.0f - (float)acc[x][y] / (float)min(sT[li], sT[lj]);
         }
         ++pos;
       }
@@ -513,8 +522,8 @@ static int launch_tiles(crispy_ctx *ctx, int32_t n, int nb, int64_t tb, int64_t 
   hipEvent_t ev;
   timer_begin(ctx, "kmer_tile", &ev);
   kmer_tile_kernel<K, CAND><<<(unsigned)grid, 256, 0, ctx->stream>>>(
-      km.bitmap.as<uint32_t>(), km.T.as<int32_t>(), km.hinfo.as<uint32_t>(), km.rcnt.as<int32_t>(),
-      km.roff.as<int64_t>(), km.rlist.as<uint32_t>(), km.excess.as<int32_t>(), n, nb, tb, nt,
+      km.bitmap.as<uint32_t>(), km.T.as<int32_t>(), km.rcnt.as<int32_t>(), km.roff.as<int64_t>(),
+      km.rlist.as<uint32_t>(), km.excess.as<int32_t>(), n, nb, tb, nt,
       per_xcd, thr, 1.0f - thr - 2e-5f, tshift, km.counter.as<unsigned long long>(), out_keys,
       out_dists, cap);
   CRISPY_LAUNCHED(ctx);
