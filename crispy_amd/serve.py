@@ -9,8 +9,8 @@ with HBM-budget admission.  Two policies are compared on the same arrivals:
   batched    slots=S, jobs admitted while others run share the device
 
 Jobs: a seeded pool of synthetic read sets (crispy_amd.synth), sizes
-log-uniform in [n_lo, n_hi] reads, log-normal lengths (median 300, sigma
-0.35, clipped to 150-550), k=6, band 10, default thresholds.  Arrival rates
+uniform in {n_lo..n_hi} reads, log-normal lengths (median 300, sigma 0.35,
+clipped to 150-550), k=6, band 10, default thresholds.  Arrival rates
 are multiples of the serial service rate (1 / mean serial job time).  Every
 job's result in the mix is compared bit for bit with its result alone.
 
@@ -19,7 +19,6 @@ job's result in the mix is compared bit for bit with its result alone.
 import argparse
 import hashlib
 import json
-import math
 import threading
 import time
 
@@ -29,13 +28,16 @@ from . import api, synth
 
 
 def make_pool(count, seed, n_lo=500, n_hi=20000, K=6, band=10):
-    """Seeded job pool: list of dicts with reads and stage parameters."""
+    """Seeded job pool: job k has n ~ U{n_lo..n_hi} reads (drawn from `seed`)
+    generated with seed 1000 + k: log-normal lengths (median 300, sigma
+    0.35, clipped to 150-550), n/200 species at 3-10 %, 1 % substitutions,
+    0.5 % indels."""
     rng = np.random.default_rng(seed)
     pool = []
     for k in range(count):
-        n = int(round(math.exp(rng.uniform(math.log(n_lo), math.log(n_hi)))))
+        n = int(rng.integers(n_lo, n_hi + 1))
         species = max(5, n // 200)
-        seqs, _ = synth.generate(n=n, seed=int(rng.integers(1 << 30)), species=species,
+        seqs, _ = synth.generate(n=n, seed=1000 + k, species=species,
                                  div=(0.03, 0.10), law=("lognormal", 300, 0.35, 150, 550),
                                  p_sub=0.01, p_indel=0.005)
         reads = api.Reads.from_list(seqs)
