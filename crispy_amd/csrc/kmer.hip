@@ -31,6 +31,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <utility>
 
 #include "common.hpp"
 
@@ -156,14 +157,6 @@ __device__ __forceinline__ uint32_t repeat_min_sum(P A, int na, P B, int nb) {
   return sum;
 }
 
-// acc + popcount(x) as one v_bcnt_u32_b32 (the compiler otherwise splits
-// the sum into a popcount and an add tree: 3 instead of 2 ops per word).
-__device__ __forceinline__ uint32_t bcnt_acc(uint32_t x, uint32_t acc) {
-  uint32_t r;
-  asm("v_bcnt_u32_b32 %0, %1, %2" : "=v"(r) : "v"(x), "v"(acc));
-  return r;
-}
-
 // kmerMain.cpp:180 / kmerCPU.cpp:45
 __device__ __forceinline__ bool passes(float dist, float thr) {
   const float diff = dist - thr;
@@ -182,22 +175,44 @@ __device__ __forceinline__ void tile_coords(int64_t t, int nb, int &bi, int &bj)
   bj = r + (int)(t - start(r));
 }
 
-// One 128x128 tile of read pairs per block of 256 threads; each thread owns
-// an 8x8 block of pairs: rows ty*4+{0..3} and 64+ty*4+{0..3}, columns
-// likewise with tx.  Bitmap words are staged in LDS 16 at a time, stored
-// word-major so each thread reads its 8 rows (or columns) with two 128-bit
-// LDS loads.
+// ------------------------------------------------------ matrix-core tiles
+// One 128x128 tile of read pairs per block.  The popcount product runs on the
+// matrix cores: popcount(a AND b) = sum_k a_k * b_k over {0,1} bytes, one
+// v_mfma_i32_32x32x32_i8 per 32 x 32 pairs x 32 codes.  Operand lanes (checked
+// on the device): lane l holds row (A) / column (B) l % 32 and 16 k-slots, the
+// same 16 for A and B; accumulator r of lane l is pair (8(r/4) + 4(l/32) + r%4,
+// l % 32).  Which 16 of a step's codes a lane holds is free as long as A and B
+// agree, so for an 8-word group lane half g takes words 4g..4g+3 and step s
+// (bit plane) expands each word as (w >> s) & 0x01010101 -- 2 VALU ops per
+// 4 k-slots, 48 per 8 MFMAs.
+// Block: 2 waves, wave w owns tile rows [64w, 64w + 64) x all 128 columns
+// (2 x 4 fragments, 128 accumulators per lane).  Bitmap chunks of 16 words are
+// staged row-major (padded) in LDS, the next chunk prefetched into registers;
+// tiles are mapped XCD-contiguously (work-group b -> XCD b mod 8 gets a
+// contiguous run, so tiles sharing a row block share an L2).
 // Epilogue: matches <= popcount + min(excess_i, excess_j) (excess = sum of
 // count-1 over a read's codes), so a pair with
-//   ub < T_min * (1 - thr - 2e-5) - 1
-// cannot pass (the margin covers the fp32 roundings); only the others pay
-// for the repeat-list merge and the division.  The tile's 256 repeat lists
-// are copied into the (then free) chunk buffers when they fit, so the
-// merges read LDS instead of L2; a passing pair's match count replaces its
-// popcount for the output pass.
-// CAND: K = 7, 8 candidates -- no merge, the bound alone (see below).
+//   ub < (T_min - tshift) * (1 - thr - 2e-5) - 1
+// cannot pass (the margin covers the fp32 roundings); only the others pay for
+// the repeat-list merge and the division.  The tile's 256 repeat lists are
+// copied into the (then idle) chunk buffer when they fit, so the merges read
+// LDS; a passing pair's match count replaces its popcount for the output pass.
+// CAND (K = 7, 8 candidates, tshift = K - 6): no merge, the bound alone.
+template <class F, int... I>
+__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+// f(integral_constant<int, 0>) ... f(integral_constant<int, N - 1>), unrolled
+template <int N, class F>
+__device__ __forceinline__ void static_for(F &&f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+typedef int mfma_v4i __attribute__((ext_vector_type(4)));
+typedef int mfma_v16i __attribute__((ext_vector_type(16)));
+
 template <int K, bool CAND>
-__global__ __launch_bounds__(256, 4) void kmer_tile_kernel(
+__global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
     const uint32_t *__restrict__ bitmap, const int32_t *__restrict__ Tn,
     const int32_t *__restrict__ rcnt, const int64_t *__restrict__ roff,
     const uint32_t *__restrict__ rlist, const int32_t *__restrict__ excess, int32_t n, int nb,
@@ -207,19 +222,16 @@ __global__ __launch_bounds__(256, 4) void kmer_tile_kernel(
   static_assert(K >= 1 && K <= 6, "bitmap tiles hold k <= 6");
   constexpr int WP = kmer_words(K);
   constexpr int NCH = WP / kChunkWords;
-  constexpr int kStageWords = 2 * kChunkWords * kTile;
-  __shared__ uint32_t smem[kStageWords];  // As | Bs in the main loop, then repeat lists
-  uint32_t(*As)[kTile] = reinterpret_cast<uint32_t(*)[kTile]>(smem);
-  uint32_t(*Bs)[kTile] = reinterpret_cast<uint32_t(*)[kTile]>(smem + kChunkWords * kTile);
+  constexpr int kRow = kChunkWords + 4;  // padded row stride of a staged chunk (words)
+  constexpr int kStageWords = 2 * kTile * kRow;
+  __shared__ uint32_t Ps[kStageWords];  // bitmap chunk in the main loop, then repeat lists
   __shared__ int32_t sT[2 * kTile];
   __shared__ int32_t sRc[2 * kTile];
   __shared__ int64_t sRo[2 * kTile];
   __shared__ int32_t sE[2 * kTile];
   __shared__ int32_t sLo[2 * kTile];
-  __shared__ int32_t sWave[4];
+  __shared__ int32_t sWave[2];
 
-  // Work-groups are dispatched round-robin over the 8 XCDs; give each XCD a
-  // contiguous run of tiles so tiles that share a row block share an L2.
   const int64_t bid = blockIdx.x;
   const int64_t local = (bid & 7) * per_xcd + (bid >> 3);
   if (local >= ntiles) return;
@@ -227,86 +239,127 @@ __global__ __launch_bounds__(256, 4) void kmer_tile_kernel(
   tile_coords(tile_begin + local, nb, bi, bj);
   const int row0 = bi * kTile, col0 = bj * kTile;
   const int tid = threadIdx.x;
-  const int ty = tid >> 4, tx = tid & 15;
+  const int lane = tid & 63, wv = tid >> 6;
+  const int g = lane >> 5, r32 = lane & 31;
 
-  {
-    const int r = tid < kTile ? row0 + tid : col0 + (tid - kTile);
+  for (int q = tid; q < 2 * kTile; q += 128) {
+    const int r = q < kTile ? row0 + q : col0 + (q - kTile);
     const bool ok = r < n;
-    sT[tid] = ok ? Tn[r] : 1;
-    sRc[tid] = ok ? rcnt[r] : 0;
-    sRo[tid] = ok ? roff[r] : 0;
-    sE[tid] = ok ? excess[r] : 0;
+    sT[q] = ok ? Tn[r] : 1;
+    sRc[q] = ok ? rcnt[r] : 0;
+    sRo[q] = ok ? roff[r] : 0;
+    sE[q] = ok ? excess[r] : 0;
   }
 
-  uint32_t acc[8][8];
+  // loader: thread t stages row t (read row0 + t) and row 128 + t (read col0 + t)
+  const bool aok = row0 + tid < n, bok = col0 + tid < n;
+  const uint4 *ga = reinterpret_cast<const uint4 *>(bitmap + (int64_t)(aok ? row0 + tid : 0) * WP);
+  const uint4 *gb = reinterpret_cast<const uint4 *>(bitmap + (int64_t)(bok ? col0 + tid : 0) * WP);
+  // rows past n load row 0 (always valid) and are zeroed by the mask
+  const uint32_t amask = aok ? 0xFFFFFFFFu : 0u, bmask = bok ? 0xFFFFFFFFu : 0u;
+  auto masked = [](uint4 v, uint32_t m) { return make_uint4(v.x & m, v.y & m, v.z & m, v.w & m); };
+  uint4 pa[4], pb[4];
 #pragma unroll
-  for (int x = 0; x < 8; ++x)
+  for (int q = 0; q < 4; ++q) {
+    pa[q] = masked(ga[q], amask);
+    pb[q] = masked(gb[q], bmask);
+  }
+
+  mfma_v16i acc[2][4];
 #pragma unroll
-    for (int y = 0; y < 8; ++y) acc[x][y] = 0;
-
-  const int lr = tid >> 1, lh = tid & 1;  // loader: row within the tile, half chunk
-  const bool aok = row0 + lr < n, bok = col0 + lr < n;
-  const uint32_t *ga = bitmap + (int64_t)(aok ? row0 + lr : 0) * WP + lh * 8;
-  const uint32_t *gb = bitmap + (int64_t)(bok ? col0 + lr : 0) * WP + lh * 8;
-  const uint4 z4 = make_uint4(0, 0, 0, 0);
-
-  uint4 pa0 = aok ? *(const uint4 *)(ga) : z4;
-  uint4 pa1 = aok ? *(const uint4 *)(ga + 4) : z4;
-  uint4 pb0 = bok ? *(const uint4 *)(gb) : z4;
-  uint4 pb1 = bok ? *(const uint4 *)(gb + 4) : z4;
+  for (int fa = 0; fa < 2; ++fa)
+#pragma unroll
+    for (int fb = 0; fb < 4; ++fb)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[fa][fb][r] = 0;
 
   for (int ch = 0; ch < NCH; ++ch) {
     __syncthreads();
     {
-      const int w = lh * 8;
-      As[w + 0][lr] = pa0.x; As[w + 1][lr] = pa0.y; As[w + 2][lr] = pa0.z; As[w + 3][lr] = pa0.w;
-      As[w + 4][lr] = pa1.x; As[w + 5][lr] = pa1.y; As[w + 6][lr] = pa1.z; As[w + 7][lr] = pa1.w;
-      Bs[w + 0][lr] = pb0.x; Bs[w + 1][lr] = pb0.y; Bs[w + 2][lr] = pb0.z; Bs[w + 3][lr] = pb0.w;
-      Bs[w + 4][lr] = pb1.x; Bs[w + 5][lr] = pb1.y; Bs[w + 6][lr] = pb1.z; Bs[w + 7][lr] = pb1.w;
+      uint4 *da = reinterpret_cast<uint4 *>(&Ps[tid * kRow]);
+      uint4 *db = reinterpret_cast<uint4 *>(&Ps[(kTile + tid) * kRow]);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        da[q] = pa[q];
+        db[q] = pb[q];
+      }
     }
     __syncthreads();
     if (ch + 1 < NCH) {  // prefetch the next chunk while this one is consumed
-      const int cn = (ch + 1) * kChunkWords;
-      pa0 = aok ? *(const uint4 *)(ga + cn) : z4;
-      pa1 = aok ? *(const uint4 *)(ga + cn + 4) : z4;
-      pb0 = bok ? *(const uint4 *)(gb + cn) : z4;
-      pb1 = bok ? *(const uint4 *)(gb + cn + 4) : z4;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        pa[q] = masked(ga[(ch + 1) * 4 + q], amask);
+        pb[q] = masked(gb[(ch + 1) * 4 + q], bmask);
+      }
     }
 #pragma unroll
-    for (int w = 0; w < kChunkWords; ++w) {
-      const uint4 ra0 = *(const uint4 *)&As[w][ty * 4];
-      const uint4 ra1 = *(const uint4 *)&As[w][64 + ty * 4];
-      const uint4 rb0 = *(const uint4 *)&Bs[w][tx * 4];
-      const uint4 rb1 = *(const uint4 *)&Bs[w][64 + tx * 4];
-      const uint32_t a[8] = {ra0.x, ra0.y, ra0.z, ra0.w, ra1.x, ra1.y, ra1.z, ra1.w};
-      const uint32_t b[8] = {rb0.x, rb0.y, rb0.z, rb0.w, rb1.x, rb1.y, rb1.z, rb1.w};
+    for (int m = 0; m < 2; ++m) {  // two 8-word groups per chunk
+      uint4 xa[2], xb[4];
 #pragma unroll
-      for (int x = 0; x < 8; ++x)
+      for (int fa = 0; fa < 2; ++fa)
+        xa[fa] = *reinterpret_cast<const uint4 *>(&Ps[(wv * 64 + fa * 32 + r32) * kRow + 8 * m + 4 * g]);
 #pragma unroll
-        for (int y = 0; y < 8; ++y) acc[x][y] = bcnt_acc(a[x] & b[y], acc[x][y]);
+      for (int fb = 0; fb < 4; ++fb)
+        xb[fb] = *reinterpret_cast<const uint4 *>(&Ps[(kTile + fb * 32 + r32) * kRow + 8 * m + 4 * g]);
+      // one bit plane per step; the words shift in place so the planes are
+      // not all expanded (and live) at once
+#pragma unroll 1
+      for (int s = 0; s < 8; ++s) {
+        constexpr uint32_t M = 0x01010101u;
+        mfma_v4i ea[2], eb[4];
+#pragma unroll
+        for (int fa = 0; fa < 2; ++fa) {
+          ea[fa][0] = (int)(xa[fa].x & M);
+          ea[fa][1] = (int)(xa[fa].y & M);
+          ea[fa][2] = (int)(xa[fa].z & M);
+          ea[fa][3] = (int)(xa[fa].w & M);
+          xa[fa].x >>= 1;
+          xa[fa].y >>= 1;
+          xa[fa].z >>= 1;
+          xa[fa].w >>= 1;
+        }
+#pragma unroll
+        for (int fb = 0; fb < 4; ++fb) {
+          eb[fb][0] = (int)(xb[fb].x & M);
+          eb[fb][1] = (int)(xb[fb].y & M);
+          eb[fb][2] = (int)(xb[fb].z & M);
+          eb[fb][3] = (int)(xb[fb].w & M);
+          xb[fb].x >>= 1;
+          xb[fb].y >>= 1;
+          xb[fb].z >>= 1;
+          xb[fb].w >>= 1;
+        }
+#pragma unroll
+        for (int fa = 0; fa < 2; ++fa)
+#pragma unroll
+          for (int fb = 0; fb < 4; ++fb)
+            acc[fa][fb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ea[fa], eb[fb], acc[fa][fb], 0, 0, 0);
+      }
     }
   }
 
-  // ---- stage the repeat lists in LDS when all 256 fit
+  // ---- stage the repeat lists in LDS when all 256 fit (thread t: reads 2t, 2t+1)
   bool staged = false;
   if (!CAND) {
-    const int own = sRc[tid];
-    int incl = own;
+    const int c0 = sRc[2 * tid], c1 = sRc[2 * tid + 1];
+    int incl = c0 + c1;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
       const int v = __shfl_up(incl, off, 64);
-      if ((tid & 63) >= off) incl += v;
+      if (lane >= off) incl += v;
     }
-    if ((tid & 63) == 63) sWave[tid >> 6] = incl;
-    __syncthreads();  // also: every thread is done with As / Bs
-    int before = 0;
-    for (int w = 0; w < (tid >> 6); ++w) before += sWave[w];
-    staged = sWave[0] + sWave[1] + sWave[2] + sWave[3] <= kStageWords;
+    if (lane == 63) sWave[wv] = incl;
+    __syncthreads();  // also: both waves are done with the chunk buffer
+    const int before = wv ? sWave[0] : 0;
+    staged = sWave[0] + sWave[1] <= kStageWords;
     if (staged) {
-      const int o = before + incl - own;
-      sLo[tid] = o;
-      const uint32_t *src = rlist + sRo[tid];
-      for (int e = 0; e < own; ++e) smem[o + e] = src[e];
+      const int o = before + incl - c0 - c1;
+      sLo[2 * tid] = o;
+      sLo[2 * tid + 1] = o + c0;
+      const uint32_t *src0 = rlist + sRo[2 * tid];
+      for (int e = 0; e < c0; ++e) Ps[o + e] = src0[e];
+      const uint32_t *src1 = rlist + sRo[2 * tid + 1];
+      for (int e = 0; e < c1; ++e) Ps[o + c0 + e] = src1[e];
     }
     __syncthreads();
   }
@@ -314,43 +367,44 @@ __global__ __launch_bounds__(256, 4) void kmer_tile_kernel(
     const int na = sRc[li], nbr = sRc[lj];
     if (na > 0 && nbr > 0) {
       if (staged)
-        pop += repeat_min_sum(smem + sLo[li], na, smem + sLo[lj], nbr);
+        pop += repeat_min_sum(Ps + sLo[li], na, Ps + sLo[lj], nbr);
       else
         pop += repeat_min_sum(rlist + sRo[li], na, rlist + sRo[lj], nbr);
     }
     return pop;
   };
 
-  // ---- epilogue: bound, exact count, filter, compaction
-  uint64_t passmask = 0;
+  // ---- epilogue: bound, exact count, filter, compaction (128 pairs per lane,
+  // compile-time register indices); a kept pair's count replaces its popcount
+  uint64_t passmask[2] = {0, 0};
   int mycount = 0;
-#pragma unroll
-  for (int x = 0; x < 8; ++x) {
-    const int li = x < 4 ? ty * 4 + x : 64 + ty * 4 + (x - 4);
-    const int i = row0 + li;
-#pragma unroll
-    for (int y = 0; y < 8; ++y) {
-      const int lj = kTile + (y < 4 ? tx * 4 + y : 64 + tx * 4 + (y - 4));
-      const int j = col0 + lj - kTile;
+  static_for<8>([&](auto F) {
+    constexpr int fa = decltype(F)::value / 4, fb = decltype(F)::value % 4;
+    static_for<16>([&](auto R) {
+      constexpr int r = decltype(R)::value;
+      constexpr int p = (fa * 4 + fb) * 16 + r;
+      const int li = wv * 64 + fa * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
+      const int lj = kTile + fb * 32 + r32;
+      const int i = row0 + li, j = col0 + lj - kTile;
       if (i < j && j < n) {
         const int tmin = min(sT[li], sT[lj]);
-        const uint32_t ub = acc[x][y] + (uint32_t)min(sE[li], sE[lj]);
-        if ((float)ub < (float)(tmin - tshift) * cut - 1.0f) continue;
-        bool keep = CAND;
-        if (!CAND) {
-          const uint32_t m = matches_of(li, lj, acc[x][y]);
-          keep = passes(1.0f - (float)m / (float)tmin, thr);
-          acc[x][y] = m;
-        }
-        if (keep) {
-          passmask |= 1ull << (x * 8 + y);
-          ++mycount;
+        const uint32_t pop = (uint32_t)acc[fa][fb][r];
+        const uint32_t ub = pop + (uint32_t)min(sE[li], sE[lj]);
+        if ((float)ub >= (float)(tmin - tshift) * cut - 1.0f) {
+          bool keep = CAND;
+          if (!CAND) {
+            const uint32_t m = matches_of(li, lj, pop);
+            keep = passes(1.0f - (float)m / (float)tmin, thr);
+            acc[fa][fb][r] = (int)m;
+          }
+          if (keep) {
+            passmask[p >> 6] |= 1ull << (p & 63);
+            ++mycount;
+          }
         }
       }
-    }
-  }
-  // wave-level exclusive scan of the per-thread counts, one atomic per wave
-  const int lane = tid & 63;
+    });
+  });
   int incl = mycount;
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1) {
@@ -365,23 +419,22 @@ __global__ __launch_bounds__(256, 4) void kmer_tile_kernel(
   }
   if (mycount == 0) return;
   int64_t pos = (int64_t)base + (incl - mycount);
-#pragma unroll
-  for (int x = 0; x < 8; ++x) {
-    const int li = x < 4 ? ty * 4 + x : 64 + ty * 4 + (x - 4);
-    const int i = row0 + li;
-#pragma unroll
-    for (int y = 0; y < 8; ++y) {
-      if (passmask & (1ull << (x * 8 + y))) {
-        const int lj = kTile + (y < 4 ? tx * 4 + y : 64 + tx * 4 + (y - 4));
-        const int j = col0 + lj - kTile;
+  static_for<8>([&](auto F) {
+    constexpr int fa = decltype(F)::value / 4, fb = decltype(F)::value % 4;
+    static_for<16>([&](auto R) {
+      constexpr int r = decltype(R)::value;
+      constexpr int p = (fa * 4 + fb) * 16 + r;
+      if (passmask[p >> 6] & (1ull << (p & 63))) {
+        const int li = wv * 64 + fa * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
+        const int lj = kTile + fb * 32 + r32;
         if (pos < cap) {
-          out_keys[pos] = ((uint64_t)i << 20) | (uint64_t)j;
-          if (!CAND) out_dists[pos] = 1.0f - (float)acc[x][y] / (float)min(sT[li], sT[lj]);
+          out_keys[pos] = ((uint64_t)(row0 + li) << 20) | (uint64_t)(col0 + lj - kTile);
+          if (!CAND) out_dists[pos] = 1.0f - (float)acc[fa][fb][r] / (float)min(sT[li], sT[lj]);
         }
         ++pos;
       }
-    }
-  }
+    });
+  });
 }
 
 __global__ void unpack_keys_kernel(const uint64_t *__restrict__ keys, int64_t n, int2 *__restrict__ pairs) {
@@ -396,7 +449,7 @@ __global__ void unpack_keys_kernel(const uint64_t *__restrict__ keys, int64_t n,
 // Every shared K-mer occurrence (K > 6) starts with a shared 6-mer at the same
 // position, so matches_K <= I_K <= I_6 <= popcount_6 + min(excess_6): the
 // k = 6 bitmap tiles, bounded against T_K = T_6 - (K - 6) and the K
-// threshold, give a candidate superset (kmer_tile_kernel<6, true>).  The
+// threshold, give a candidate superset (kmer_tile_mfma_kernel<6, true>).  The
 // candidates then run the reference's merge (kmerCPU.cpp:29-43) over their
 // K-mer lists sorted as signed short (kmerMain.cpp:156-159) -- its K = 8
 // ordering included -- for the exact count.
@@ -521,7 +574,7 @@ static int launch_tiles(crispy_ctx *ctx, int32_t n, int nb, int64_t tb, int64_t 
   const int64_t grid = per_xcd * 8;
   hipEvent_t ev;
   timer_begin(ctx, "kmer_tile", &ev);
-  kmer_tile_kernel<K, CAND><<<(unsigned)grid, 256, 0, ctx->stream>>>(
+  kmer_tile_mfma_kernel<K, CAND><<<(unsigned)grid, 128, 0, ctx->stream>>>(
       km.bitmap.as<uint32_t>(), km.T.as<int32_t>(), km.rcnt.as<int32_t>(), km.roff.as<int64_t>(),
       km.rlist.as<uint32_t>(), km.excess.as<int32_t>(), n, nb, tb, nt,
       per_xcd, thr, 1.0f - thr - 2e-5f, tshift, km.counter.as<unsigned long long>(), out_keys,
