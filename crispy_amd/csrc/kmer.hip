@@ -177,14 +177,16 @@ __device__ __forceinline__ void tile_coords(int64_t t, int nb, int &bi, int &bj)
 
 // ------------------------------------------------------ matrix-core tiles
 // One 128x128 tile of read pairs per block.  The popcount product runs on the
-// matrix cores: popcount(a AND b) = sum_k a_k * b_k over {0,1} bytes, one
-// v_mfma_i32_32x32x32_i8 per 32 x 32 pairs x 32 codes.  Operand lanes (checked
-// on the device): lane l holds row (A) / column (B) l % 32 and 16 k-slots, the
-// same 16 for A and B; accumulator r of lane l is pair (8(r/4) + 4(l/32) + r%4,
-// l % 32).  Which 16 of a step's codes a lane holds is free as long as A and B
-// agree, so for an 8-word group lane half g takes words 4g..4g+3 and step s
-// (bit plane) expands each word as (w >> s) & 0x01010101 -- 2 VALU ops per
-// 4 k-slots, 48 per 8 MFMAs.
+// matrix cores: popcount(a AND b) = sum_k a_k * b_k over {0,1} codes, as
+// v_mfma_scale_f32_32x32x64_f8f6f4 with both operands FP4 (E2M1) and unit
+// scales: a set bit becomes the nibble 0b0001 = 0.5, so each product is 0.25
+// and the fp32 accumulator holds popcount / 4 exactly (at most 1024).  Operand
+// lanes (checked on the device): lane l holds row (A) / column (B) l % 32 and
+// 32 k-slots, the same 32 for A and B; accumulator r of lane l is pair
+// (8(r/4) + 4(l/32) + r%4, l % 32).  Which 32 of a step's 64 codes a lane holds
+// is free as long as A and B agree, so for an 8-word group lane half g takes
+// words 4g..4g+3 and step s (bit plane) expands each word as
+// (w >> s) & 0x11111111 -- 2 VALU ops per 8 k-slots, 48 per 8 MFMAs.
 // Block: 2 waves, wave w owns tile rows [64w, 64w + 64) x all 128 columns
 // (2 x 4 fragments, 128 accumulators per lane).  Bitmap chunks of 16 words are
 // staged row-major (padded) in LDS, the next chunk prefetched into registers;
@@ -208,8 +210,10 @@ __device__ __forceinline__ void static_for(F &&f) {
   static_for_impl(f, std::make_integer_sequence<int, N>{});
 }
 
-typedef int mfma_v4i __attribute__((ext_vector_type(4)));
-typedef int mfma_v16i __attribute__((ext_vector_type(16)));
+typedef int mfma_v8i __attribute__((ext_vector_type(8)));
+typedef float mfma_v16f __attribute__((ext_vector_type(16)));
+constexpr int kFp4 = 4;        // f8f6f4 operand format code of FP4 (E2M1)
+constexpr int kUnitScale = 127;  // E8M0 scale 2^0
 
 template <int K, bool CAND>
 __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
@@ -265,13 +269,13 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
     pb[q] = masked(gb[q], bmask);
   }
 
-  mfma_v16i acc[2][4];
+  mfma_v16f acc[2][4];
 #pragma unroll
   for (int fa = 0; fa < 2; ++fa)
 #pragma unroll
     for (int fb = 0; fb < 4; ++fb)
 #pragma unroll
-      for (int r = 0; r < 16; ++r) acc[fa][fb][r] = 0;
+      for (int r = 0; r < 16; ++r) acc[fa][fb][r] = 0.0f;
 
   for (int ch = 0; ch < NCH; ++ch) {
     __syncthreads();
@@ -304,15 +308,16 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
       // one bit plane per step; the words shift in place so the planes are
       // not all expanded (and live) at once
 #pragma unroll 1
-      for (int s = 0; s < 8; ++s) {
-        constexpr uint32_t M = 0x01010101u;
-        mfma_v4i ea[2], eb[4];
+      for (int s = 0; s < 4; ++s) {
+        constexpr uint32_t M = 0x11111111u;
+        mfma_v8i ea[2], eb[4];
 #pragma unroll
         for (int fa = 0; fa < 2; ++fa) {
           ea[fa][0] = (int)(xa[fa].x & M);
           ea[fa][1] = (int)(xa[fa].y & M);
           ea[fa][2] = (int)(xa[fa].z & M);
           ea[fa][3] = (int)(xa[fa].w & M);
+          ea[fa][4] = ea[fa][5] = ea[fa][6] = ea[fa][7] = 0;  // unused by FP4 operands
           xa[fa].x >>= 1;
           xa[fa].y >>= 1;
           xa[fa].z >>= 1;
@@ -324,6 +329,7 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
           eb[fb][1] = (int)(xb[fb].y & M);
           eb[fb][2] = (int)(xb[fb].z & M);
           eb[fb][3] = (int)(xb[fb].w & M);
+          eb[fb][4] = eb[fb][5] = eb[fb][6] = eb[fb][7] = 0;
           xb[fb].x >>= 1;
           xb[fb].y >>= 1;
           xb[fb].z >>= 1;
@@ -333,10 +339,20 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
         for (int fa = 0; fa < 2; ++fa)
 #pragma unroll
           for (int fb = 0; fb < 4; ++fb)
-            acc[fa][fb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ea[fa], eb[fb], acc[fa][fb], 0, 0, 0);
+            acc[fa][fb] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
+                ea[fa], eb[fb], acc[fa][fb], kFp4, kFp4, 0, kUnitScale, 0, kUnitScale);
       }
     }
   }
+
+  // popcounts (exact: the accumulators hold multiples of 1/4 below 2^10)
+  uint32_t cnt[2][4][16];
+#pragma unroll
+  for (int fa = 0; fa < 2; ++fa)
+#pragma unroll
+    for (int fb = 0; fb < 4; ++fb)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) cnt[fa][fb][r] = (uint32_t)(acc[fa][fb][r] * 4.0f);
 
   // ---- stage the repeat lists in LDS when all 256 fit (thread t: reads 2t, 2t+1)
   bool staged = false;
@@ -388,14 +404,14 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
       const int i = row0 + li, j = col0 + lj - kTile;
       if (i < j && j < n) {
         const int tmin = min(sT[li], sT[lj]);
-        const uint32_t pop = (uint32_t)acc[fa][fb][r];
+        const uint32_t pop = cnt[fa][fb][r];
         const uint32_t ub = pop + (uint32_t)min(sE[li], sE[lj]);
         if ((float)ub >= (float)(tmin - tshift) * cut - 1.0f) {
           bool keep = CAND;
           if (!CAND) {
             const uint32_t m = matches_of(li, lj, pop);
             keep = passes(1.0f - (float)m / (float)tmin, thr);
-            acc[fa][fb][r] = (int)m;
+            cnt[fa][fb][r] = m;
           }
           if (keep) {
             passmask[p >> 6] |= 1ull << (p & 63);
@@ -429,7 +445,7 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
         const int lj = kTile + fb * 32 + r32;
         if (pos < cap) {
           out_keys[pos] = ((uint64_t)(row0 + li) << 20) | (uint64_t)(col0 + lj - kTile);
-          if (!CAND) out_dists[pos] = 1.0f - (float)acc[fa][fb][r] / (float)min(sT[li], sT[lj]);
+          if (!CAND) out_dists[pos] = 1.0f - (float)cnt[fa][fb][r] / (float)min(sT[li], sT[lj]);
         }
         ++pos;
       }
