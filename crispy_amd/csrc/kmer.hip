@@ -235,6 +235,7 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
   __shared__ int32_t sE[2 * kTile];
   __shared__ int32_t sLo[2 * kTile];
   __shared__ int32_t sWave[2];
+  __shared__ uint32_t sQ[CAND ? 1 : 2][CAND ? 1 : 1024];  // per-wave merge queue
 
   const int64_t bid = blockIdx.x;
   const int64_t local = (bid & 7) * per_xcd + (bid >> 3);
@@ -372,30 +373,29 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
       const int o = before + incl - c0 - c1;
       sLo[2 * tid] = o;
       sLo[2 * tid + 1] = o + c0;
+      // the two lists back to back; independent loads, unrolled so they overlap
       const uint32_t *src0 = rlist + sRo[2 * tid];
-      for (int e = 0; e < c0; ++e) Ps[o + e] = src0[e];
       const uint32_t *src1 = rlist + sRo[2 * tid + 1];
-      for (int e = 0; e < c1; ++e) Ps[o + c0 + e] = src1[e];
+#pragma unroll 8
+      for (int e = 0; e < c0 + c1; ++e) Ps[o + e] = e < c0 ? src0[e] : src1[e - c0];
     }
     __syncthreads();
   }
-  auto matches_of = [&](int li, int lj, uint32_t pop) -> uint32_t {
-    const int na = sRc[li], nbr = sRc[lj];
-    if (na > 0 && nbr > 0) {
-      if (staged)
-        pop += repeat_min_sum(Ps + sLo[li], na, Ps + sLo[lj], nbr);
-      else
-        pop += repeat_min_sum(rlist + sRo[li], na, rlist + sRo[lj], nbr);
-    }
-    return pop;
-  };
 
   // ---- epilogue: bound, exact count, filter, compaction (128 pairs per lane,
-  // compile-time register indices); a kept pair's count replaces its popcount
+  // compile-time register indices).  Pairs the bound keeps whose reads both
+  // have repeats need a merge: they go to the wave's LDS queue, fragment by
+  // fragment, the whole wave merges them 64 at a time (a lane-per-pair merge
+  // would keep the wave busy for every lane's near pair) and writes (count,
+  // pass) back to the slot, and the owning lanes take the results back.  The
+  // other kept pairs are decided from the popcount alone.  One atomic per
+  // wave reserves the output.
   uint64_t passmask[2] = {0, 0};
   int mycount = 0;
+  uint32_t *queue = sQ[wv];
   static_for<8>([&](auto F) {
     constexpr int fa = decltype(F)::value / 4, fb = decltype(F)::value % 4;
+    uint32_t nd = 0;
     static_for<16>([&](auto R) {
       constexpr int r = decltype(R)::value;
       constexpr int p = (fa * 4 + fb) * 16 + r;
@@ -409,9 +409,10 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
         if ((float)ub >= (float)(tmin - tshift) * cut - 1.0f) {
           bool keep = CAND;
           if (!CAND) {
-            const uint32_t m = matches_of(li, lj, pop);
-            keep = passes(1.0f - (float)m / (float)tmin, thr);
-            cnt[fa][fb][r] = m;
+            if (sRc[li] > 0 && sRc[lj] > 0)
+              nd |= 1u << r;
+            else
+              keep = passes(1.0f - (float)pop / (float)tmin, thr);
           }
           if (keep) {
             passmask[p >> 6] |= 1ull << (p & 63);
@@ -420,7 +421,57 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
         }
       }
     });
+    if (CAND) return;
+    const int c = __popc(nd);
+    int incl = c;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int v = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += v;
+    }
+    const int qn = __shfl(incl, 63, 64);
+    if (qn == 0) return;  // wave-uniform
+    int o = incl - c;
+    static_for<16>([&](auto R) {
+      constexpr int r = decltype(R)::value;
+      if (nd & (1u << r)) {
+        const int li = wv * 64 + fa * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
+        queue[o++] = (uint32_t)li | ((uint32_t)(fb * 32 + r32) << 7) | (cnt[fa][fb][r] << 14);
+      }
+    });
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    for (int e = lane; e < qn; e += 64) {
+      const uint32_t q = queue[e];
+      const int li = (int)(q & 127u);
+      const int lj = kTile + (int)((q >> 7) & 127u);
+      const int na = sRc[li], nbr = sRc[lj];
+      const uint32_t m = (q >> 14) + (staged ? repeat_min_sum(Ps + sLo[li], na, Ps + sLo[lj], nbr)
+                                             : repeat_min_sum(rlist + sRo[li], na, rlist + sRo[lj], nbr));
+      const bool ok = passes(1.0f - (float)m / (float)min(sT[li], sT[lj]), thr);
+      queue[e] = (m << 1) | (ok ? 1u : 0u);  // back to the owner
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    o = incl - c;
+    static_for<16>([&](auto R) {
+      constexpr int r = decltype(R)::value;
+      constexpr int p = (fa * 4 + fb) * 16 + r;
+      if (nd & (1u << r)) {
+        const uint32_t v = queue[o++];
+        if (v & 1u) {
+          passmask[p >> 6] |= 1ull << (p & 63);
+          ++mycount;
+          cnt[fa][fb][r] = v >> 1;
+        }
+      }
+    });
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   });
+  auto emit = [&](int64_t pos, int li, int lj, uint32_t m) {
+    if (pos < cap) {
+      out_keys[pos] = ((uint64_t)(row0 + li) << 20) | (uint64_t)(col0 + lj - kTile);
+      if (!CAND) out_dists[pos] = 1.0f - (float)m / (float)min(sT[li], sT[lj]);
+    }
+  };
   int incl = mycount;
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1) {
@@ -442,11 +493,7 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
       constexpr int p = (fa * 4 + fb) * 16 + r;
       if (passmask[p >> 6] & (1ull << (p & 63))) {
         const int li = wv * 64 + fa * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
-        const int lj = kTile + fb * 32 + r32;
-        if (pos < cap) {
-          out_keys[pos] = ((uint64_t)(row0 + li) << 20) | (uint64_t)(col0 + lj - kTile);
-          if (!CAND) out_dists[pos] = 1.0f - (float)cnt[fa][fb][r] / (float)min(sT[li], sT[lj]);
-        }
+        emit(pos, li, kTile + fb * 32 + r32, cnt[fa][fb][r]);
         ++pos;
       }
     });
