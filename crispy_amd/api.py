@@ -71,6 +71,16 @@ def load_library():
         "crispy_timing_reset": ([_vp], ctypes.c_int),
         "crispy_timing_query": ([_vp, ctypes.c_char_p, _f64p, _i64p, _f64p], ctypes.c_int),
         "crispy_selftest_division": ([_vp, ctypes.c_int32, _f32p, _f32p], ctypes.c_int),
+        "crispy_preprocess_read": ([ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(_vp),
+                                    ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                    ctypes.POINTER(_vp)], ctypes.c_int32),
+        "crispy_preprocess_length_filter": ([_i64p, ctypes.c_char_p, ctypes.c_int32, ctypes.c_double,
+                                             _i32p], ctypes.c_int32),
+        "crispy_preprocess_order": ([_vp, ctypes.c_char_p, _i64p, _i32p, ctypes.c_int32, _i32p, _i32p,
+                                     _i32p], ctypes.c_int),
+        "crispy_preprocess_write": ([ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, _i64p,
+                                     ctypes.c_char_p, _i64p, _i32p, _i32p, ctypes.c_int32],
+                                    ctypes.c_int),
         "crispy_server_create": ([ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.POINTER(_vp)],
                                  ctypes.c_int),
         "crispy_server_destroy": ([_vp], None),
@@ -141,6 +151,62 @@ def read_fasta(path, variant=KMER):
         L.crispy_free(b)
         L.crispy_free(o)
     return Reads(bases, offs, ml.value)
+
+
+class Records:
+    """Records of the preprocess reader: sequences (Reads), labels (bytes +
+    offsets, whitespace removed) and a frequency per record."""
+
+    def __init__(self, reads, labels, label_offsets, freq):
+        self.reads = reads
+        self.labels = bytes(labels)
+        self.label_offsets = np.ascontiguousarray(label_offsets, dtype=np.int64)
+        self.freq = np.ascontiguousarray(freq, dtype=np.int32)
+
+    @property
+    def n(self):
+        return self.reads.n
+
+    def label(self, i):
+        return self.labels[self.label_offsets[i]:self.label_offsets[i + 1]]
+
+
+def preprocess_read(path, fastq=False):
+    """The reference preprocess reader (preprocess.cpp:74-240)."""
+    L = load_library()
+    b, o, lb, lo, fr = _vp(), _vp(), _vp(), _vp(), _vp()
+    n = L.crispy_preprocess_read(os.fsencode(path), int(bool(fastq)), ctypes.byref(b), ctypes.byref(o),
+                                 ctypes.byref(lb), ctypes.byref(lo), ctypes.byref(fr))
+    if n < 0:
+        raise CrispyError(n, f"cannot read {path}")
+    try:
+        offs = np.ctypeslib.as_array(ctypes.cast(o, _i64p), shape=(n + 1,)).copy()
+        loffs = np.ctypeslib.as_array(ctypes.cast(lo, _i64p), shape=(n + 1,)).copy()
+        freq = np.ctypeslib.as_array(ctypes.cast(fr, _i32p), shape=(max(n, 1),)).copy()[:n]
+        bases = ctypes.string_at(b, int(offs[-1])) if offs[-1] > 0 else b""
+        labels = ctypes.string_at(lb, int(loffs[-1])) if loffs[-1] > 0 else b""
+    finally:
+        for p in (b, o, lb, lo, fr):
+            L.crispy_free(p)
+    return Records(Reads(bases, offs), labels, loffs, freq)
+
+
+def preprocess_length_filter(rec: Records, sd_allowed):
+    """-s filter (preprocess.cpp:322-392): zeroes the frequency of outliers and
+    reads with N in place; returns the number discarded."""
+    return int(load_library().crispy_preprocess_length_filter(
+        _p(rec.reads.offsets, _i64p), rec.reads.bases, rec.n, float(sd_allowed), _p(rec.freq, _i32p)))
+
+
+def preprocess_write(path, rec: Records, order, freq, fastq=False):
+    """writeToFile (preprocess.cpp:562-601): <path> and <path>.frq."""
+    order = np.ascontiguousarray(order, dtype=np.int32)
+    freq = np.ascontiguousarray(freq, dtype=np.int32)
+    rc = load_library().crispy_preprocess_write(
+        os.fsencode(path), int(bool(fastq)), rec.reads.bases, _p(rec.reads.offsets, _i64p), rec.labels,
+        _p(rec.label_offsets, _i64p), _p(order, _i32p), _p(freq, _i32p), len(order))
+    if rc != 0:
+        raise CrispyError(rc, f"cannot write {path}")
 
 
 def kmer_default_threshold(n):
@@ -245,6 +311,19 @@ class Context:
         self._check(self._L.crispy_sort_by_dist(self._h, _p(pairs, _i32p), _p(dists, _f32p),
                                                 len(dists)))
         return pairs, dists
+
+    # -- input side
+    def preprocess_order(self, reads: Reads, freq):
+        """mergeReads on the device: (order, frequencies) of the output reads."""
+        freq = np.ascontiguousarray(freq, dtype=np.int32)
+        n = reads.n
+        order = np.zeros(max(n, 1), np.int32)
+        ofreq = np.zeros(max(n, 1), np.int32)
+        c = ctypes.c_int32()
+        self._check(self._L.crispy_preprocess_order(self._h, reads.bases, _p(reads.offsets, _i64p),
+                                                    _p(freq, _i32p), n, _p(order, _i32p),
+                                                    _p(ofreq, _i32p), ctypes.byref(c)))
+        return order[:c.value].copy(), ofreq[:c.value].copy()
 
     # -- timing
     def timing(self, enable=True):

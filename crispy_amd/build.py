@@ -2,7 +2,8 @@
 
   crispy_amd/lib/libcrispy_hip.so   HIP kernels + the C ABI (include/crispy_hip.h),
                                     including the job server
-  bin/kmerDist, bin/genDist         the reference's command lines over that ABI
+  bin/preprocess, bin/kmerDist,     the reference's command lines over that ABI
+  bin/genDist
 
 Usage: python -m crispy_amd.build   (or crispy_amd.build.build()).
 """
@@ -25,10 +26,10 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["api.hip", "kmer.hip", "nw.hip", "nw_fam_reg0.hip", "nw_fam_reg1.hip",
+HIP_SOURCES = ["api.hip", "kmer.hip", "nw.hip", "preprocess.hip", "nw_fam_reg0.hip", "nw_fam_reg1.hip",
                "nw_fam_reg2.hip", "nw_fam_reg3.hip", "nw_fam_sliced.hip"]
 HOST_SOURCES = ["host.cpp", "serve.cpp"]
-CLI = {"kmerDist": "cli_kmer.cpp", "genDist": "cli_gen.cpp"}
+CLI = {"kmerDist": "cli_kmer.cpp", "genDist": "cli_gen.cpp", "preprocess": "cli_preprocess.cpp"}
 
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -213,7 +213,10 @@ static void release_all(crispy_ctx *ctx) {
                     &ctx->nw.in_pairs, &ctx->nw.keys,      &ctx->nw.keys_alt,  &ctx->nw.idx,
                     &ctx->nw.idx_alt,  &ctx->nw.dist,      &ctx->nw.hist,      &ctx->nw.cells,
                     &ctx->nw.flags,    &ctx->nw.sel,       &ctx->nw.nsel,      &ctx->nw.out_pairs,
-                    &ctx->nw.out_dists, &ctx->nw.scratch,  &ctx->nw.tasks,     &ctx->temp};
+                    &ctx->nw.out_dists, &ctx->nw.scratch,  &ctx->nw.tasks,     &ctx->temp,
+                    &ctx->pre.bases,   &ctx->pre.off,      &ctx->pre.freq,     &ctx->pre.keys,
+                    &ctx->pre.keys_alt, &ctx->pre.flags,   &ctx->pre.sel,      &ctx->pre.nsel,
+                    &ctx->pre.csum,    &ctx->pre.out_order, &ctx->pre.out_freq};
   for (DevBuf *b : bufs) release(ctx, *b);
   for (auto &r : ctx->reads) {
     DevBuf *rb[] = {&r.bases, &r.off, &r.len, &r.pack, &r.poff, &r.clean};
@@ -256,6 +259,7 @@ int crispy_ctx_trim(crispy_ctx *ctx) {
   }
   ctx->km = KmerState();
   ctx->nw = NwState();
+  ctx->pre = PreState();
   return CRISPY_OK;
 }
 

@@ -115,6 +115,11 @@ struct NwState {
   bool valid = false;
 };
 
+// Input side (preprocess-equivalent ordering and merge).
+struct PreState {
+  DevBuf bases, off, freq, keys, keys_alt, flags, sel, nsel, csum, out_order, out_freq;
+};
+
 struct KStat {
   double ms = 0;
   int64_t launches = 0;
@@ -141,6 +146,7 @@ struct crispy_ctx {
   ReadStore reads[2];
   KmerState km;
   NwState nw;
+  PreState pre;
   DevBuf temp;  // rocPRIM temporary storage
   void *h_stage = nullptr;  // pinned host staging for small async uploads
   size_t h_stage_bytes = 0;

@@ -77,3 +77,18 @@ def compute_gen_dist(ctx: Context, reads: Reads, in_pair_path: str, pair_prefix:
         total += nacc
         flush()
     return nfiles, total
+
+
+def preprocess(ctx: Context, in_path: str, out_path: str = None, fastq: bool = False,
+               sd_allowed: float = None):
+    """The reference preprocess main (preprocess.cpp:686-726) without primer
+    trimming: read, optional -s length filter, merge exact duplicates on the
+    device, write <out> (default <in>_Clean) and <out>.frq.  Returns the
+    number of reads written."""
+    from .api import preprocess_length_filter, preprocess_read, preprocess_write
+    rec = preprocess_read(in_path, fastq)
+    if sd_allowed is not None:
+        preprocess_length_filter(rec, sd_allowed)
+    order, freq = ctx.preprocess_order(rec.reads, rec.freq)
+    preprocess_write(out_path or (in_path + "_Clean"), rec, order, freq, fastq)
+    return len(order)

@@ -12,6 +12,7 @@
  *   launchGenKernel_band   genKernel.h:48 (and _full, :46)
  *   writeVectorToFile_GPU  genCUDA.cu:17-81 (its thrust::sort_by_key)
  *   readFile / thresholds  kmerMain.cpp:298-397,43-53; genMain.cpp:184-277,42-47
+ *   preprocess (input side) preprocess.cpp:74-280,322-392,562-601
  *
  * Errors: every entry point returns CRISPY_OK (0) or a negative code and
  * never exits the process (the reference calls exit(-1) from
@@ -154,6 +155,40 @@ int crispy_timing_query(crispy_ctx *ctx, const char *kernel, double *total_ms,
  * 0<=m<=maxv, and out_nw[a*(maxv+1)+m] = m/a likewise.  Lets a test compare
  * the device rounding with the host's bit for bit. */
 int crispy_selftest_division(crispy_ctx *ctx, int32_t maxv, float *out_kmer, float *out_nw);
+
+/* ------------------------------------------------------------ input side */
+/* The reference's preprocess stage on the path crispy.sh runs
+ * (`preprocess -i <in>`, preprocess.cpp:686-726): read the records, merge
+ * exact duplicates, write <out> (default <in>_Clean) and <out>.frq in
+ * (length, strcmp) order.
+ *
+ * Reader of preprocess.cpp:74-240 (FASTA, or FASTQ with fastq != 0): records
+ * as concatenated sequence bytes + n+1 offsets, labels (every whitespace
+ * character removed, READ::initialize :31-39) + n+1 offsets, and a frequency
+ * per record (1; 0 for an empty FASTQ sequence).  Buffers are malloc'ed and
+ * released with crispy_free.  Returns n, CRISPY_ERR_IO, or CRISPY_ERR_ARG when a
+ * FASTA file does not start with '>'. */
+int32_t crispy_preprocess_read(const char *path, int fastq, char **bases, int64_t **offsets,
+                               char **labels, int64_t **label_offsets, int32_t **freq);
+/* The -s length filter (processReadLength, preprocess.cpp:322-392): sets the
+ * frequency of outliers and of reads with 'n'/'N' to 0; returns the count. */
+int32_t crispy_preprocess_length_filter(const int64_t *offsets, const char *bases, int32_t n,
+                                        double sd_allowed, int32_t *freq);
+/* mergeReads (preprocess.cpp:244-280) on the device: the reads with frequency
+ * > 0 in stable (length, strcmp) order (READ::operator< :42-49), each run of
+ * identical sequences collapsed onto its last read with the run's summed
+ * frequency.  order[k] / out_freq[k] (capacity n) receive the k-th output
+ * read's index and frequency; *count the number of output reads. */
+int crispy_preprocess_order(crispy_ctx *ctx, const char *bases, const int64_t *offsets,
+                            const int32_t *freq, int32_t n, int32_t *order, int32_t *out_freq,
+                            int32_t *count);
+/* writeToFile (preprocess.cpp:562-601): "> " + label without its first
+ * character (FASTQ: the whole label) and the sequence per output read, and
+ * "label freq" lines in <out_path>.frq. */
+int crispy_preprocess_write(const char *out_path, int fastq, const char *bases,
+                            const int64_t *offsets, const char *labels,
+                            const int64_t *label_offsets, const int32_t *order,
+                            const int32_t *freq, int32_t count);
 
 /* --------------------------------------------------------------- serving */
 /* No counterpart in the reference, which runs one sample per process

@@ -515,3 +515,196 @@ int oracle_gendist_files(const char *bases, const int64_t *offs, int32_t n,
 }
 
 int oracle_num_threads(void) { return omp_get_num_procs(); }
+
+/* ------------------------------------------------------------------------ */
+/* preprocess (input side): readFile preprocess.cpp:74-240, processReadLength */
+/* :322-392, mergeReads :244-280 with READ::operator< :42-49, writeToFile     */
+/* :562-601.                                                                 */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    char *seq, *label;
+    size_t len, llen;
+    int freq;
+    int idx;
+} or_rec;
+
+typedef struct {
+    or_rec *r;
+    int n, cap;
+} or_recs;
+
+/* READ::initialize (:31-39) strips every whitespace character of the label */
+static void or_rec_push(or_recs *R, const or_str *seq, const or_str *label, int freq) {
+    if (R->n == R->cap) {
+        R->cap = R->cap ? 2 * R->cap : 1024;
+        R->r = (or_rec *)realloc(R->r, sizeof(or_rec) * R->cap);
+    }
+    or_rec *x = &R->r[R->n];
+    x->len = seq->n;
+    x->seq = (char *)malloc(seq->n + 1);
+    memcpy(x->seq, seq->p ? seq->p : "", seq->n);
+    x->seq[seq->n] = 0;
+    x->label = (char *)malloc(label->n + 1);
+    x->llen = 0;
+    for (size_t k = 0; k < label->n; ++k) {
+        const char c = label->p[k];
+        if (!(c == ' ' || c == '\t' || c == '\n' || c == '\v' || c == '\f' || c == '\r'))
+            x->label[x->llen++] = c;
+    }
+    x->label[x->llen] = 0;
+    x->freq = freq;
+    x->idx = R->n;
+    ++R->n;
+}
+
+static void or_read_pre(FILE *f, int fastq, or_recs *R, int *bad) {
+    char buf[OR_BUF_SIZE];
+    or_str line = {0}, seq = {0}, label = {0};
+    or_str_set(&seq, "", 0);
+    *bad = 0;
+    if (!fastq) {
+        if (!fgets(buf, OR_BUF_SIZE, f)) buf[0] = 0;
+        if (buf[0] != '>') {
+            *bad = 1;
+            goto done;
+        }
+        or_str_set(&line, buf, strlen(buf));
+        or_trim_space(&line);
+        or_str_set(&label, line.p, line.n);
+        int end = 0;
+        while (!feof(f)) {
+            buf[0] = '$';
+            if (!fgets(buf, OR_BUF_SIZE, f)) { /* buf[0] stays '$' */ }
+            or_str_set(&line, buf, strlen(buf));
+            or_trim_space(&line);
+            const char c0 = line.n ? line.p[0] : 0;
+            if (c0 != '>' && c0 != '$') {
+                or_trim_invalid(&line, "nNATGCatgc");
+                or_str_append(&seq, line.p, line.n);
+                continue;
+            } else if (seq.n == 0) {
+                if (buf[0] == '$') { end = 1; break; }
+                continue;
+            } else {
+                or_trim_invalid(&seq, "nNATGCatgc");
+                or_rec_push(R, &seq, &label, 1);
+                seq.n = 0;
+                if (buf[0] == '$') { end = 1; break; }
+                or_str_set(&label, line.p, line.n);
+            }
+        }
+        if (!end && seq.n) {
+            or_trim_invalid(&seq, "nNATGCatgc");
+            or_rec_push(R, &seq, &label, 1);
+        }
+    } else {
+        while (!feof(f)) {
+            buf[0] = '$';
+            if (!fgets(buf, OR_BUF_SIZE, f) || feof(f) || buf[0] == '$') break;
+            or_str_set(&label, buf, strlen(buf));
+            or_trim_space(&label);
+            if (!fgets(buf, OR_BUF_SIZE, f) || feof(f) || buf[0] == '$') break;
+            or_str_set(&seq, buf, strlen(buf));
+            or_trim_space(&seq);
+            or_rec_push(R, &seq, &label, seq.n == 0 ? 0 : 1);
+            if (!fgets(buf, OR_BUF_SIZE, f) || feof(f) || buf[0] == '$') break;
+            if (!fgets(buf, OR_BUF_SIZE, f) || feof(f) || buf[0] == '$') break;
+        }
+    }
+done:
+    free(line.p);
+    free(seq.p);
+    free(label.p);
+}
+
+/* READ::operator< (length, then strcmp), ties by file position (list::sort
+ * is stable) */
+static int or_rec_cmp(const void *a, const void *b) {
+    const or_rec *x = (const or_rec *)a, *y = (const or_rec *)b;
+    if (x->len != y->len) return x->len < y->len ? -1 : 1;
+    const int c = strcmp(x->seq, y->seq);
+    if (c) return c;
+    return (x->idx > y->idx) - (x->idx < y->idx);
+}
+
+/* Runs the preprocess path on `in` and writes `out` and `out`.frq.  sd >= 0
+ * applies the -s filter.  Returns the number of reads written, -1 when a file
+ * cannot be opened, -2 when a FASTA input does not start with '>'. */
+int oracle_preprocess_file(const char *in, const char *out, int fastq, double sd) {
+    FILE *f = fopen(in, "rb");
+    if (!f) return -1;
+    or_recs R = {0};
+    int bad = 0;
+    or_read_pre(f, fastq, &R, &bad);
+    fclose(f);
+    int written = -2;
+    if (bad) goto free_all;
+    if (sd >= 0) { /* processReadLength */
+        double sum = 0, cnt = 0, sq = 0;
+        int minLen = 65535, maxLen = 0;
+        for (int k = 0; k < R.n; ++k)
+            if (R.r[k].freq) { sum += (double)R.r[k].len; cnt += 1; }
+        if (cnt >= 1) {
+            const double ave = sum / cnt;
+            for (int k = 0; k < R.n; ++k)
+                if (R.r[k].freq) {
+                    const double d = (double)R.r[k].len - ave;
+                    sq += d * d;
+                    if ((int)R.r[k].len < minLen) minLen = (int)R.r[k].len;
+                    if ((int)R.r[k].len > maxLen) maxLen = (int)R.r[k].len;
+                }
+            const double s = sqrt(sq / cnt);
+            const int lo = (int)(ave - sd * s), hi = (int)(ave + sd * s);
+            if (lo > minLen) minLen = lo;
+            if (hi < maxLen) maxLen = hi;
+            for (int k = 0; k < R.n; ++k) {
+                or_rec *x = &R.r[k];
+                if (!x->freq) continue;
+                if ((int)x->len < minLen || (int)x->len > maxLen) { x->freq = 0; continue; }
+                for (size_t q = 0; q < x->len; ++q)
+                    if (x->seq[q] == 'n' || x->seq[q] == 'N') { x->freq = 0; break; }
+            }
+        }
+    }
+    qsort(R.r, R.n, sizeof(or_rec), or_rec_cmp);
+    /* mergeReads: a run of equal sequences collapses onto its last member */
+    int prev = -1;
+    for (int k = 0; k < R.n; ++k) {
+        if (!R.r[k].freq) continue;
+        if (prev >= 0 && R.r[prev].len == R.r[k].len && strcmp(R.r[prev].seq, R.r[k].seq) == 0) {
+            R.r[k].freq += R.r[prev].freq;
+            R.r[prev].freq = 0;
+        }
+        prev = k;
+    }
+    {
+        char frq[4096];
+        snprintf(frq, sizeof frq, "%s.frq", out);
+        FILE *o = fopen(out, "wb"), *q = fopen(frq, "wb");
+        if (!o || !q) {
+            if (o) fclose(o);
+            if (q) fclose(q);
+            written = -1;
+            goto free_all;
+        }
+        written = 0;
+        for (int k = 0; k < R.n; ++k) {
+            const or_rec *x = &R.r[k];
+            if (!x->freq) continue;
+            const char *lab = fastq ? x->label : (x->llen ? x->label + 1 : x->label);
+            fprintf(o, "> %s\n%s\n", lab, x->seq);
+            fprintf(q, "%s %d\n", lab, x->freq);
+            ++written;
+        }
+        fclose(o);
+        fclose(q);
+    }
+free_all:
+    for (int k = 0; k < R.n; ++k) {
+        free(R.r[k].seq);
+        free(R.r[k].label);
+    }
+    free(R.r);
+    return written;
+}

@@ -62,6 +62,9 @@ def lib():
                                            ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p,
                                            _i64p]
         L.oracle_num_threads.restype = ctypes.c_int
+        L.oracle_preprocess_file.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
+                                             ctypes.c_double]
+        L.oracle_preprocess_file.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -196,6 +199,16 @@ def gendist_files(reads: Reads, pairs: np.ndarray, band: int, thr: float, prefix
     if nf < 0:
         raise IOError("oracle_gendist_files failed")
     return nf, tot.value
+
+
+def preprocess_file(in_path, out_path, fastq=False, sd=None):
+    """The reference preprocess path (preprocess.cpp:686-726, no primer step):
+    writes out_path and out_path.frq, returns the number of reads written."""
+    n = lib().oracle_preprocess_file(os.fsencode(in_path), os.fsencode(out_path), int(bool(fastq)),
+                                     -1.0 if sd is None else float(sd))
+    if n < 0:
+        raise IOError(f"oracle_preprocess_file({in_path}) failed: {n}")
+    return n
 
 
 def num_threads():

@@ -1,0 +1,113 @@
+"""Writes the input-side golden fixtures in tests/golden/pre/ with the
+reference's own preprocess, compiled from the reference sources into
+oracle/_ref by `make -C oracle ref`.
+
+Run from the repository root in a container that has the reference sources:
+    python tests/golden/make_golden_preprocess.py
+The fixtures are data only: the input files and the reference's outputs
+(<input>_Clean / .frq, and <input>_s1 / .frq for -s 1.0).
+"""
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+from crispy_amd import synth  # noqa: E402
+
+OUT = os.path.join(ROOT, "tests", "golden", "pre")
+REF = os.path.join(ROOT, "oracle", "_ref", "preprocess")
+
+
+def adversarial():
+    long = ("ACGT" * 1300).encode()  # a 5,200-byte line: read in 4095-byte pieces
+    recs = [
+        b">r1 first read\nACGTACGTAC\nGTACGTTT\n",
+        b">r2 duplicate of r1\nACGTACGTACGTACGTTT\n",
+        b">r3\ttab label \nacgtNNACGT  \n\nACGT\n",
+        b">empty1\n>empty2 header after header\nTTTTGGGG\n",
+        b">r4 crlf\r\nACGTACGTACGTACGTTT\r\n",
+        b">r5 trailing junk\nACGTXYZ\n",
+        b">r6 long line\n" + long + b"\n",
+        b">r7 same as r2\nACGTACGTACGTACGTTT\n",
+        b">r8 lowercase\nacgtacgtacgtacgtttt\n",
+        b">r9 short\nA\n",
+        b">r10 high bytes\nACG\xc3\xa9TT\n",
+        b">r11\nGGGGCCCC\n",
+        b">r12 blank inside\nGGGG\n   \nCCCC\n",
+        b">r13 same as r11\nGGGGCCCC\n",
+        b">r14 N only\nNNNN\n",
+        b">r15 dup of long\n" + long + b"\n",
+        b"$ a line starting with a dollar ends the file\n",
+        b">r16 never read\nACGT\n",
+    ]
+    return b"".join(recs)
+
+
+def raw_with_duplicates(n_unique=400, n=700, seed=21):
+    seqs, _ = synth.generate(n=n_unique, seed=seed, species=20, div=(0.02, 0.08),
+                             law=("lognormal", 300, 0.35, 150, 550), p_sub=0.01, p_indel=0.005)
+    rng = np.random.default_rng(seed)
+    pick = rng.integers(0, len(seqs), n)
+    out = []
+    for k, p in enumerate(pick):
+        s = bytearray(seqs[p])
+        u = rng.random()
+        if u < 0.05:
+            s[int(rng.integers(0, len(s)))] = ord("N")
+        elif u < 0.08:
+            s = bytearray(s.lower())
+        label = f">read_{k} sample={int(rng.integers(0, 9))}\tx"
+        body = bytes(s)
+        lines = [body[i:i + 80] for i in range(0, len(body), 80)]
+        out.append(label.encode() + b"\n" + b"\n".join(lines) + b"\n")
+    return b"".join(out)
+
+
+def fastq():
+    rng = np.random.default_rng(5)
+    base = [bytes(rng.choice(np.frombuffer(b"ACGT", np.uint8), int(L)).tobytes())
+            for L in rng.integers(20, 60, 40)]
+    out = []
+    for k in range(120):
+        s = base[int(rng.integers(0, len(base)))] if k != 50 else b""
+        out.append(b"@q%d run=1\n%s\n+\n%s\n" % (k, s, b"I" * len(s)))
+    return b"".join(out)
+
+
+def run(args, cwd):
+    r = subprocess.run(args, cwd=cwd, capture_output=True)
+    if r.returncode not in (0, 1):  # preprocess's main returns 1 (preprocess.cpp:725)
+        raise RuntimeError(f"{args}: {r.returncode} {r.stdout} {r.stderr}")
+
+
+def main():
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
+    os.makedirs(OUT, exist_ok=True)
+    tmp = tempfile.mkdtemp(prefix="golden_pre_")
+    try:
+        inputs = {"pre_adv.fa": adversarial(), "pre_dups.fa": raw_with_duplicates(),
+                  "pre_reads.fq": fastq()}
+        for name, data in inputs.items():
+            with open(os.path.join(OUT, name), "wb") as f:
+                f.write(data)
+            shutil.copyfile(os.path.join(OUT, name), os.path.join(tmp, name))
+        jobs = [(["-i", "pre_adv.fa"], "pre_adv.fa_Clean"),
+                (["-i", "pre_dups.fa"], "pre_dups.fa_Clean"),
+                (["-i", "pre_dups.fa", "-s", "1.0", "-o", "pre_dups.fa_s1"], "pre_dups.fa_s1"),
+                (["-i", "pre_reads.fq", "-f", "q"], "pre_reads.fq_Clean")]
+        for args, out in jobs:
+            run([REF] + args, tmp)
+            for suffix in ("", ".frq"):
+                shutil.copyfile(os.path.join(tmp, out + suffix), os.path.join(OUT, out + suffix))
+            print(out, sum(1 for _ in open(os.path.join(tmp, out + ".frq"), "rb")), "reads")
+    finally:
+        shutil.rmtree(tmp)
+
+
+if __name__ == "__main__":
+    main()

@@ -140,3 +140,50 @@ def test_server_job_estimate():
     assert full == 8 * min(256 * big.n, 32768) * (max(np.diff(big.offsets)) + 1)
     dirty = ca.Reads.from_list([seqs[0][:100] + b"N" + seqs[0][101:]] + seqs[1:])
     assert ca.job_bytes(dirty, 6, 10) - ca.job_bytes(big, 6, 10) == full
+
+
+PRE = os.path.join(GOLD, "pre")
+PRE_CASES = [("pre_adv.fa", "pre_adv.fa_Clean", False, None),
+             ("pre_dups.fa", "pre_dups.fa_Clean", False, None),
+             ("pre_dups.fa", "pre_dups.fa_s1", False, 1.0),
+             ("pre_reads.fq", "pre_reads.fq_Clean", True, None),
+             ("../V2Mice_sample1000", "../V2Mice_sample1000_Clean", False, None)]
+
+
+def py_merge(rec):
+    """Stable (length, bytes) order, runs collapsed onto their last read."""
+    seqs = [rec.reads.seq(i) for i in range(rec.n)]
+    keep = sorted((i for i in range(rec.n) if rec.freq[i] != 0), key=lambda i: (len(seqs[i]), seqs[i], i))
+    order, freq = [], []
+    for k, i in enumerate(keep):
+        if order and seqs[order[-1]] == seqs[i]:
+            order[-1] = i
+            freq[-1] += int(rec.freq[i])
+        else:
+            order.append(i)
+            freq.append(int(rec.freq[i]))
+    return np.array(order, np.int32), np.array(freq, np.int32)
+
+
+@pytest.mark.parametrize("inp,out,fastq,sd", PRE_CASES)
+def test_preprocess_host_parts(tmp_path, inp, out, fastq, sd):
+    """The product's reader, -s filter and writer around a Python merge give
+    the reference's files (the device merge is checked on the GPU)."""
+    rec = ca.preprocess_read(os.path.join(PRE, inp), fastq)
+    if sd is not None:
+        ca.preprocess_length_filter(rec, sd)
+    order, freq = py_merge(rec)
+    o = str(tmp_path / "o_Clean")
+    ca.preprocess_write(o, rec, order, freq, fastq)
+    assert open(o, "rb").read() == open(os.path.join(PRE, out), "rb").read()
+    assert open(o + ".frq", "rb").read() == open(os.path.join(PRE, out) + ".frq", "rb").read()
+
+
+def test_preprocess_cli_error_paths(tmp_path):
+    exe = os.path.join(BIN, "preprocess")
+    assert subprocess.run([exe, "-h"], capture_output=True).returncode == 255
+    assert subprocess.run([exe, "-i", "x.fa", "-p", "primer.txt"], capture_output=True).returncode == 2
+    assert subprocess.run([exe, "-i", str(tmp_path / "missing.fa")], capture_output=True).returncode == 255
+    bad = tmp_path / "bad.fa"
+    bad.write_bytes(b"ACGT\n")
+    assert subprocess.run([exe, "-i", str(bad)], capture_output=True).returncode == 255

@@ -342,3 +342,31 @@ def test_gendist_file_split_golden(tmp_path):
         rec = np.zeros(len(c), dtype=[("d", "<u4"), ("i", "<u4"), ("j", "<u4")])
         rec["d"], rec["i"], rec["j"] = c[:, 0], c[:, 1], c[:, 2]
         assert hashlib.sha256(rec.tobytes()).hexdigest() == str(g[f"f{f}_sha256"])
+
+
+PRE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "pre")
+PRE_CASES = [("pre_adv.fa", "pre_adv.fa_Clean", False, None),
+             ("pre_dups.fa", "pre_dups.fa_Clean", False, None),
+             ("pre_dups.fa", "pre_dups.fa_s1", False, 1.0),
+             ("pre_reads.fq", "pre_reads.fq_Clean", True, None)]
+
+
+@pytest.mark.parametrize("inp,out,fastq,sd", PRE_CASES + [("../V2Mice_sample1000", "../V2Mice_sample1000_Clean", False, None)])
+def test_oracle_preprocess_matches_reference(tmp_path, inp, out, fastq, sd):
+    """The preprocess restatement against the reference binary's _Clean/.frq."""
+    o = str(tmp_path / "out_Clean")
+    O.preprocess_file(os.path.join(PRE, inp), o, fastq, sd)
+    assert open(o, "rb").read() == open(os.path.join(PRE, out), "rb").read()
+    assert open(o + ".frq", "rb").read() == open(os.path.join(PRE, out) + ".frq", "rb").read()
+
+
+def test_oracle_preprocess_merge_is_stable_last_wins(tmp_path):
+    """Closed-form check: duplicates collapse onto their last occurrence with
+    the run length as frequency, output in (length, bytes) order."""
+    fa = tmp_path / "d.fa"
+    recs = [("a", b"CCCC"), ("b", b"AAAA"), ("c", b"CCCC"), ("d", b"AC"), ("e", b"CCCC"), ("f", b"AAAA")]
+    fa.write_bytes(b"".join(b">" + l.encode() + b"\n" + s + b"\n" for l, s in recs))
+    o = str(tmp_path / "d_Clean")
+    assert O.preprocess_file(str(fa), o) == 3
+    assert open(o, "rb").read() == b"> d\nAC\n> f\nAAAA\n> e\nCCCC\n"
+    assert open(o + ".frq", "rb").read() == b"d 1\nf 2\ne 3\n"
