@@ -14,6 +14,7 @@ stream it runs on) and the CPU oracle timed on a bounded sample.
 """
 import argparse
 import json
+import shutil
 import os
 import sys
 import time
@@ -114,6 +115,57 @@ def describe_law(law):
     return f"log-normal median {law[1]} bp sigma {law[2]} clipped {law[3]}-{law[4]} bp"
 
 
+def input_side(args):
+    """Raw input of the workload: its reads drawn with replacement (about a
+    third duplicates), shuffled, labelled, 80-column FASTA.  Times the device
+    merge (sort + duplicate runs, HIP events) and, on the host, the reader,
+    the writer, and the oracle's whole preprocess (std qsort restatement)."""
+    import tempfile
+    import crispy_amd as ca
+    from crispy_amd import synth
+    from oracle import oracle as O
+    seqs, _, cfg = synth.make(args.workload)
+    rng = np.random.default_rng(99)
+    n = int(len(seqs) * 1.5)
+    pick = rng.integers(0, len(seqs), n)
+    tmp = tempfile.mkdtemp(prefix="crispy_input_")
+    raw = os.path.join(tmp, "raw.fa")
+    with open(raw, "wb") as f:
+        for k, p in enumerate(pick):
+            s = seqs[p]
+            f.write(b">read_%d sample\n" % k + b"\n".join(s[i:i + 80] for i in range(0, len(s), 80)) + b"\n")
+    t0 = time.perf_counter()
+    rec = ca.preprocess_read(raw)
+    t_read = time.perf_counter() - t0
+    ctx = ca.Context(0)
+    for _ in range(args.warmup):
+        ctx.preprocess_order(rec.reads, rec.freq)
+    ctx.timing(True)
+    ctx.timing_reset()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        order, freq = ctx.preprocess_order(rec.reads, rec.freq)
+    dt = (time.perf_counter() - t0) / args.steps
+    sort_ms = ctx.timing_query("pre_sort")[0] / args.steps
+    merge_ms = ctx.timing_query("pre_merge")[0] / args.steps
+    t0 = time.perf_counter()
+    ca.preprocess_write(os.path.join(tmp, "raw.fa_Clean"), rec, order, freq)
+    t_write = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    m = O.preprocess_file(raw, os.path.join(tmp, "oracle_Clean"))
+    t_oracle = time.perf_counter() - t0
+    same = (open(os.path.join(tmp, "raw.fa_Clean"), "rb").read() ==
+            open(os.path.join(tmp, "oracle_Clean"), "rb").read())
+    ctx.close()
+    print(json.dumps({"metric": "preprocess-equivalent merge, records/s", "value": rec.n / dt,
+                      "unit": "records/s", "records": rec.n, "unique": len(order), "oracle_unique": m,
+                      "identical_to_oracle": same, "ms_per_merge": dt * 1e3, "device_sort_ms": sort_ms,
+                      "device_runs_ms": merge_ms, "host_read_s": t_read, "host_write_s": t_write,
+                      "oracle_whole_preprocess_s": t_oracle, "workload": args.workload,
+                      "steps": args.steps, "warmup": args.warmup}))
+    shutil.rmtree(tmp)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,7 +178,12 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 1 GiB read + 1 GiB write elementwise kernel (3x), to scale "
                          "the FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--input-side", action="store_true",
+                    help="time the preprocess-equivalent merge (device) against the oracle's "
+                         "preprocess on a raw FASTA with duplicates, instead of the step")
     args = ap.parse_args()
+    if args.input_side:
+        return input_side(args)
     if args.copy_calibration:
         import torch
         a = torch.ones(1 << 30, dtype=torch.uint8, device="cuda")
