@@ -28,7 +28,7 @@ METRIC = "k-mer pairs/s + banded-NW GCUPS, 1 GPU; served jobs/s with p50/p99 lat
 VALU_PEAK_TOPS = 39.3    # 256 CU x 64 lanes x 2.4 GHz int32 lane-ops/s
 MFMA_FP4_PEAK_TOPS = 10100.0  # MI355X dense FP4 matrix ops/s (10.1 PFLOP/s)
 NW_OPS_PER_CELL = 13     # nominal int ops per in-band DP cell
-KMER_KERNELS = ("kmer_profile", "kmer_tile", "kmer_cand", "kmer_sort")
+KMER_KERNELS = ("kmer_profile", "kmer_tile", "kmer_cand_sort", "kmer_cand", "kmer_sort")
 NW_KERNELS = ("nw_bucket", "nw_band_reg", "nw_band_sliced", "nw_generic", "nw_filter")
 # the NW launches of one step run concurrently on side streams; this event
 # pair on the main stream brackets all of them (fork to join)

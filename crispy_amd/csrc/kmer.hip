@@ -176,6 +176,39 @@ __device__ __forceinline__ void tile_coords(int64_t t, int nb, int &bi, int &bj)
 }
 
 // ------------------------------------------------------ matrix-core tiles
+// Tile order: the upper-triangular grid of 128x128 tiles is cut into
+// kSuperTiles x kSuperTiles super-tiles, enumerated row-major over I <= J, and
+// the tiles of a super-tile row-major (upper triangle only on the diagonal).
+// Each XCD takes a contiguous run of that order, so the tiles it has in flight
+// share a few dozen row and column blocks, which stay in its L2 (a plain
+// row-major order re-streams every column block once per tile row).
+// start[s] = first tile index of super-tile s, ij[s] = I << 16 | J.
+constexpr int kSuperTiles = 16;
+
+__device__ __forceinline__ void super_tile_coords(int64_t t, const int64_t *__restrict__ start,
+                                                  const int32_t *__restrict__ ij, int nst, int nb,
+                                                  int &bi, int &bj) {
+  int lo = 0, hi = nst - 1;  // last s with start[s] <= t
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= t) lo = mid; else hi = mid - 1;
+  }
+  const int64_t o = t - start[lo];
+  const int I = ij[lo] >> 16, J = ij[lo] & 0xFFFF;
+  const int r0 = I * kSuperTiles, c0 = J * kSuperTiles;
+  const int c1 = min(nb, c0 + kSuperTiles);
+  if (I < J) {
+    const int w = c1 - c0;
+    bi = r0 + (int)(o / w);
+    bj = c0 + (int)(o % w);
+  } else {
+    int li, lj;
+    tile_coords(o, c1 - r0, li, lj);
+    bi = r0 + li;
+    bj = r0 + lj;
+  }
+}
+
 // One 128x128 tile of read pairs per block.  The popcount product runs on the
 // matrix cores: popcount(a AND b) = sum_k a_k * b_k over {0,1} codes, as
 // v_mfma_scale_f32_32x32x64_f8f6f4 with both operands FP4 (E2M1) and unit
@@ -220,7 +253,8 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
     const uint32_t *__restrict__ bitmap, const int32_t *__restrict__ Tn,
     const int32_t *__restrict__ rcnt, const int64_t *__restrict__ roff,
     const uint32_t *__restrict__ rlist, const int32_t *__restrict__ excess, int32_t n, int nb,
-    int64_t tile_begin, int64_t ntiles, int64_t per_xcd, float thr, float cut, int tshift,
+    int64_t tile_begin, int64_t ntiles, int64_t per_xcd, const int64_t *__restrict__ sstart,
+    const int32_t *__restrict__ sij, int nst, float thr, float cut, int tshift,
     unsigned long long *__restrict__ counter, uint64_t *__restrict__ out_keys,
     float *__restrict__ out_dists, int64_t cap) {
   static_assert(K >= 1 && K <= 6, "bitmap tiles hold k <= 6");
@@ -241,7 +275,7 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
   const int64_t local = (bid & 7) * per_xcd + (bid >> 3);
   if (local >= ntiles) return;
   int bi, bj;
-  tile_coords(tile_begin + local, nb, bi, bj);
+  super_tile_coords(tile_begin + local, sstart, sij, nst, nb, bi, bj);
   const int row0 = bi * kTile, col0 = bj * kTile;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
@@ -629,33 +663,75 @@ static int launch_profile(crispy_ctx *ctx, ReadStore &rs) {
   return CRISPY_OK;
 }
 
+// Host side of the tile order: super-tile table for nb tile rows, and the
+// (bi, bj) of every tile index (for the timing units of partial ranges).
+static void super_tiles(int nb, std::vector<int64_t> &start, std::vector<int32_t> &ij) {
+  const int ns = (nb + kSuperTiles - 1) / kSuperTiles;
+  start.assign(1, 0);
+  ij.clear();
+  for (int I = 0; I < ns; ++I)
+    for (int J = I; J < ns; ++J) {
+      const int64_t r0 = (int64_t)I * kSuperTiles, c0 = (int64_t)J * kSuperTiles;
+      const int64_t r1 = std::min<int64_t>(nb, r0 + kSuperTiles), c1 = std::min<int64_t>(nb, c0 + kSuperTiles);
+      const int64_t cnt = I < J ? (r1 - r0) * (c1 - c0) : (r1 - r0) * (r1 - r0 + 1) / 2;
+      ij.push_back((I << 16) | J);
+      start.push_back(start.back() + cnt);
+    }
+}
+
+static int ensure_super_table(crispy_ctx *ctx, int nb) {
+  KmerState &km = ctx->km;
+  if (km.super_nb == nb) return CRISPY_OK;
+  std::vector<int64_t> start;
+  std::vector<int32_t> ij;
+  super_tiles(nb, start, ij);
+  km.super_count = (int)ij.size();
+  CRISPY_TRY(ensure(ctx, km.sstart, sizeof(int64_t) * start.size()));
+  CRISPY_TRY(ensure(ctx, km.sij, sizeof(int32_t) * ij.size()));
+  CRISPY_HIP(ctx, hipMemcpyAsync(km.sstart.p, start.data(), sizeof(int64_t) * start.size(),
+                                 hipMemcpyHostToDevice, ctx->stream));
+  CRISPY_HIP(ctx, hipMemcpyAsync(km.sij.p, ij.data(), sizeof(int32_t) * ij.size(), hipMemcpyHostToDevice,
+                                 ctx->stream));
+  CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  km.super_nb = nb;
+  return CRISPY_OK;
+}
+
 template <int K, bool CAND>
 static int launch_tiles(crispy_ctx *ctx, int32_t n, int nb, int64_t tb, int64_t nt, float thr,
                         int tshift, uint64_t *out_keys, float *out_dists, int64_t cap) {
   KmerState &km = ctx->km;
+  CRISPY_TRY(ensure_super_table(ctx, nb));
   const int64_t per_xcd = (nt + 7) / 8;
   const int64_t grid = per_xcd * 8;
   hipEvent_t ev;
   timer_begin(ctx, "kmer_tile", &ev);
   kmer_tile_mfma_kernel<K, CAND><<<(unsigned)grid, 128, 0, ctx->stream>>>(
       km.bitmap.as<uint32_t>(), km.T.as<int32_t>(), km.rcnt.as<int32_t>(), km.roff.as<int64_t>(),
-      km.rlist.as<uint32_t>(), km.excess.as<int32_t>(), n, nb, tb, nt,
-      per_xcd, thr, 1.0f - thr - 2e-5f, tshift, km.counter.as<unsigned long long>(), out_keys,
-      out_dists, cap);
+      km.rlist.as<uint32_t>(), km.excess.as<int32_t>(), n, nb, tb, nt, per_xcd,
+      km.sstart.as<int64_t>(), km.sij.as<int32_t>(), km.super_count, thr, 1.0f - thr - 2e-5f, tshift,
+      km.counter.as<unsigned long long>(), out_keys, out_dists, cap);
   CRISPY_LAUNCHED(ctx);
   double units = 0;  // read pairs i < j < n inside the selected tiles
   if (ctx->timing) {
-    int64_t rs = 0;
-    for (int bi = 0; bi < nb; ++bi) {
-      const int64_t re = rs + (nb - bi);
-      const int64_t a = std::max(rs, tb), b = std::min(re, tb + nt);
-      const int64_t rh = std::min<int64_t>(kTile, n - (int64_t)bi * kTile);
-      for (int64_t t = a; t < b; ++t) {
-        const int bj = bi + (int)(t - rs);
-        const int64_t chh = std::min<int64_t>(kTile, n - (int64_t)bj * kTile);
-        units += bi == bj ? (double)rh * (rh - 1) / 2 : (double)rh * chh;
+    const int64_t total = (int64_t)nb * (nb + 1) / 2;
+    if (tb == 0 && nt == total) {
+      units = (double)n * (n - 1) / 2;
+    } else {
+      std::vector<int64_t> start;
+      std::vector<int32_t> ij;
+      super_tiles(nb, start, ij);
+      for (size_t sidx = 0; sidx < ij.size(); ++sidx) {
+        const int I = ij[sidx] >> 16, J = ij[sidx] & 0xFFFF;
+        int64_t t = start[sidx];
+        for (int bi = I * kSuperTiles; bi < std::min(nb, (I + 1) * kSuperTiles); ++bi)
+          for (int bj = std::max(bi, J * kSuperTiles); bj < std::min(nb, (J + 1) * kSuperTiles); ++bj, ++t) {
+            if (t < tb || t >= tb + nt) continue;
+            const int64_t rh = std::min<int64_t>(kTile, n - (int64_t)bi * kTile);
+            const int64_t chh = std::min<int64_t>(kTile, n - (int64_t)bj * kTile);
+            units += bi == bj ? (double)rh * (rh - 1) / 2 : (double)rh * chh;
+          }
       }
-      rs = re;
     }
   }
   timer_end(ctx, "kmer_tile", ev, units);
@@ -827,13 +903,30 @@ int crispy_kmer_run(crispy_ctx *ctx, int K, float threshold, int64_t tile_begin,
                                    km.cand.as<uint64_t>(), nullptr, km.cand_cap);
     }));
     km.candidates = ncand;
+    // (i, j) order, so the lanes of a wave merge against the same few row lists
+    const uint64_t *cand = km.cand.as<uint64_t>();
+    if (ncand > 1) {
+      CRISPY_TRY(ensure(ctx, km.cand_alt, sizeof(uint64_t) * ncand));
+      size_t tbytes = 0;
+      CRISPY_HIP(ctx, rocprim::radix_sort_keys(nullptr, tbytes, km.cand.as<uint64_t>(),
+                                               km.cand_alt.as<uint64_t>(), (size_t)ncand, 0, 40,
+                                               ctx->stream));
+      CRISPY_TRY(ensure(ctx, ctx->temp, tbytes));
+      hipEvent_t sev;
+      timer_begin(ctx, "kmer_cand_sort", &sev);
+      CRISPY_HIP(ctx, rocprim::radix_sort_keys(ctx->temp.p, tbytes, km.cand.as<uint64_t>(),
+                                               km.cand_alt.as<uint64_t>(), (size_t)ncand, 0, 40,
+                                               ctx->stream));
+      timer_end(ctx, "kmer_cand_sort", sev, (double)ncand);
+      cand = km.cand_alt.as<uint64_t>();
+    }
     CRISPY_TRY(run_counted(ctx, km.cap, km.keys, &km.dists, &km.count, [&]() -> int {
       if (ncand == 0) return CRISPY_OK;
       hipEvent_t ev;
       timer_begin(ctx, "kmer_cand", &ev);
       const int blocks = (int)std::min<int64_t>((ncand + 255) / 256, 8192);
       kmer_cand_kernel<<<blocks, 256, 0, ctx->stream>>>(
-          km.cand.as<uint64_t>(), ncand, km.lists.as<uint16_t>(), km.loff.as<int64_t>(), K,
+          cand, ncand, km.lists.as<uint16_t>(), km.loff.as<int64_t>(), K,
           rs.len.as<int32_t>(), threshold, km.counter.as<unsigned long long>(), km.keys.as<uint64_t>(),
           km.dists.as<float>(), km.cap);
       CRISPY_LAUNCHED(ctx);
