@@ -20,8 +20,9 @@
 //   nw_band_reg_kernel<KB>  one lane per pair, the whole band row of 2kB+1
 //     cells held in VGPRs in band coordinates d = j - i + kB, updated in
 //     place in ascending d (diag = old[d], left = old[d+1], up = new[d-1]).
-//     Pairs are bucketed by kB (template) and sorted by (L2, L1) so the
-//     64 lanes of a wave walk rows in lock step.  Reads are 2-bit packed; a
+//     Pairs are bucketed by kB (template) and sorted by (R, L1), R =
+//     min(L2, L1 + kB) the pair's last row with an in-band cell, so the 64
+//     lanes of a wave walk rows in lock step and stop together.  Reads are 2-bit packed; a
 //     row's match bits come from XOR of a sliding 2-bit window of read 1
 //     with read 2's base replicated.  Requires ACGT-only reads.
 //   nw_band_sliced: kKbRegMax < kB <= kKbSliceMax, G lanes per pair.
@@ -52,7 +53,8 @@ const int2 *kmer_result_pairs(crispy_ctx *ctx);
 constexpr uint32_t kGenericKey = 0xFFFFFFFFu;
 
 // ---------------------------------------------------------------- bucketing
-// Sort key per pair: (kB << 22 | L2 << 11 | L1), lengths clamped to 2047,
+// Sort key per pair: (kB << 22 | R << 11 | L1), R = min(L2, L1 + kB) the last
+// row with an in-band cell, clamped to 2047,
 // for the register and sliced kernels; all ones for the generic kernel.  Per-bucket pair and cell counts are
 // reduced in LDS first (a handful of buckets would serialise global atomics).
 __global__ __launch_bounds__(256) void nw_bucket_kernel(
@@ -78,7 +80,9 @@ __global__ __launch_bounds__(256) void nw_bucket_kernel(
       const int kb = (L1 + L2) / (2 * band);
       nc = band_cells(L1, L2, kb);
       if (kb <= kKbSliceMax && clean[pr.x] && clean[pr.y]) {
-        key = ((uint32_t)kb << 22) | ((uint32_t)min(L2, 2047) << 11) | (uint32_t)min(L1, 2047);
+        // rows past L1 + kB hold no in-band cell: the DP ends at min(L2, L1 + kB)
+        const int rows = min(L2, L1 + kb);
+        key = ((uint32_t)kb << 22) | ((uint32_t)min(rows, 2047) << 11) | (uint32_t)min(L1, 2047);
         bucket = kb;
       }
     } else {

@@ -118,8 +118,13 @@ __device__ __forceinline__ void nw_reg_wave(const uint32_t *__restrict__ pack,
   const uint32_t *s2 = pack + poff[pr.y];
   const int nw1 = (L1 + 15) >> 4, nw2 = (L2 + 15) >> 4;
 
-  // wave-uniform row bounds (pairs are sorted by (kB, L2, L1), so these are close)
-  int l1m = L1, l2lo = L2, l2hi = L2;
+  // Rows past L1 + kB hold no in-band cell: the last column is out of reach
+  // and the last row, if that low, is out of band.  So a pair's DP ends at
+  // row R = min(L2, L1 + kB).  Wave-uniform bounds (pairs are sorted by
+  // (kB, R, L1), so these are close); lanes past their own R compute rows
+  // that no capture reads.
+  const int R = min(L2, L1 + KB);
+  int l1m = L1, l2lo = L2, l2hi = R;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     l1m = min(l1m, __shfl_xor(l1m, off, 64));
@@ -128,7 +133,7 @@ __device__ __forceinline__ void nw_reg_wave(const uint32_t *__restrict__ pack,
   }
   const int L1min = __builtin_amdgcn_readfirstlane(l1m);
   const int L2min = __builtin_amdgcn_readfirstlane(l2lo);
-  const int L2max = __builtin_amdgcn_readfirstlane(l2hi);
+  const int Rmax = __builtin_amdgcn_readfirstlane(l2hi);
 
   int Mt[W], Mv[W];
   uint32_t LA[W];
@@ -203,12 +208,12 @@ __device__ __forceinline__ void nw_reg_wave(const uint32_t *__restrict__ pack,
   // Rows 1..kB hold the boundary column; rows below L1min-kB cannot reach
   // column L1 of any lane.  Three loops, one row variant each, so the
   // state keeps one register assignment per loop.
-  const int top_end = min(KB, L2max);
-  const int mid_end = min(L2max, max(top_end, L1min - KB - 1));
+  const int top_end = min(KB, Rmax);
+  const int mid_end = min(Rmax, max(top_end, L1min - KB - 1));
   int i = 1;
   for (; i <= top_end; ++i) step(i, std::integral_constant<int, kRowTop>{});
   for (; i <= mid_end; ++i) step(i, std::integral_constant<int, kRowFull>{});
-  for (; i <= L2max; ++i) step(i, std::integral_constant<int, kRowCapture>{});
+  for (; i <= Rmax; ++i) step(i, std::integral_constant<int, kRowCapture>{});
   uint32_t la;
   int j;
   if (bestI > bestJ) {
@@ -224,6 +229,10 @@ __device__ __forceinline__ void nw_reg_wave(const uint32_t *__restrict__ pack,
 }
 
 // -------------------------------------------------------- sliced kernel
+// DPP controls (GFX9 wave-wide shifts by one lane)
+constexpr int kDppWaveShl1 = 0x130;  // lane l takes lane l+1
+constexpr int kDppWaveShr1 = 0x138;  // lane l takes lane l-1
+
 // Wide bands (kB > kKbRegMax): G = ceil(W / 24) lanes per pair, lane g owns
 // band positions [24g, 24g + 24) of every row in VGPRs (same per-cell state
 // and arithmetic as nw_band_reg_kernel).  Lane g processes row t - g at step
@@ -299,11 +308,11 @@ __device__ __forceinline__ void nw_sliced_wave(const uint32_t *__restrict__ pack
   const uint32_t *s2 = pack + poff[pr.y];
   const int nw1 = (L1 + 15) >> 4;
   const bool is_end = g == G - 1;
-  const int src_up = g > 0 ? lane - 1 : lane;
-  const int src_left = g < G - 1 ? lane + 1 : lane;
   const int base_d = g * C;
 
-  int tm = has_pair ? L2 + g : 0;
+  // the pair's DP ends at row R = min(L2, L1 + kB) (no in-band cell below)
+  const int R = min(L2, L1 + KB);
+  int tm = has_pair ? R + g : 0;
   int l1m = has_pair ? L1 : 0x7fffffff;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -336,13 +345,21 @@ __device__ __forceinline__ void nw_sliced_wave(const uint32_t *__restrict__ pack
   uint32_t upLA = 0;
   int bIM = kNeg, bIj = 0, bJM = kNeg, bJr = 0;
   uint32_t bILA = 0, bJLA = 0;
+  // read-2 words: R2c holds rows 16w+1 .. 16w+16 of the current row i
+  // (w = (i-1) >> 4), R2n the next 16, loaded 16 steps before it is needed
+  const int nw2 = (L2 + 15) >> 4;
+  uint32_t R2c = load_word(s2, 0, nw2), R2n = load_word(s2, 1, nw2);
 
   auto step = [&](const int t, auto mode_tag) {
     constexpr int MODE = decltype(mode_tag)::value;
     const int i = t - g;
-    const bool active = has_pair && i >= 1 && i <= L2;
+    const bool active = has_pair && i >= 1 && i <= R;
     const int im1 = i - 1;
-    const uint32_t w2 = (im1 >= 0 && im1 < L2) ? s2[im1 >> 4] : 0u;
+    if (im1 > 0 && (im1 & 15) == 0) {
+      R2c = R2n;
+      R2n = load_word(s2, (im1 >> 4) + 1, nw2);
+    }
+    const uint32_t w2 = (im1 >= 0 && im1 < L2) ? R2c : 0u;
     const uint32_t bb = ((w2 >> (2 * (im1 & 15))) & 3u) * 0x55555555u;
     uint32_t m[NMW];
 #pragma unroll
@@ -362,8 +379,10 @@ __device__ __forceinline__ void nw_sliced_wave(const uint32_t *__restrict__ pack
                                 Mh, LAu, k0, kc, capM, capLA);
     }
     // lane g+1's first cell of row i-1 is this lane's "left" for its last cell
-    const int xM = __shfl(Mv[0], src_left, 64);
-    const uint32_t xLA = (uint32_t)__shfl((int)LA[0], src_left, 64);
+    // (a DPP wave shift: lane l reads lane l+1; the last lane of a group ignores it)
+    const int xM = __builtin_amdgcn_update_dpp(Mv[0], Mv[0], kDppWaveShl1, 0xF, 0xF, false);
+    const uint32_t xLA = (uint32_t)__builtin_amdgcn_update_dpp((int)LA[0], (int)LA[0], kDppWaveShl1, 0xF,
+                                                               0xF, false);
     if (active) {
 #pragma unroll
       for (int k = 1; k < C; ++k) {
@@ -404,10 +423,12 @@ __device__ __forceinline__ void nw_sliced_wave(const uint32_t *__restrict__ pack
       }
     }
     // this lane's last cell of row i is lane g+1's "up" at the next step
-    const int nM = __shfl(Mh, src_up, 64);
-    const uint32_t nLA = (uint32_t)__shfl((int)LAu, src_up, 64);
+    // (a DPP wave shift: lane l reads lane l-1; the first lane of a group ignores it)
+    const int nM = __builtin_amdgcn_update_dpp(Mh, Mh, kDppWaveShr1, 0xF, 0xF, false);
+    const uint32_t nLA = (uint32_t)__builtin_amdgcn_update_dpp((int)LAu, (int)LAu, kDppWaveShr1, 0xF, 0xF,
+                                                               false);
     upM = g > 0 ? nM : kNeg;
-    upLA = nLA;
+    upLA = g > 0 ? nLA : LAu;
 #pragma unroll
     for (int k = 0; k < NB - 1; ++k) B[k] = __builtin_amdgcn_alignbit(B[k + 1], B[k], 2);
     B[NB - 1] = __builtin_amdgcn_alignbit(F, B[NB - 1], 2);
