@@ -66,6 +66,7 @@ def load_library():
         "crispy_nw_run_on_kmer": ([_vp, ctypes.c_int, ctypes.c_float, _i64p], ctypes.c_int),
         "crispy_nw_fetch": ([_vp, ctypes.c_int64, ctypes.c_int64, _i32p, _f32p], ctypes.c_int),
         "crispy_nw_dists": ([_vp, _i32p, ctypes.c_int64, ctypes.c_int, _f32p], ctypes.c_int),
+        "crispy_nw_csr": ([_vp, ctypes.c_int32, _i64p, _i32p, _f32p], ctypes.c_int),
         "crispy_sort_by_dist": ([_vp, _i32p, _f32p, ctypes.c_int64], ctypes.c_int),
         "crispy_timing_enable": ([_vp, ctypes.c_int], ctypes.c_int),
         "crispy_timing_reset": ([_vp], ctypes.c_int),
@@ -94,6 +95,7 @@ def load_library():
                                     ctypes.c_int),
         "crispy_server_job_error": ([_vp, ctypes.c_int64], ctypes.c_char_p),
         "crispy_server_fetch": ([_vp, ctypes.c_int64, _i32p, _f32p], ctypes.c_int),
+        "crispy_server_fetch_csr": ([_vp, ctypes.c_int64, _i64p, _i32p, _f32p], ctypes.c_int),
         "crispy_server_status": ([_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                   _i64p, _i64p], ctypes.c_int),
     }
@@ -298,6 +300,15 @@ class Context:
         self._check(self._L.crispy_nw_fetch(self._h, first, count, _p(p, _i32p), _p(d, _f32p)))
         return p, d
 
+    def nw_csr(self, n_rows):
+        """(row_ptr, cols, vals) of the last NW result, built on the device."""
+        rp = np.zeros(n_rows + 1, np.int64)
+        c = self.nw_count
+        cols = np.zeros(max(c, 1), np.int32)
+        vals = np.zeros(max(c, 1), np.float32)
+        self._check(self._L.crispy_nw_csr(self._h, n_rows, _p(rp, _i64p), _p(cols, _i32p), _p(vals, _f32p)))
+        return rp, cols[:c], vals[:c]
+
     def nw_dists(self, pairs, band):
         pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
         d = np.zeros(len(pairs), np.float32)
@@ -423,6 +434,17 @@ class Server:
         d = np.zeros(n, np.float32)
         self._job_check(jid, self._L.crispy_server_fetch(self._h, jid, _p(p, _i32p), _p(d, _f32p)))
         return p, d
+
+    def fetch_csr(self, jid, n_rows):
+        """Waits for job `jid`; returns its result as (row_ptr, cols, vals) and forgets it."""
+        info = self.wait(jid)
+        c = info["nw_pairs"]
+        rp = np.zeros(n_rows + 1, np.int64)
+        cols = np.zeros(max(c, 1), np.int32)
+        vals = np.zeros(max(c, 1), np.float32)
+        self._job_check(jid, self._L.crispy_server_fetch_csr(self._h, jid, _p(rp, _i64p), _p(cols, _i32p),
+                                                             _p(vals, _f32p)))
+        return rp, cols[:c], vals[:c]
 
     def status(self):
         r, q, b, pk = ctypes.c_int(), ctypes.c_int(), ctypes.c_int64(), ctypes.c_int64()

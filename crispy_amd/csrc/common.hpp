@@ -102,6 +102,7 @@ struct KmerState {
   DevBuf sstart, sij;
   int super_nb = -1;
   int super_count = 0;
+  int kshift = 20;  // result keys are i << kshift | j
   int64_t cand_cap = 0;
   int64_t candidates = 0;
   // result
@@ -115,6 +116,7 @@ struct NwState {
   DevBuf in_pairs, keys, keys_alt, idx, idx_alt, dist, hist, cells, flags, sel, nsel;
   DevBuf out_pairs, out_dists, scratch;
   DevBuf tasks;  // bucket table + wave tasks of the fused DP launches
+  DevBuf csr_rows, csr_cols;  // CSR view of the (i, j)-ordered result
   int64_t count = 0;
   bool valid = false;
 };

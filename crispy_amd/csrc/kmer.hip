@@ -254,7 +254,7 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
     const int32_t *__restrict__ rcnt, const int64_t *__restrict__ roff,
     const uint32_t *__restrict__ rlist, const int32_t *__restrict__ excess, int32_t n, int nb,
     int64_t tile_begin, int64_t ntiles, int64_t per_xcd, const int64_t *__restrict__ sstart,
-    const int32_t *__restrict__ sij, int nst, float thr, float cut, int tshift,
+    const int32_t *__restrict__ sij, int nst, float thr, float cut, int tshift, int kshift,
     unsigned long long *__restrict__ counter, uint64_t *__restrict__ out_keys,
     float *__restrict__ out_dists, int64_t cap) {
   static_assert(K >= 1 && K <= 6, "bitmap tiles hold k <= 6");
@@ -502,7 +502,7 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
   });
   auto emit = [&](int64_t pos, int li, int lj, uint32_t m) {
     if (pos < cap) {
-      out_keys[pos] = ((uint64_t)(row0 + li) << 20) | (uint64_t)(col0 + lj - kTile);
+      out_keys[pos] = ((uint64_t)(row0 + li) << kshift) | (uint64_t)(col0 + lj - kTile);
       if (!CAND) out_dists[pos] = 1.0f - (float)m / (float)min(sT[li], sT[lj]);
     }
   };
@@ -534,11 +534,13 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
   });
 }
 
-__global__ void unpack_keys_kernel(const uint64_t *__restrict__ keys, int64_t n, int2 *__restrict__ pairs) {
+__global__ void unpack_keys_kernel(const uint64_t *__restrict__ keys, int64_t n, int kshift,
+                                   int2 *__restrict__ pairs) {
+  const uint64_t jmask = (1ull << kshift) - 1;
   for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n;
        k += (int64_t)gridDim.x * blockDim.x) {
     const uint64_t v = keys[k];
-    pairs[k] = make_int2((int)(v >> 20), (int)(v & 0xFFFFFu));
+    pairs[k] = make_int2((int)(v >> kshift), (int)(v & jmask));
   }
 }
 
@@ -601,7 +603,7 @@ __global__ __launch_bounds__(256) void kmer_list_kernel(const uint32_t *__restri
 
 __global__ __launch_bounds__(256) void kmer_cand_kernel(
     const uint64_t *__restrict__ cand, int64_t ncand, const uint16_t *__restrict__ lists,
-    const int64_t *__restrict__ loff, int K, const int32_t *__restrict__ len, float thr,
+    const int64_t *__restrict__ loff, int K, const int32_t *__restrict__ len, float thr, int kshift,
     unsigned long long *__restrict__ counter, uint64_t *__restrict__ out_keys,
     float *__restrict__ out_dists, int64_t cap) {
   const int lane = threadIdx.x & 63;
@@ -615,7 +617,7 @@ __global__ __launch_bounds__(256) void kmer_cand_kernel(
     uint64_t key = 0;
     if (c < ncand) {
       key = cand[c];
-      const int i = (int)(key >> 20), j = (int)(key & 0xFFFFFu);
+      const int i = (int)(key >> kshift), j = (int)(key & ((1ull << kshift) - 1));
       const uint16_t *A = lists + loff[i];
       const uint16_t *Bl = lists + loff[j];
       const unsigned short l1 = (unsigned short)(len[i] - K + 1), l2 = (unsigned short)(len[j] - K + 1);
@@ -710,7 +712,7 @@ static int launch_tiles(crispy_ctx *ctx, int32_t n, int nb, int64_t tb, int64_t 
       km.bitmap.as<uint32_t>(), km.T.as<int32_t>(), km.rcnt.as<int32_t>(), km.roff.as<int64_t>(),
       km.rlist.as<uint32_t>(), km.excess.as<int32_t>(), n, nb, tb, nt, per_xcd,
       km.sstart.as<int64_t>(), km.sij.as<int32_t>(), km.super_count, thr, 1.0f - thr - 2e-5f, tshift,
-      km.counter.as<unsigned long long>(), out_keys, out_dists, cap);
+      km.kshift, km.counter.as<unsigned long long>(), out_keys, out_dists, cap);
   CRISPY_LAUNCHED(ctx);
   double units = 0;  // read pairs i < j < n inside the selected tiles
   if (ctx->timing) {
@@ -838,14 +840,14 @@ static int sort_keys(crispy_ctx *ctx, int64_t count) {
   size_t tbytes = 0;
   CRISPY_HIP(ctx, rocprim::radix_sort_pairs(nullptr, tbytes, km.keys.as<uint64_t>(),
                                             km.keys_alt.as<uint64_t>(), km.dists.as<float>(),
-                                            km.dists_alt.as<float>(), (size_t)count, 0, 40,
+                                            km.dists_alt.as<float>(), (size_t)count, 0, 2 * km.kshift,
                                             ctx->stream));
   CRISPY_TRY(ensure(ctx, ctx->temp, tbytes));
   hipEvent_t ev;
   timer_begin(ctx, "kmer_sort", &ev);
   CRISPY_HIP(ctx, rocprim::radix_sort_pairs(ctx->temp.p, tbytes, km.keys.as<uint64_t>(),
                                             km.keys_alt.as<uint64_t>(), km.dists.as<float>(),
-                                            km.dists_alt.as<float>(), (size_t)count, 0, 40,
+                                            km.dists_alt.as<float>(), (size_t)count, 0, 2 * km.kshift,
                                             ctx->stream));
   timer_end(ctx, "kmer_sort", ev, (double)count);
   return CRISPY_OK;
@@ -885,6 +887,9 @@ int crispy_kmer_run(crispy_ctx *ctx, int K, float threshold, int64_t tile_begin,
     if (count) *count = 0;
     return CRISPY_OK;
   }
+  // pair keys i << kshift | j with 2^kshift >= n: as few radix-sort bits as the read count needs
+  km.kshift = 1;
+  while ((1 << km.kshift) < n) ++km.kshift;
   const int KP = K <= 6 ? K : 6;
   CRISPY_TRY(build_profiles(ctx, KP, K));
   // first guess at the result size: the pairs in the range, at most 4 Mi
@@ -909,13 +914,13 @@ int crispy_kmer_run(crispy_ctx *ctx, int K, float threshold, int64_t tile_begin,
       CRISPY_TRY(ensure(ctx, km.cand_alt, sizeof(uint64_t) * ncand));
       size_t tbytes = 0;
       CRISPY_HIP(ctx, rocprim::radix_sort_keys(nullptr, tbytes, km.cand.as<uint64_t>(),
-                                               km.cand_alt.as<uint64_t>(), (size_t)ncand, 0, 40,
+                                               km.cand_alt.as<uint64_t>(), (size_t)ncand, 0, 2 * km.kshift,
                                                ctx->stream));
       CRISPY_TRY(ensure(ctx, ctx->temp, tbytes));
       hipEvent_t sev;
       timer_begin(ctx, "kmer_cand_sort", &sev);
       CRISPY_HIP(ctx, rocprim::radix_sort_keys(ctx->temp.p, tbytes, km.cand.as<uint64_t>(),
-                                               km.cand_alt.as<uint64_t>(), (size_t)ncand, 0, 40,
+                                               km.cand_alt.as<uint64_t>(), (size_t)ncand, 0, 2 * km.kshift,
                                                ctx->stream));
       timer_end(ctx, "kmer_cand_sort", sev, (double)ncand);
       cand = km.cand_alt.as<uint64_t>();
@@ -927,7 +932,7 @@ int crispy_kmer_run(crispy_ctx *ctx, int K, float threshold, int64_t tile_begin,
       const int blocks = (int)std::min<int64_t>((ncand + 255) / 256, 8192);
       kmer_cand_kernel<<<blocks, 256, 0, ctx->stream>>>(
           cand, ncand, km.lists.as<uint16_t>(), km.loff.as<int64_t>(), K,
-          rs.len.as<int32_t>(), threshold, km.counter.as<unsigned long long>(), km.keys.as<uint64_t>(),
+          rs.len.as<int32_t>(), threshold, km.kshift, km.counter.as<unsigned long long>(), km.keys.as<uint64_t>(),
           km.dists.as<float>(), km.cap);
       CRISPY_LAUNCHED(ctx);
       timer_end(ctx, "kmer_cand", ev, (double)ncand);
@@ -941,7 +946,7 @@ int crispy_kmer_run(crispy_ctx *ctx, int K, float threshold, int64_t tile_begin,
   if (km.count > 0) {
     const int blocks = (int)std::min<int64_t>((km.count + 255) / 256, 4096);
     unpack_keys_kernel<<<blocks, 256, 0, ctx->stream>>>(km.keys_alt.as<uint64_t>(), km.count,
-                                                        km.pairs.as<int2>());
+                                                        km.kshift, km.pairs.as<int2>());
     CRISPY_LAUNCHED(ctx);
   }
   CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));

@@ -50,11 +50,15 @@ namespace crispy {
 
 const int2 *kmer_result_pairs(crispy_ctx *ctx);
 
-constexpr uint32_t kGenericKey = 0xFFFFFFFFu;
+// Bucket keys kB << 11 | R (21 bits: kB <= kKbSliceMax < 1024, R <= 2047);
+// the generic kernel's pairs sort last.
+constexpr int kBucketKeyBits = 21;
+constexpr uint32_t kGenericKey = (1u << kBucketKeyBits) - 1;
+static_assert(((uint32_t)kKbSliceMax << 11 | 2047u) < kGenericKey, "bucket keys fit below the generic key");
 
 // ---------------------------------------------------------------- bucketing
-// Sort key per pair: (kB << 22 | R << 11 | L1), R = min(L2, L1 + kB) the last
-// row with an in-band cell, clamped to 2047,
+// Sort key per pair: (kB << 11 | R), R = min(L2, L1 + kB) the last row with
+// an in-band cell, clamped to 2047,
 // for the register and sliced kernels; all ones for the generic kernel.  Per-bucket pair and cell counts are
 // reduced in LDS first (a handful of buckets would serialise global atomics).
 __global__ __launch_bounds__(256) void nw_bucket_kernel(
@@ -82,7 +86,7 @@ __global__ __launch_bounds__(256) void nw_bucket_kernel(
       if (kb <= kKbSliceMax && clean[pr.x] && clean[pr.y]) {
         // rows past L1 + kB hold no in-band cell: the DP ends at min(L2, L1 + kB)
         const int rows = min(L2, L1 + kb);
-        key = ((uint32_t)kb << 22) | ((uint32_t)min(rows, 2047) << 11) | (uint32_t)min(L1, 2047);
+        key = ((uint32_t)kb << 11) | (uint32_t)min(rows, 2047);
         bucket = kb;
       }
     } else {
@@ -231,6 +235,27 @@ __global__ void unpack_pair_keys_kernel(const uint64_t *__restrict__ keys, const
   }
 }
 
+// CSR row starts of the (i, j)-ordered result: row_ptr[r] = first k with
+// pairs[k].x >= r (lower bound), r = 0..n.
+__global__ void nw_csr_rows_kernel(const int2 *__restrict__ pairs, int64_t count, int32_t n,
+                                   int64_t *__restrict__ row_ptr) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r <= n;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    int64_t lo = 0, hi = count;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (pairs[mid].x < r) lo = mid + 1; else hi = mid;
+    }
+    row_ptr[r] = lo;
+  }
+}
+
+__global__ void nw_csr_cols_kernel(const int2 *__restrict__ pairs, int64_t count, int32_t *__restrict__ cols) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < count;
+       k += (int64_t)gridDim.x * blockDim.x)
+    cols[k] = pairs[k].y;
+}
+
 static inline int grid_for(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192)); }
 
 
@@ -263,11 +288,11 @@ static int nw_core(crispy_ctx *ctx, const int2 *d_pairs, int64_t np, int band) {
   size_t tbytes = 0;
   CRISPY_HIP(ctx, rocprim::radix_sort_pairs(nullptr, tbytes, nw.keys.as<uint32_t>(),
                                             nw.keys_alt.as<uint32_t>(), nw.idx.as<uint32_t>(),
-                                            nw.idx_alt.as<uint32_t>(), (size_t)np, 0, 32, ctx->stream));
+                                            nw.idx_alt.as<uint32_t>(), (size_t)np, 0, kBucketKeyBits, ctx->stream));
   CRISPY_TRY(ensure(ctx, ctx->temp, tbytes));
   CRISPY_HIP(ctx, rocprim::radix_sort_pairs(ctx->temp.p, tbytes, nw.keys.as<uint32_t>(),
                                             nw.keys_alt.as<uint32_t>(), nw.idx.as<uint32_t>(),
-                                            nw.idx_alt.as<uint32_t>(), (size_t)np, 0, 32, ctx->stream));
+                                            nw.idx_alt.as<uint32_t>(), (size_t)np, 0, kBucketKeyBits, ctx->stream));
   timer_end(ctx, "nw_bucket", ev, (double)np);
   unsigned int hist[kNumBuckets];
   unsigned long long cells[kNumBuckets];
@@ -470,6 +495,32 @@ int crispy_nw_fetch(crispy_ctx *ctx, int64_t first, int64_t count, int32_t *pair
   if (dists)
     CRISPY_HIP(ctx, hipMemcpyAsync(dists, nw.out_dists.as<float>() + first, sizeof(float) * count,
                                    hipMemcpyDeviceToHost, ctx->stream));
+  CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return CRISPY_OK;
+}
+
+int crispy_nw_csr(crispy_ctx *ctx, int32_t n_rows, int64_t *row_ptr, int32_t *cols, float *vals) {
+  if (!ctx || n_rows < 0 || !row_ptr) return CRISPY_ERR_ARG;
+  NwState &nw = ctx->nw;
+  if (!nw.valid) return set_err(ctx, CRISPY_ERR_STATE, "no NW result");
+  CRISPY_HIP(ctx, hipSetDevice(ctx->device));
+  const int64_t c = nw.count;
+  CRISPY_TRY(ensure(ctx, nw.csr_rows, sizeof(int64_t) * ((size_t)n_rows + 1)));
+  CRISPY_TRY(ensure(ctx, nw.csr_cols, sizeof(int32_t) * (c + 1)));
+  nw_csr_rows_kernel<<<grid_for((int64_t)n_rows + 1), 256, 0, ctx->stream>>>(
+      nw.out_pairs.as<int2>(), c, n_rows, nw.csr_rows.as<int64_t>());
+  CRISPY_LAUNCHED(ctx);
+  if (c > 0) {
+    nw_csr_cols_kernel<<<grid_for(c), 256, 0, ctx->stream>>>(nw.out_pairs.as<int2>(), c,
+                                                           nw.csr_cols.as<int32_t>());
+    CRISPY_LAUNCHED(ctx);
+  }
+  CRISPY_HIP(ctx, hipMemcpyAsync(row_ptr, nw.csr_rows.p, sizeof(int64_t) * ((size_t)n_rows + 1),
+                                 hipMemcpyDeviceToHost, ctx->stream));
+  if (c > 0 && cols)
+    CRISPY_HIP(ctx, hipMemcpyAsync(cols, nw.csr_cols.p, sizeof(int32_t) * c, hipMemcpyDeviceToHost, ctx->stream));
+  if (c > 0 && vals)
+    CRISPY_HIP(ctx, hipMemcpyAsync(vals, nw.out_dists.p, sizeof(float) * c, hipMemcpyDeviceToHost, ctx->stream));
   CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return CRISPY_OK;
 }

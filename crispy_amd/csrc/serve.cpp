@@ -38,6 +38,7 @@ struct Job {
   int64_t kmer_pairs = 0;
   std::vector<int32_t> pairs;
   std::vector<float> dists;
+  std::vector<int64_t> row_ptr;  // CSR rows of the result (built on the device)
   Clock::time_point t_submit, t_start, t_end;
 };
 
@@ -79,6 +80,10 @@ void run_job(crispy_ctx *ctx, Job &job) {
     job.pairs.resize(2 * c);
     job.dists.resize(c);
     rc = crispy_nw_fetch(ctx, 0, c, job.pairs.data(), job.dists.data());
+  }
+  if (rc == CRISPY_OK) {
+    job.row_ptr.resize((size_t)job.n + 1);
+    rc = crispy_nw_csr(ctx, job.n, job.row_ptr.data(), nullptr, nullptr);
   }
   job.status = rc;
   if (rc != CRISPY_OK) job.error = crispy_ctx_error(ctx);
@@ -313,6 +318,26 @@ int crispy_server_fetch(crispy_server *srv, int64_t job_id, int32_t *pairs, floa
   if (status == CRISPY_OK) {
     if (pairs && !job->pairs.empty()) std::memcpy(pairs, job->pairs.data(), job->pairs.size() * sizeof(int32_t));
     if (dists && !job->dists.empty()) std::memcpy(dists, job->dists.data(), job->dists.size() * sizeof(float));
+  }
+  srv->jobs.erase(it);
+  return status;
+}
+
+int crispy_server_fetch_csr(crispy_server *srv, int64_t job_id, int64_t *row_ptr, int32_t *cols,
+                            float *vals) {
+  if (!srv || !row_ptr) return CRISPY_ERR_ARG;
+  std::unique_lock<std::mutex> lk(srv->mu);
+  auto it = srv->jobs.find(job_id);
+  if (it == srv->jobs.end()) return CRISPY_ERR_ARG;
+  Job *job = it->second.get();
+  srv->cv_done.wait(lk, [&] { return job->done; });
+  const int status = job->status;
+  if (status == CRISPY_OK) {
+    std::memcpy(row_ptr, job->row_ptr.data(), job->row_ptr.size() * sizeof(int64_t));
+    const size_t c = job->dists.size();
+    if (cols)
+      for (size_t k = 0; k < c; ++k) cols[k] = job->pairs[2 * k + 1];
+    if (vals && c) std::memcpy(vals, job->dists.data(), c * sizeof(float));
   }
   srv->jobs.erase(it);
   return status;

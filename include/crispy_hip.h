@@ -130,6 +130,12 @@ int crispy_nw_run(crispy_ctx *ctx, const int32_t *pairs, int64_t npairs, int ban
 int crispy_nw_run_on_kmer(crispy_ctx *ctx, int band, float threshold, int64_t *count);
 int crispy_nw_fetch(crispy_ctx *ctx, int64_t first, int64_t count, int32_t *pairs,
                     float *dists);
+/* The result as a CSR matrix over n_rows reads (upper triangle, i < j): built
+ * on the device from the (i, j)-ordered pairs of crispy_nw_run_on_kmer (or of
+ * crispy_nw_run on (i, j)-ordered input); row_ptr[n_rows + 1] on the host,
+ * cols / vals (count entries, either may be NULL) the column ids and
+ * distances. */
+int crispy_nw_csr(crispy_ctx *ctx, int32_t n_rows, int64_t *row_ptr, int32_t *cols, float *vals);
 /* Unfiltered distances for host pairs (every pair gets a distance). */
 int crispy_nw_dists(crispy_ctx *ctx, const int32_t *pairs, int64_t npairs, int band,
                     float *dists);
@@ -228,6 +234,10 @@ const char *crispy_server_job_error(crispy_server *srv, int64_t job_id);
 /* Waits for the job, copies its NW-passing pairs (int32 x2, ascending (i, j))
  * and distances out and forgets the job. */
 int crispy_server_fetch(crispy_server *srv, int64_t job_id, int32_t *pairs, float *dists);
+/* Like crispy_server_fetch, as CSR: row_ptr[n + 1] (built on the device),
+ * cols and vals (nw_pairs entries, either may be NULL); forgets the job. */
+int crispy_server_fetch_csr(crispy_server *srv, int64_t job_id, int64_t *row_ptr, int32_t *cols,
+                            float *vals);
 /* Jobs running and queued, device bytes charged to the slots now and at most
  * so far (idle slots are charged what their contexts hold and give it back
  * when the queue head would not fit otherwise). */

@@ -156,3 +156,23 @@ def test_errors():
         for reads in (ca.Reads(b"", np.zeros(1, np.int64)), ca.Reads.from_list([b"ACGTACGTAA"])):
             p, d = srv.fetch(srv.submit(reads, 6, 0.3, 10, 0.3))
             assert len(d) == 0 and p.shape == (0, 2)
+
+
+def test_csr_views(jobs, solo):
+    """The device CSR of a result equals the (i, j) pairs grouped by row, from
+    a context and from the server."""
+    job, (p, d) = jobs[1], solo[1]
+    n = job["reads"].n
+    want_rp = np.searchsorted(p[:, 0], np.arange(n + 1), side="left").astype(np.int64)
+    with ca.Context(0) as c:
+        c.upload_reads(ca.KMER, job["reads"])
+        c.upload_reads(ca.NW, job["reads"])
+        c.kmer_run(job["K"], job["kthr"])
+        c.nw_run_on_kmer(job["band"], job["nthr"])
+        rp, cols, vals = c.nw_csr(n)
+    assert np.array_equal(rp, want_rp) and np.array_equal(cols, p[:, 1])
+    assert np.array_equal(vals.view(np.uint32), d.view(np.uint32))
+    with ca.Server(0, 2, 32 << 30) as srv:
+        rp2, cols2, vals2 = srv.fetch_csr(submit(srv, job), n)
+    assert np.array_equal(rp2, want_rp) and np.array_equal(cols2, p[:, 1])
+    assert np.array_equal(vals2.view(np.uint32), d.view(np.uint32))
