@@ -193,7 +193,10 @@ int crispy_ctx_create(int device, crispy_ctx **out) {
   crispy_ctx *ctx = new crispy_ctx();
   ctx->device = device;
   bool ok = hipSetDevice(device) == hipSuccess && hipStreamCreate(&ctx->stream) == hipSuccess &&
-            hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming) == hipSuccess;
+            hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming) == hipSuccess &&
+            hipEventCreateWithFlags(&ctx->hist_ev, hipEventDisableTiming) == hipSuccess &&
+            hipHostMalloc(&ctx->h_hist, kNumBuckets * (sizeof(unsigned int) + sizeof(unsigned long long)),
+                          hipHostMallocDefault) == hipSuccess;
   for (int w = 0; ok && w < kNumWorkers; ++w)
     ok = hipStreamCreate(&ctx->workers[w]) == hipSuccess &&
          hipEventCreateWithFlags(&ctx->join_ev[w], hipEventDisableTiming) == hipSuccess;
@@ -245,6 +248,8 @@ void crispy_ctx_destroy(crispy_ctx *ctx) {
     if (ctx->join_ev[w]) (void)hipEventDestroy(ctx->join_ev[w]);
   }
   if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
+  if (ctx->hist_ev) (void)hipEventDestroy(ctx->hist_ev);
+  if (ctx->h_hist) (void)hipHostFree(ctx->h_hist);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }

@@ -156,6 +156,8 @@ struct crispy_ctx {
   DevBuf temp;  // rocPRIM temporary storage
   void *h_stage = nullptr;  // pinned host staging for small async uploads
   size_t h_stage_bytes = 0;
+  void *h_hist = nullptr;      // pinned NW bucket histogram (cells, then pairs)
+  hipEvent_t hist_ev = nullptr;  // the histogram has reached h_hist
   bool timing = false;
   std::vector<PendingEvent> pending;
   std::vector<hipEvent_t> pool;

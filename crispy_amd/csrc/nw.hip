@@ -285,6 +285,15 @@ static int nw_core(crispy_ctx *ctx, const int2 *d_pairs, int64_t np, int band) {
       d_pairs, np, rs.len.as<int32_t>(), rs.clean.as<uint8_t>(), band, nw.keys.as<uint32_t>(),
       nw.idx.as<uint32_t>(), nw.hist.as<unsigned int>(), nw.cells.as<unsigned long long>());
   CRISPY_LAUNCHED(ctx);
+  // The histogram is final here: bring it to the host now and build the task
+  // table there while the device sorts the pairs.
+  unsigned long long *cells = reinterpret_cast<unsigned long long *>(ctx->h_hist);
+  unsigned int *hist = reinterpret_cast<unsigned int *>(cells + kNumBuckets);
+  CRISPY_HIP(ctx, hipMemcpyAsync(hist, nw.hist.p, sizeof(unsigned int) * kNumBuckets, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+  CRISPY_HIP(ctx, hipMemcpyAsync(cells, nw.cells.p, sizeof(unsigned long long) * kNumBuckets,
+                                 hipMemcpyDeviceToHost, ctx->stream));
+  CRISPY_HIP(ctx, hipEventRecord(ctx->hist_ev, ctx->stream));
   size_t tbytes = 0;
   CRISPY_HIP(ctx, rocprim::radix_sort_pairs(nullptr, tbytes, nw.keys.as<uint32_t>(),
                                             nw.keys_alt.as<uint32_t>(), nw.idx.as<uint32_t>(),
@@ -294,11 +303,7 @@ static int nw_core(crispy_ctx *ctx, const int2 *d_pairs, int64_t np, int band) {
                                             nw.keys_alt.as<uint32_t>(), nw.idx.as<uint32_t>(),
                                             nw.idx_alt.as<uint32_t>(), (size_t)np, 0, kBucketKeyBits, ctx->stream));
   timer_end(ctx, "nw_bucket", ev, (double)np);
-  unsigned int hist[kNumBuckets];
-  unsigned long long cells[kNumBuckets];
-  CRISPY_HIP(ctx, hipMemcpyAsync(hist, nw.hist.p, sizeof hist, hipMemcpyDeviceToHost, ctx->stream));
-  CRISPY_HIP(ctx, hipMemcpyAsync(cells, nw.cells.p, sizeof cells, hipMemcpyDeviceToHost, ctx->stream));
-  CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  CRISPY_HIP(ctx, hipEventSynchronize(ctx->hist_ev));
   const uint32_t *order = nw.idx_alt.as<uint32_t>();
   float *out = nw.dist.as<float>();
   // Bucket b's pairs are order[start[b] .. start[b] + hist[b]).
