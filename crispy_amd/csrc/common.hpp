@@ -96,6 +96,8 @@ struct KmerState {
   int K = 0;
   int profile_gen = -1;
   DevBuf bitmap, T, hinfo, rcnt, roff, rlist, excess;
+  // the repeat lists back to back in read order (rlistc + roffc[r])
+  DevBuf roffc, rlistc;
   // K = 7, 8: signed-sorted K-mer lists and the k = 6 candidate pairs
   DevBuf loff, lists, cand, cand_alt;
   // tile order: super-tile table for super_nb tile rows

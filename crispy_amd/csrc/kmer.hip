@@ -26,6 +26,7 @@
 #include <cstring>
 
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 #include <rocprim/device/device_select.hpp>
 #include <rocprim/iterator/counting_iterator.hpp>
 
@@ -211,15 +212,16 @@ __device__ __forceinline__ void super_tile_coords(int64_t t, const int64_t *__re
 
 // One 128x128 tile of read pairs per block.  The popcount product runs on the
 // matrix cores: popcount(a AND b) = sum_k a_k * b_k over {0,1} codes, as
-// v_mfma_scale_f32_32x32x64_f8f6f4 with both operands FP4 (E2M1) and unit
-// scales: a set bit becomes the nibble 0b0001 = 0.5, so each product is 0.25
-// and the fp32 accumulator holds popcount / 4 exactly (at most 1024).  Operand
+// v_mfma_scale_f32_32x32x64_f8f6f4 with both operands FP4 (E2M1).  Operand
 // lanes (checked on the device): lane l holds row (A) / column (B) l % 32 and
 // 32 k-slots, the same 32 for A and B; accumulator r of lane l is pair
 // (8(r/4) + 4(l/32) + r%4, l % 32).  Which 32 of a step's 64 codes a lane holds
 // is free as long as A and B agree, so for an 8-word group lane half g takes
-// words 4g..4g+3 and step s (bit plane) expands each word as
-// (w >> s) & 0x11111111 -- 2 VALU ops per 8 k-slots, 48 per 8 MFMAs.
+// words 4g..4g+3 and step s takes bit s of every nibble: w & (0x11111111 << s)
+// for s = 0, 1, 2 (one VALU op per word; the nibbles read 0.5, 1 and 2) and
+// (w >> 2) & 0x22222222 for s = 3 (bit 3 is the FP4 sign; the nibbles read 1).
+// The A scale 2^2, 2^0, 2^-2, 2^0 makes every product 1, so the fp32
+// accumulator holds the popcount exactly (at most 4096).
 // Block: 2 waves, wave w owns tile rows [64w, 64w + 64) x all 128 columns
 // (2 x 4 fragments, 128 accumulators per lane).  Bitmap chunks of 16 words are
 // staged row-major (padded) in LDS, the next chunk prefetched into registers;
@@ -227,11 +229,20 @@ __device__ __forceinline__ void super_tile_coords(int64_t t, const int64_t *__re
 // contiguous run, so tiles sharing a row block share an L2).
 // Epilogue: matches <= popcount + min(excess_i, excess_j) (excess = sum of
 // count-1 over a read's codes), so a pair with
-//   ub < (T_min - tshift) * (1 - thr - 2e-5) - 1
-// cannot pass (the margin covers the fp32 roundings); only the others pay for
-// the repeat-list merge and the division.  The tile's 256 repeat lists are
-// copied into the (then idle) chunk buffer when they fit, so the merges read
-// LDS; a passing pair's match count replaces its popcount for the output pass.
+//   popcount + min(excess_i, excess_j) < min(F_i, F_j),  F = (T - tshift) * cut - 1
+// cannot pass (F rises with T, so min(F_i, F_j) = F(min(T_i, T_j)); the margin
+// in cut covers the fp32 roundings).  The test is branch-free: 5 VALU ops per
+// pair (the minima of the non-negative F and excess as unsigned minima of
+// their bit patterns), the sign of the difference shifted into a 16-bit reject
+// mask per fragment.  F <= 0 is stored as 0: such a pair survives either way.
+// Diagonal tiles (i < j) and the tiles of the last row or column block (reads
+// past n) mask their fragments explicitly.  The few survivors of a fragment go
+// through the wave's LDS queue, 64 at a time, for the exact count (popcount
+// plus the repeat-list merge when both reads have repeats) and the threshold;
+// the results return to the owning lanes and one atomic per wave reserves the
+// output.  The tile's repeat lists are copied into the then idle chunk buffer
+// (two coalesced runs of the compacted lists) only when the block has enough
+// survivors to pay for it.
 // CAND (K = 7, 8 candidates, tshift = K - 6): no merge, the bound alone.
 template <class F, int... I>
 __device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, I...>) {
@@ -245,8 +256,25 @@ __device__ __forceinline__ void static_for(F &&f) {
 
 typedef int mfma_v8i __attribute__((ext_vector_type(8)));
 typedef float mfma_v16f __attribute__((ext_vector_type(16)));
-constexpr int kFp4 = 4;        // f8f6f4 operand format code of FP4 (E2M1)
+typedef float lds_f4 __attribute__((ext_vector_type(4)));
+constexpr int kFp4 = 4;          // f8f6f4 operand format code of FP4 (E2M1)
 constexpr int kUnitScale = 127;  // E8M0 scale 2^0
+constexpr int kStageMin = 16;    // survivors in a block that pay for staging its repeat lists
+
+// min of two non-negative floats (no NaN) as the unsigned min of their bits
+__device__ __forceinline__ float umin_f(float a, float b) {
+  return __uint_as_float(min(__float_as_uint(a), __float_as_uint(b)));
+}
+
+// inclusive prefix sum over the 64 lanes of a wave
+__device__ __forceinline__ int wave_incl_sum(int v, int lane) {
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int u = __shfl_up(v, off, 64);
+    if (lane >= off) v += u;
+  }
+  return v;
+}
 
 template <int K, bool CAND>
 __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
@@ -263,13 +291,14 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
   constexpr int kRow = kChunkWords + 4;  // padded row stride of a staged chunk (words)
   constexpr int kStageWords = 2 * kTile * kRow;
   __shared__ uint32_t Ps[kStageWords];  // bitmap chunk in the main loop, then repeat lists
+  __shared__ __align__(16) float sF[2 * kTile];
+  __shared__ __align__(16) float sEf[2 * kTile];
   __shared__ int32_t sT[2 * kTile];
   __shared__ int32_t sRc[2 * kTile];
   __shared__ int64_t sRo[2 * kTile];
-  __shared__ int32_t sE[2 * kTile];
   __shared__ int32_t sLo[2 * kTile];
   __shared__ int32_t sWave[2];
-  __shared__ uint32_t sQ[CAND ? 1 : 2][CAND ? 1 : 1024];  // per-wave merge queue
+  __shared__ uint32_t sQ[2][1024];  // per-wave survivor queue
 
   const int64_t bid = blockIdx.x;
   const int64_t local = (bid & 7) * per_xcd + (bid >> 3);
@@ -284,10 +313,13 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
   for (int q = tid; q < 2 * kTile; q += 128) {
     const int r = q < kTile ? row0 + q : col0 + (q - kTile);
     const bool ok = r < n;
-    sT[q] = ok ? Tn[r] : 1;
+    const int T = ok ? Tn[r] : 1;
+    sT[q] = T;
     sRc[q] = ok ? rcnt[r] : 0;
     sRo[q] = ok ? roff[r] : 0;
-    sE[q] = ok ? excess[r] : 0;
+    sEf[q] = ok ? (float)excess[r] : 0.0f;
+    const float F = fmaf((float)(T - tshift), cut, -1.0f);
+    sF[q] = ok && F > 0.0f ? F : 0.0f;
   }
 
   // loader: thread t stages row t (read row0 + t) and row 128 + t (read col0 + t)
@@ -311,6 +343,32 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
     for (int fb = 0; fb < 4; ++fb)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[fa][fb][r] = 0.0f;
+
+  auto mfma_step = [&](const uint4 *xa, const uint4 *xb, uint32_t M, int shift, int scale) {
+    mfma_v8i ea[2], eb[4];
+#pragma unroll
+    for (int fa = 0; fa < 2; ++fa) {
+      ea[fa][0] = (int)((xa[fa].x >> shift) & M);
+      ea[fa][1] = (int)((xa[fa].y >> shift) & M);
+      ea[fa][2] = (int)((xa[fa].z >> shift) & M);
+      ea[fa][3] = (int)((xa[fa].w >> shift) & M);
+      ea[fa][4] = ea[fa][5] = ea[fa][6] = ea[fa][7] = 0;  // unused by FP4 operands
+    }
+#pragma unroll
+    for (int fb = 0; fb < 4; ++fb) {
+      eb[fb][0] = (int)((xb[fb].x >> shift) & M);
+      eb[fb][1] = (int)((xb[fb].y >> shift) & M);
+      eb[fb][2] = (int)((xb[fb].z >> shift) & M);
+      eb[fb][3] = (int)((xb[fb].w >> shift) & M);
+      eb[fb][4] = eb[fb][5] = eb[fb][6] = eb[fb][7] = 0;
+    }
+#pragma unroll
+    for (int fa = 0; fa < 2; ++fa)
+#pragma unroll
+      for (int fb = 0; fb < 4; ++fb)
+        acc[fa][fb] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(ea[fa], eb[fb], acc[fa][fb], kFp4, kFp4, 0,
+                                                                      scale, 0, kUnitScale);
+  };
 
   for (int ch = 0; ch < NCH; ++ch) {
     __syncthreads();
@@ -340,194 +398,147 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
 #pragma unroll
       for (int fb = 0; fb < 4; ++fb)
         xb[fb] = *reinterpret_cast<const uint4 *>(&Ps[(kTile + fb * 32 + r32) * kRow + 8 * m + 4 * g]);
-      // one bit plane per step; the words shift in place so the planes are
-      // not all expanded (and live) at once
+      // bit planes 0..2 in a rolled loop (one plane's operands live at a time)
 #pragma unroll 1
-      for (int s = 0; s < 4; ++s) {
-        constexpr uint32_t M = 0x11111111u;
-        mfma_v8i ea[2], eb[4];
-#pragma unroll
-        for (int fa = 0; fa < 2; ++fa) {
-          ea[fa][0] = (int)(xa[fa].x & M);
-          ea[fa][1] = (int)(xa[fa].y & M);
-          ea[fa][2] = (int)(xa[fa].z & M);
-          ea[fa][3] = (int)(xa[fa].w & M);
-          ea[fa][4] = ea[fa][5] = ea[fa][6] = ea[fa][7] = 0;  // unused by FP4 operands
-          xa[fa].x >>= 1;
-          xa[fa].y >>= 1;
-          xa[fa].z >>= 1;
-          xa[fa].w >>= 1;
-        }
-#pragma unroll
-        for (int fb = 0; fb < 4; ++fb) {
-          eb[fb][0] = (int)(xb[fb].x & M);
-          eb[fb][1] = (int)(xb[fb].y & M);
-          eb[fb][2] = (int)(xb[fb].z & M);
-          eb[fb][3] = (int)(xb[fb].w & M);
-          eb[fb][4] = eb[fb][5] = eb[fb][6] = eb[fb][7] = 0;
-          xb[fb].x >>= 1;
-          xb[fb].y >>= 1;
-          xb[fb].z >>= 1;
-          xb[fb].w >>= 1;
-        }
-#pragma unroll
-        for (int fa = 0; fa < 2; ++fa)
-#pragma unroll
-          for (int fb = 0; fb < 4; ++fb)
-            acc[fa][fb] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
-                ea[fa], eb[fb], acc[fa][fb], kFp4, kFp4, 0, kUnitScale, 0, kUnitScale);
-      }
+      for (int s = 0; s < 3; ++s) mfma_step(xa, xb, 0x11111111u << s, 0, kUnitScale + 2 - 2 * s);
+      mfma_step(xa, xb, 0x22222222u, 2, kUnitScale);
     }
   }
 
-  // popcounts (exact: the accumulators hold multiples of 1/4 below 2^10)
-  uint32_t cnt[2][4][16];
+  // ---- bound: a 16-bit mask of possible pairs per fragment, branch-free
+  const bool diag = bi == bj;
+  const int rlim = n - row0, clim = n - col0;  // valid rows / columns of the tile
+  const bool partial = diag || rlim < kTile || clim < kTile;  // block-uniform
+  uint32_t keep[8];
+  int nkeep = 0;
+  static_for<2>([&](auto FA) {
+    constexpr int fa = decltype(FA)::value;
+    const int rb = wv * 64 + fa * 32 + 4 * g;  // rows rb + 8q + t hold accumulators 4q + t
+    lds_f4 fr[4], er[4];
 #pragma unroll
-  for (int fa = 0; fa < 2; ++fa)
-#pragma unroll
-    for (int fb = 0; fb < 4; ++fb)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) cnt[fa][fb][r] = (uint32_t)(acc[fa][fb][r] * 4.0f);
-
-  // ---- stage the repeat lists in LDS when all 256 fit (thread t: reads 2t, 2t+1)
-  bool staged = false;
-  if (!CAND) {
-    const int c0 = sRc[2 * tid], c1 = sRc[2 * tid + 1];
-    int incl = c0 + c1;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const int v = __shfl_up(incl, off, 64);
-      if (lane >= off) incl += v;
+    for (int q = 0; q < 4; ++q) {
+      fr[q] = *reinterpret_cast<const lds_f4 *>(&sF[rb + 8 * q]);
+      er[q] = *reinterpret_cast<const lds_f4 *>(&sEf[rb + 8 * q]);
     }
-    if (lane == 63) sWave[wv] = incl;
-    __syncthreads();  // also: both waves are done with the chunk buffer
-    const int before = wv ? sWave[0] : 0;
-    staged = sWave[0] + sWave[1] <= kStageWords;
+    static_for<4>([&](auto FB) {
+      constexpr int fb = decltype(FB)::value;
+      const int lj = kTile + fb * 32 + r32;
+      const float fc = sF[lj], ec = sEf[lj];
+      uint32_t rej = 0;
+      static_for<16>([&](auto R) {
+        constexpr int r = 15 - decltype(R)::value;  // bit r of the mask ends up holding pair r
+        const float d = (acc[fa][fb][r] + umin_f(er[r >> 2][r & 3], ec)) - umin_f(fr[r >> 2][r & 3], fc);
+        rej = __builtin_amdgcn_alignbit(rej, __float_as_uint(d), 31);  // rej << 1 | sign(d)
+      });
+      uint32_t k = ~rej & 0xFFFFu;
+      if (partial) {  // i < j (row0 == col0 on the diagonal), i < n, j < n
+        const int cj = lj - kTile;
+        uint32_t ok = 0;
+        static_for<16>([&](auto R) {
+          constexpr int r = decltype(R)::value;
+          const int ri = rb + 8 * (r >> 2) + (r & 3);
+          ok |= (uint32_t)((!diag || ri < cj) && ri < rlim && cj < clim) << r;
+        });
+        k &= ok;
+      }
+      keep[fa * 4 + fb] = k;
+      nkeep += __popc(k);
+    });
+  });
+
+  // ---- block survivors; stage the repeat lists when they pay
+  int wsum = nkeep;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) wsum += __shfl_xor(wsum, off, 64);
+  if (lane == 0) sWave[wv] = wsum;
+  __syncthreads();  // also: both waves are done with the chunk buffer
+  const int btot = sWave[0] + sWave[1];
+  if (btot == 0) return;  // block-uniform
+  bool staged = false;
+  if (!CAND && btot >= kStageMin) {  // block-uniform
+    const int la = min(kTile, n - row0) - 1, lb = min(kTile, n - col0) - 1;
+    const int64_t a0 = sRo[0], b0 = sRo[kTile];
+    const int lenA = (int)(sRo[la] + sRc[la] - a0);
+    const int lenB = diag ? 0 : (int)(sRo[kTile + lb] + sRc[kTile + lb] - b0);
+    staged = lenA + lenB <= kStageWords;
     if (staged) {
-      const int o = before + incl - c0 - c1;
-      sLo[2 * tid] = o;
-      sLo[2 * tid + 1] = o + c0;
-      // the two lists back to back; independent loads, unrolled so they overlap
-      const uint32_t *src0 = rlist + sRo[2 * tid];
-      const uint32_t *src1 = rlist + sRo[2 * tid + 1];
-#pragma unroll 8
-      for (int e = 0; e < c0 + c1; ++e) Ps[o + e] = e < c0 ? src0[e] : src1[e - c0];
+      for (int w = tid; w < lenA; w += 128) Ps[w] = rlist[a0 + w];
+      for (int w = tid; w < lenB; w += 128) Ps[lenA + w] = rlist[b0 + w];
+      for (int q = tid; q < 2 * kTile; q += 128)
+        sLo[q] = q < kTile || diag ? (int)(sRo[q] - a0) : lenA + (int)(sRo[q] - b0);
     }
     __syncthreads();
   }
 
-  // ---- epilogue: bound, exact count, filter, compaction (128 pairs per lane,
-  // compile-time register indices).  Pairs the bound keeps whose reads both
-  // have repeats need a merge: they go to the wave's LDS queue, fragment by
-  // fragment, the whole wave merges them 64 at a time (a lane-per-pair merge
-  // would keep the wave busy for every lane's near pair) and writes (count,
-  // pass) back to the slot, and the owning lanes take the results back.  The
-  // other kept pairs are decided from the popcount alone.  One atomic per
-  // wave reserves the output.
-  uint64_t passmask[2] = {0, 0};
-  int mycount = 0;
+  // ---- survivors: exact count and threshold through the wave's queue; the
+  // results go back to the owning lanes (pass bit in keep, distance in the
+  // accumulator), then one atomic per wave reserves the output
   uint32_t *queue = sQ[wv];
-  static_for<8>([&](auto F) {
-    constexpr int fa = decltype(F)::value / 4, fb = decltype(F)::value % 4;
-    uint32_t nd = 0;
-    static_for<16>([&](auto R) {
-      constexpr int r = decltype(R)::value;
-      constexpr int p = (fa * 4 + fb) * 16 + r;
-      const int li = wv * 64 + fa * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
-      const int lj = kTile + fb * 32 + r32;
-      const int i = row0 + li, j = col0 + lj - kTile;
-      if (i < j && j < n) {
-        const int tmin = min(sT[li], sT[lj]);
-        const uint32_t pop = cnt[fa][fb][r];
-        const uint32_t ub = pop + (uint32_t)min(sE[li], sE[lj]);
-        if ((float)ub >= (float)(tmin - tshift) * cut - 1.0f) {
-          bool keep = CAND;
-          if (!CAND) {
-            if (sRc[li] > 0 && sRc[lj] > 0)
-              nd |= 1u << r;
-            else
-              keep = passes(1.0f - (float)pop / (float)tmin, thr);
-          }
-          if (keep) {
-            passmask[p >> 6] |= 1ull << (p & 63);
-            ++mycount;
-          }
+  if (!CAND) {
+    static_for<8>([&](auto F) {
+      constexpr int fa = decltype(F)::value / 4, fb = decltype(F)::value % 4;
+      uint32_t &k = keep[decltype(F)::value];
+      if (__ballot(k != 0) == 0) return;  // wave-uniform
+      const int c = __popc(k);
+      const int incl = wave_incl_sum(c, lane);
+      const int qn = __shfl(incl, 63, 64);
+      int o = incl - c;
+      static_for<16>([&](auto R) {
+        constexpr int r = decltype(R)::value;
+        if (k & (1u << r)) {
+          const int li = wv * 64 + fa * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
+          queue[o++] = (uint32_t)li | ((uint32_t)(fb * 32 + r32) << 7) | ((uint32_t)acc[fa][fb][r] << 14);
         }
+      });
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      for (int e = lane; e < qn; e += 64) {
+        const uint32_t q = queue[e];
+        const int li = (int)(q & 127u);
+        const int lj = kTile + (int)((q >> 7) & 127u);
+        uint32_t m = q >> 14;
+        const int na = sRc[li], nbr = sRc[lj];
+        if (na > 0 && nbr > 0)
+          m += staged ? repeat_min_sum(Ps + sLo[li], na, Ps + sLo[lj], nbr)
+                      : repeat_min_sum(rlist + sRo[li], na, rlist + sRo[lj], nbr);
+        const float dist = 1.0f - (float)m / (float)min(sT[li], sT[lj]);
+        queue[e] = passes(dist, thr) ? __float_as_uint(dist) : 0xFFFFFFFFu;  // (a NaN: no pass)
       }
-    });
-    if (CAND) return;
-    const int c = __popc(nd);
-    int incl = c;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const int v = __shfl_up(incl, off, 64);
-      if (lane >= off) incl += v;
-    }
-    const int qn = __shfl(incl, 63, 64);
-    if (qn == 0) return;  // wave-uniform
-    int o = incl - c;
-    static_for<16>([&](auto R) {
-      constexpr int r = decltype(R)::value;
-      if (nd & (1u << r)) {
-        const int li = wv * 64 + fa * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
-        queue[o++] = (uint32_t)li | ((uint32_t)(fb * 32 + r32) << 7) | (cnt[fa][fb][r] << 14);
-      }
-    });
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    for (int e = lane; e < qn; e += 64) {
-      const uint32_t q = queue[e];
-      const int li = (int)(q & 127u);
-      const int lj = kTile + (int)((q >> 7) & 127u);
-      const int na = sRc[li], nbr = sRc[lj];
-      const uint32_t m = (q >> 14) + (staged ? repeat_min_sum(Ps + sLo[li], na, Ps + sLo[lj], nbr)
-                                             : repeat_min_sum(rlist + sRo[li], na, rlist + sRo[lj], nbr));
-      const bool ok = passes(1.0f - (float)m / (float)min(sT[li], sT[lj]), thr);
-      queue[e] = (m << 1) | (ok ? 1u : 0u);  // back to the owner
-    }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    o = incl - c;
-    static_for<16>([&](auto R) {
-      constexpr int r = decltype(R)::value;
-      constexpr int p = (fa * 4 + fb) * 16 + r;
-      if (nd & (1u << r)) {
-        const uint32_t v = queue[o++];
-        if (v & 1u) {
-          passmask[p >> 6] |= 1ull << (p & 63);
-          ++mycount;
-          cnt[fa][fb][r] = v >> 1;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      o = incl - c;
+      static_for<16>([&](auto R) {
+        constexpr int r = decltype(R)::value;
+        if (k & (1u << r)) {
+          const uint32_t v = queue[o++];
+          if (v == 0xFFFFFFFFu)
+            k &= ~(1u << r);
+          else
+            acc[fa][fb][r] = __uint_as_float(v);
         }
-      }
+      });
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     });
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  });
-  auto emit = [&](int64_t pos, int li, int lj, uint32_t m) {
-    if (pos < cap) {
-      out_keys[pos] = ((uint64_t)(row0 + li) << kshift) | (uint64_t)(col0 + lj - kTile);
-      if (!CAND) out_dists[pos] = 1.0f - (float)m / (float)min(sT[li], sT[lj]);
-    }
-  };
-  int incl = mycount;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const int v = __shfl_up(incl, off, 64);
-    if (lane >= off) incl += v;
   }
+  int mine = 0;
+#pragma unroll
+  for (int f = 0; f < 8; ++f) mine += __popc(keep[f]);
+  const int incl = wave_incl_sum(mine, lane);
   const int total = __shfl(incl, 63, 64);
+  if (total == 0) return;  // wave-uniform
   unsigned long long base = 0;
-  if (total > 0) {
-    if (lane == 63) base = atomicAdd(counter, (unsigned long long)total);
-    base = __shfl(base, 63, 64);
-  }
-  if (mycount == 0) return;
-  int64_t pos = (int64_t)base + (incl - mycount);
+  if (lane == 63) base = atomicAdd(counter, (unsigned long long)total);
+  int64_t pos = (int64_t)__shfl(base, 63, 64) + (incl - mine);
   static_for<8>([&](auto F) {
     constexpr int fa = decltype(F)::value / 4, fb = decltype(F)::value % 4;
+    const uint32_t k = keep[decltype(F)::value];
+    if (__ballot(k != 0) == 0) return;  // wave-uniform
     static_for<16>([&](auto R) {
       constexpr int r = decltype(R)::value;
-      constexpr int p = (fa * 4 + fb) * 16 + r;
-      if (passmask[p >> 6] & (1ull << (p & 63))) {
-        const int li = wv * 64 + fa * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
-        emit(pos, li, kTile + fb * 32 + r32, cnt[fa][fb][r]);
+      if (k & (1u << r)) {
+        if (pos < cap) {
+          const int li = wv * 64 + fa * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
+          out_keys[pos] = ((uint64_t)(row0 + li) << kshift) | (uint64_t)(col0 + fb * 32 + r32);
+          if (!CAND) out_dists[pos] = acc[fa][fb][r];
+        }
         ++pos;
       }
     });
@@ -650,6 +661,23 @@ __global__ __launch_bounds__(256) void kmer_cand_kernel(
   }
 }
 
+// The repeat lists back to back in read order: roffc = exclusive sum of the
+// counts, one wave per read copies its list.
+__global__ __launch_bounds__(256) void compact_repeats_kernel(const int32_t *__restrict__ rcnt,
+                                                              const int64_t *__restrict__ roff,
+                                                              const uint32_t *__restrict__ rlist,
+                                                              const int64_t *__restrict__ roffc,
+                                                              uint32_t *__restrict__ rlistc, int32_t n) {
+  const int lane = threadIdx.x & 63;
+  const int64_t waves = (int64_t)gridDim.x * 4;
+  for (int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); r < n; r += waves) {
+    const int c = rcnt[r];
+    const uint32_t *src = rlist + roff[r];
+    uint32_t *dst = rlistc + roffc[r];
+    for (int e = lane; e < c; e += 64) dst[e] = src[e];
+  }
+}
+
 // ------------------------------------------------------------ host driver
 template <int K>
 static int launch_profile(crispy_ctx *ctx, ReadStore &rs) {
@@ -709,8 +737,8 @@ static int launch_tiles(crispy_ctx *ctx, int32_t n, int nb, int64_t tb, int64_t 
   hipEvent_t ev;
   timer_begin(ctx, "kmer_tile", &ev);
   kmer_tile_mfma_kernel<K, CAND><<<(unsigned)grid, 128, 0, ctx->stream>>>(
-      km.bitmap.as<uint32_t>(), km.T.as<int32_t>(), km.rcnt.as<int32_t>(), km.roff.as<int64_t>(),
-      km.rlist.as<uint32_t>(), km.excess.as<int32_t>(), n, nb, tb, nt, per_xcd,
+      km.bitmap.as<uint32_t>(), km.T.as<int32_t>(), km.rcnt.as<int32_t>(), km.roffc.as<int64_t>(),
+      km.rlistc.as<uint32_t>(), km.excess.as<int32_t>(), n, nb, tb, nt, per_xcd,
       km.sstart.as<int64_t>(), km.sij.as<int32_t>(), km.super_count, thr, 1.0f - thr - 2e-5f, tshift,
       km.kshift, km.counter.as<unsigned long long>(), out_keys, out_dists, cap);
   CRISPY_LAUNCHED(ctx);
@@ -778,6 +806,23 @@ static int build_profiles(crispy_ctx *ctx, int KP, int K) {
   CRISPY_HIP(ctx, hipMemcpyAsync(km.roff.p, roff.data(), sizeof(int64_t) * (n + 1),
                                  hipMemcpyHostToDevice, ctx->stream));
   CRISPY_TRY(kProfile[KP](ctx, rs));
+  CRISPY_TRY(ensure(ctx, km.roffc, sizeof(int64_t) * (n + 1)));
+  CRISPY_TRY(ensure(ctx, km.rlistc, sizeof(uint32_t) * (roff[n] + 1)));
+  {
+    size_t tbytes = 0;
+    CRISPY_HIP(ctx, rocprim::exclusive_scan(nullptr, tbytes, km.rcnt.as<int32_t>(), km.roffc.as<int64_t>(),
+                                            (int64_t)0, (size_t)n, rocprim::plus<int64_t>(), ctx->stream));
+    CRISPY_TRY(ensure(ctx, ctx->temp, tbytes));
+    hipEvent_t ev;
+    timer_begin(ctx, "kmer_profile", &ev);
+    CRISPY_HIP(ctx, rocprim::exclusive_scan(ctx->temp.p, tbytes, km.rcnt.as<int32_t>(), km.roffc.as<int64_t>(),
+                                            (int64_t)0, (size_t)n, rocprim::plus<int64_t>(), ctx->stream));
+    compact_repeats_kernel<<<(int)std::min<int64_t>((n + 3) / 4, 4096), 256, 0, ctx->stream>>>(
+        km.rcnt.as<int32_t>(), km.roff.as<int64_t>(), km.rlist.as<uint32_t>(), km.roffc.as<int64_t>(),
+        km.rlistc.as<uint32_t>(), n);
+    CRISPY_LAUNCHED(ctx);
+    timer_end(ctx, "kmer_profile", ev, n);
+  }
   if (K > 6) {  // the K-mer lists for the exact merge of the candidates
     std::vector<int64_t> loff(n + 1, 0);
     for (int32_t r = 0; r < n; ++r) loff[r + 1] = loff[r] + (rs.h_len[r] - K + 1);
