@@ -221,7 +221,7 @@ static void release_all(crispy_ctx *ctx) {
                     &ctx->pre.bases,   &ctx->pre.off,      &ctx->pre.freq,     &ctx->pre.keys,
                     &ctx->pre.keys_alt, &ctx->pre.flags,   &ctx->pre.sel,      &ctx->pre.nsel,
                     &ctx->pre.csum,    &ctx->pre.out_order, &ctx->pre.out_freq,
-                    &ctx->km.sstart,   &ctx->km.sij,       &ctx->km.cand_alt,  &ctx->nw.csr_rows,
+                    &ctx->km.cand_alt,  &ctx->nw.csr_rows,
                     &ctx->nw.csr_cols};
   for (DevBuf *b : bufs) release(ctx, *b);
   for (auto &r : ctx->reads) {

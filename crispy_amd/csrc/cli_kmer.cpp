@@ -114,7 +114,7 @@ int main(int argc, char *argv[]) {
     std::fprintf(stderr, "Error: no usable HIP device (code %d). Exit...\n", rc);
     return 255;
   }
-  std::printf("USE GPU (HIP, gfx950). tiles: %lld of %dx%d reads\n",
+  std::printf("USE GPU (HIP, gfx950). tile slots: %lld (%dx%d reads each)\n",
               (long long)crispy_kmer_num_tiles(n), 128, 128);
   int64_t count = 0;
   rc = crispy_reads_upload(ctx, CRISPY_READS_KMER, bases, offs, n);

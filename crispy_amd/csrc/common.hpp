@@ -100,10 +100,6 @@ struct KmerState {
   DevBuf roffc, rlistc;
   // K = 7, 8: signed-sorted K-mer lists and the k = 6 candidate pairs
   DevBuf loff, lists, cand, cand_alt;
-  // tile order: super-tile table for super_nb tile rows
-  DevBuf sstart, sij;
-  int super_nb = -1;
-  int super_count = 0;
   int kshift = 20;  // result keys are i << kshift | j
   int64_t cand_cap = 0;
   int64_t candidates = 0;

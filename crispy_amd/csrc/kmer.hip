@@ -164,7 +164,7 @@ __device__ __forceinline__ bool passes(float dist, float thr) {
   return dist < thr || (double)fabsf(diff) < 0.00001;
 }
 
-__device__ __forceinline__ void tile_coords(int64_t t, int nb, int &bi, int &bj) {
+__host__ __device__ inline void tile_coords(int64_t t, int nb, int &bi, int &bj) {
   // t = bi*nb - bi*(bi-1)/2 + (bj - bi), bi <= bj < nb
   const double b = 2.0 * nb + 1.0;
   int r = (int)((b - sqrt(b * b - 8.0 * (double)t)) * 0.5);
@@ -178,36 +178,30 @@ __device__ __forceinline__ void tile_coords(int64_t t, int nb, int &bi, int &bj)
 
 // ------------------------------------------------------ matrix-core tiles
 // Tile order: the upper-triangular grid of 128x128 tiles is cut into
-// kSuperTiles x kSuperTiles super-tiles, enumerated row-major over I <= J, and
-// the tiles of a super-tile row-major (upper triangle only on the diagonal).
-// Each XCD takes a contiguous run of that order, so the tiles it has in flight
-// share a few dozen row and column blocks, which stay in its L2 (a plain
-// row-major order re-streams every column block once per tile row).
-// start[s] = first tile index of super-tile s, ij[s] = I << 16 | J.
+// kSuperTiles x kSuperTiles super-tiles, enumerated row-major over I <= J; each
+// super-tile owns kSuperTiles^2 slots, its tiles row-major.  Slots below the
+// diagonal or past the last tile are empty (a few per cent of the slots), so
+// a slot's tile follows from its index alone.  Each XCD takes a contiguous run
+// of slots, so the tiles it has in flight share a few dozen row and column
+// blocks, which stay in its L2 (a plain row-major order re-streams every
+// column block once per tile row).
 constexpr int kSuperTiles = 16;
+constexpr int kSuperSlots = kSuperTiles * kSuperTiles;
 
-__device__ __forceinline__ void super_tile_coords(int64_t t, const int64_t *__restrict__ start,
-                                                  const int32_t *__restrict__ ij, int nst, int nb,
-                                                  int &bi, int &bj) {
-  int lo = 0, hi = nst - 1;  // last s with start[s] <= t
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (start[mid] <= t) lo = mid; else hi = mid - 1;
-  }
-  const int64_t o = t - start[lo];
-  const int I = ij[lo] >> 16, J = ij[lo] & 0xFFFF;
-  const int r0 = I * kSuperTiles, c0 = J * kSuperTiles;
-  const int c1 = min(nb, c0 + kSuperTiles);
-  if (I < J) {
-    const int w = c1 - c0;
-    bi = r0 + (int)(o / w);
-    bj = c0 + (int)(o % w);
-  } else {
-    int li, lj;
-    tile_coords(o, c1 - r0, li, lj);
-    bi = r0 + li;
-    bj = r0 + lj;
-  }
+// tile (bi, bj) of slot t for nb tile rows; false for an empty slot
+__host__ __device__ inline bool slot_coords(int64_t t, int nb, int &bi, int &bj) {
+  const int ns = (nb + kSuperTiles - 1) / kSuperTiles;
+  int I, J;
+  tile_coords(t / kSuperSlots, ns, I, J);
+  const int o = (int)(t % kSuperSlots);
+  bi = I * kSuperTiles + o / kSuperTiles;
+  bj = J * kSuperTiles + o % kSuperTiles;
+  return bi <= bj && bj < nb;
+}
+
+__host__ __device__ inline int64_t num_slots(int nb) {
+  const int64_t ns = (nb + kSuperTiles - 1) / kSuperTiles;
+  return ns * (ns + 1) / 2 * kSuperSlots;
 }
 
 // One 128x128 tile of read pairs per block.  The popcount product runs on the
@@ -240,9 +234,9 @@ __device__ __forceinline__ void super_tile_coords(int64_t t, const int64_t *__re
 // through the wave's LDS queue, 64 at a time, for the exact count (popcount
 // plus the repeat-list merge when both reads have repeats) and the threshold;
 // the results return to the owning lanes and one atomic per wave reserves the
-// output.  The tile's repeat lists are copied into the then idle chunk buffer
-// (two coalesced runs of the compacted lists) only when the block has enough
-// survivors to pay for it.
+// output.  When the block has survivors, the tile's repeat lists are copied
+// into the then idle chunk buffer (two coalesced runs of the compacted lists)
+// if they fit, so the merges read LDS.
 // CAND (K = 7, 8 candidates, tshift = K - 6): no merge, the bound alone.
 template <class F, int... I>
 __device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, I...>) {
@@ -259,7 +253,6 @@ typedef float mfma_v16f __attribute__((ext_vector_type(16)));
 typedef float lds_f4 __attribute__((ext_vector_type(4)));
 constexpr int kFp4 = 4;          // f8f6f4 operand format code of FP4 (E2M1)
 constexpr int kUnitScale = 127;  // E8M0 scale 2^0
-constexpr int kStageMin = 16;    // survivors in a block that pay for staging its repeat lists
 
 // min of two non-negative floats (no NaN) as the unsigned min of their bits
 __device__ __forceinline__ float umin_f(float a, float b) {
@@ -281,8 +274,7 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
     const uint32_t *__restrict__ bitmap, const int32_t *__restrict__ Tn,
     const int32_t *__restrict__ rcnt, const int64_t *__restrict__ roff,
     const uint32_t *__restrict__ rlist, const int32_t *__restrict__ excess, int32_t n, int nb,
-    int64_t tile_begin, int64_t ntiles, int64_t per_xcd, const int64_t *__restrict__ sstart,
-    const int32_t *__restrict__ sij, int nst, float thr, float cut, int tshift, int kshift,
+    int64_t tile_begin, int64_t ntiles, int64_t per_xcd, float thr, float cut, int tshift, int kshift,
     unsigned long long *__restrict__ counter, uint64_t *__restrict__ out_keys,
     float *__restrict__ out_dists, int64_t cap) {
   static_assert(K >= 1 && K <= 6, "bitmap tiles hold k <= 6");
@@ -304,7 +296,7 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
   const int64_t local = (bid & 7) * per_xcd + (bid >> 3);
   if (local >= ntiles) return;
   int bi, bj;
-  super_tile_coords(tile_begin + local, sstart, sij, nst, nb, bi, bj);
+  if (!slot_coords(tile_begin + local, nb, bi, bj)) return;  // block-uniform
   const int row0 = bi * kTile, col0 = bj * kTile;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
@@ -322,18 +314,20 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
     sF[q] = ok && F > 0.0f ? F : 0.0f;
   }
 
-  // loader: thread t stages row t (read row0 + t) and row 128 + t (read col0 + t)
-  const bool aok = row0 + tid < n, bok = col0 + tid < n;
-  const uint4 *ga = reinterpret_cast<const uint4 *>(bitmap + (int64_t)(aok ? row0 + tid : 0) * WP);
-  const uint4 *gb = reinterpret_cast<const uint4 *>(bitmap + (int64_t)(bok ? col0 + tid : 0) * WP);
-  // rows past n load row 0 (always valid) and are zeroed by the mask
-  const uint32_t amask = aok ? 0xFFFFFFFFu : 0u, bmask = bok ? 0xFFFFFFFFu : 0u;
-  auto masked = [](uint4 v, uint32_t m) { return make_uint4(v.x & m, v.y & m, v.z & m, v.w & m); };
+  // loader: thread t stages row t (read row0 + t) and row 128 + t (read col0 + t);
+  // rows past n load read 0 (always valid): their pairs are masked in the epilogue
+  const uint4 *ga = reinterpret_cast<const uint4 *>(bitmap + (int64_t)(row0 + tid < n ? row0 + tid : 0) * WP);
+  const uint4 *gb = reinterpret_cast<const uint4 *>(bitmap + (int64_t)(col0 + tid < n ? col0 + tid : 0) * WP);
+  // (built field by field: a whole-struct copy into the arrays keeps them in scratch)
+  auto ld = [](const uint4 *p) {
+    const uint4 v = *p;
+    return make_uint4(v.x, v.y, v.z, v.w);
+  };
   uint4 pa[4], pb[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    pa[q] = masked(ga[q], amask);
-    pb[q] = masked(gb[q], bmask);
+    pa[q] = ld(ga + q);
+    pb[q] = ld(gb + q);
   }
 
   mfma_v16f acc[2][4];
@@ -385,8 +379,8 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
     if (ch + 1 < NCH) {  // prefetch the next chunk while this one is consumed
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        pa[q] = masked(ga[(ch + 1) * 4 + q], amask);
-        pb[q] = masked(gb[(ch + 1) * 4 + q], bmask);
+        pa[q] = ld(ga + (ch + 1) * 4 + q);
+        pb[q] = ld(gb + (ch + 1) * 4 + q);
       }
     }
 #pragma unroll
@@ -455,7 +449,7 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
   const int btot = sWave[0] + sWave[1];
   if (btot == 0) return;  // block-uniform
   bool staged = false;
-  if (!CAND && btot >= kStageMin) {  // block-uniform
+  if (!CAND) {  // block-uniform
     const int la = min(kTile, n - row0) - 1, lb = min(kTile, n - col0) - 1;
     const int64_t a0 = sRo[0], b0 = sRo[kTile];
     const int lenA = (int)(sRo[la] + sRc[la] - a0);
@@ -693,74 +687,30 @@ static int launch_profile(crispy_ctx *ctx, ReadStore &rs) {
   return CRISPY_OK;
 }
 
-// Host side of the tile order: super-tile table for nb tile rows, and the
-// (bi, bj) of every tile index (for the timing units of partial ranges).
-static void super_tiles(int nb, std::vector<int64_t> &start, std::vector<int32_t> &ij) {
-  const int ns = (nb + kSuperTiles - 1) / kSuperTiles;
-  start.assign(1, 0);
-  ij.clear();
-  for (int I = 0; I < ns; ++I)
-    for (int J = I; J < ns; ++J) {
-      const int64_t r0 = (int64_t)I * kSuperTiles, c0 = (int64_t)J * kSuperTiles;
-      const int64_t r1 = std::min<int64_t>(nb, r0 + kSuperTiles), c1 = std::min<int64_t>(nb, c0 + kSuperTiles);
-      const int64_t cnt = I < J ? (r1 - r0) * (c1 - c0) : (r1 - r0) * (r1 - r0 + 1) / 2;
-      ij.push_back((I << 16) | J);
-      start.push_back(start.back() + cnt);
-    }
-}
-
-static int ensure_super_table(crispy_ctx *ctx, int nb) {
-  KmerState &km = ctx->km;
-  if (km.super_nb == nb) return CRISPY_OK;
-  std::vector<int64_t> start;
-  std::vector<int32_t> ij;
-  super_tiles(nb, start, ij);
-  km.super_count = (int)ij.size();
-  CRISPY_TRY(ensure(ctx, km.sstart, sizeof(int64_t) * start.size()));
-  CRISPY_TRY(ensure(ctx, km.sij, sizeof(int32_t) * ij.size()));
-  CRISPY_HIP(ctx, hipMemcpyAsync(km.sstart.p, start.data(), sizeof(int64_t) * start.size(),
-                                 hipMemcpyHostToDevice, ctx->stream));
-  CRISPY_HIP(ctx, hipMemcpyAsync(km.sij.p, ij.data(), sizeof(int32_t) * ij.size(), hipMemcpyHostToDevice,
-                                 ctx->stream));
-  CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  km.super_nb = nb;
-  return CRISPY_OK;
-}
-
 template <int K, bool CAND>
 static int launch_tiles(crispy_ctx *ctx, int32_t n, int nb, int64_t tb, int64_t nt, float thr,
                         int tshift, uint64_t *out_keys, float *out_dists, int64_t cap) {
   KmerState &km = ctx->km;
-  CRISPY_TRY(ensure_super_table(ctx, nb));
   const int64_t per_xcd = (nt + 7) / 8;
   const int64_t grid = per_xcd * 8;
   hipEvent_t ev;
   timer_begin(ctx, "kmer_tile", &ev);
   kmer_tile_mfma_kernel<K, CAND><<<(unsigned)grid, 128, 0, ctx->stream>>>(
       km.bitmap.as<uint32_t>(), km.T.as<int32_t>(), km.rcnt.as<int32_t>(), km.roffc.as<int64_t>(),
-      km.rlistc.as<uint32_t>(), km.excess.as<int32_t>(), n, nb, tb, nt, per_xcd,
-      km.sstart.as<int64_t>(), km.sij.as<int32_t>(), km.super_count, thr, 1.0f - thr - 2e-5f, tshift,
-      km.kshift, km.counter.as<unsigned long long>(), out_keys, out_dists, cap);
+      km.rlistc.as<uint32_t>(), km.excess.as<int32_t>(), n, nb, tb, nt, per_xcd, thr, 1.0f - thr - 2e-5f,
+      tshift, km.kshift, km.counter.as<unsigned long long>(), out_keys, out_dists, cap);
   CRISPY_LAUNCHED(ctx);
   double units = 0;  // read pairs i < j < n inside the selected tiles
   if (ctx->timing) {
-    const int64_t total = (int64_t)nb * (nb + 1) / 2;
-    if (tb == 0 && nt == total) {
+    if (tb == 0 && nt == num_slots(nb)) {
       units = (double)n * (n - 1) / 2;
     } else {
-      std::vector<int64_t> start;
-      std::vector<int32_t> ij;
-      super_tiles(nb, start, ij);
-      for (size_t sidx = 0; sidx < ij.size(); ++sidx) {
-        const int I = ij[sidx] >> 16, J = ij[sidx] & 0xFFFF;
-        int64_t t = start[sidx];
-        for (int bi = I * kSuperTiles; bi < std::min(nb, (I + 1) * kSuperTiles); ++bi)
-          for (int bj = std::max(bi, J * kSuperTiles); bj < std::min(nb, (J + 1) * kSuperTiles); ++bj, ++t) {
-            if (t < tb || t >= tb + nt) continue;
-            const int64_t rh = std::min<int64_t>(kTile, n - (int64_t)bi * kTile);
-            const int64_t chh = std::min<int64_t>(kTile, n - (int64_t)bj * kTile);
-            units += bi == bj ? (double)rh * (rh - 1) / 2 : (double)rh * chh;
-          }
+      for (int64_t t = tb; t < tb + nt; ++t) {
+        int bi, bj;
+        if (!slot_coords(t, nb, bi, bj)) continue;
+        const int64_t rh = std::min<int64_t>(kTile, n - (int64_t)bi * kTile);
+        const int64_t chh = std::min<int64_t>(kTile, n - (int64_t)bj * kTile);
+        units += bi == bj ? (double)rh * (rh - 1) / 2 : (double)rh * chh;
       }
     }
   }
@@ -904,10 +854,7 @@ using namespace crispy;
 
 extern "C" {
 
-int64_t crispy_kmer_num_tiles(int32_t n) {
-  const int64_t nb = (n + kTile - 1) / kTile;
-  return nb * (nb + 1) / 2;
-}
+int64_t crispy_kmer_num_tiles(int32_t n) { return num_slots((n + kTile - 1) / kTile); }
 
 int crispy_kmer_run(crispy_ctx *ctx, int K, float threshold, int64_t tile_begin, int64_t tile_end,
                     int64_t *count) {
@@ -920,7 +867,7 @@ int crispy_kmer_run(crispy_ctx *ctx, int K, float threshold, int64_t tile_begin,
   ctx->nw.valid = false;
   const int32_t n = rs.n;
   const int nb = (n + kTile - 1) / kTile;
-  const int64_t total_tiles = (int64_t)nb * (nb + 1) / 2;
+  const int64_t total_tiles = num_slots(nb);
   if (tile_end < 0 || tile_end > total_tiles) tile_end = total_tiles;
   if (tile_begin < 0 || tile_begin > tile_end)
     return set_err(ctx, CRISPY_ERR_ARG, "bad tile range [%lld, %lld)", (long long)tile_begin,
@@ -938,7 +885,8 @@ int crispy_kmer_run(crispy_ctx *ctx, int K, float threshold, int64_t tile_begin,
   const int KP = K <= 6 ? K : 6;
   CRISPY_TRY(build_profiles(ctx, KP, K));
   // first guess at the result size: the pairs in the range, at most 4 Mi
-  const int64_t guess = std::min<int64_t>((int64_t)1 << 22, std::max<int64_t>(4096, nt * kTile * kTile));
+  const int64_t pairs_in_range = std::min<int64_t>(nt * kTile * kTile, (int64_t)n * (n - 1) / 2);
+  const int64_t guess = std::min<int64_t>((int64_t)1 << 22, std::max<int64_t>(4096, pairs_in_range));
   km.cap = std::max(km.cap, guess);
   if (K <= 6) {
     CRISPY_TRY(run_counted(ctx, km.cap, km.keys, &km.dists, &km.count, [&]() {

@@ -103,12 +103,14 @@ int crispy_reads_upload(crispy_ctx *ctx, int slot, const char *bases, const int6
  * kmerMain.cpp:156-159) -- including its K = 8 ordering -- and keeps pairs
  * i < j that pass `threshold`.  The result stays on the device in (i, j)
  * order; *count is the number of passing pairs.
- * tile_begin/tile_end select a range of the 128x128 upper-triangular tiles
- * (0 and -1 = all), which is how the work shards across devices.
+ * tile_begin/tile_end select a range of tile slots (0 and -1 = all), which
+ * is how the work shards across devices: the upper triangle of 128x128-read
+ * tiles in 16x16-tile super-tiles of 256 slots each (slots below the diagonal
+ * or past the last read are empty).
  * Requires 1 <= K <= 8 and every read length >= K. */
 int crispy_kmer_run(crispy_ctx *ctx, int K, float threshold, int64_t tile_begin,
                     int64_t tile_end, int64_t *count);
-/* Number of 128x128 tiles for n reads (the shard index space). */
+/* Number of tile slots for n reads (the shard index space). */
 int64_t crispy_kmer_num_tiles(int32_t num_reads);
 /* Copies result entries [first, first+count) to host: pairs as int32 (i, j),
  * dists as float32 (either pointer may be NULL). */
