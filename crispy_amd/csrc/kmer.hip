@@ -230,10 +230,10 @@ __host__ __device__ inline int64_t num_slots(int nb) {
 // their bit patterns), the sign of the difference shifted into a 16-bit reject
 // mask per fragment.  F <= 0 is stored as 0: such a pair survives either way.
 // Diagonal tiles (i < j) and the tiles of the last row or column block (reads
-// past n) mask their fragments explicitly.  The few survivors of a fragment go
-// through the wave's LDS queue, 64 at a time, for the exact count (popcount
-// plus the repeat-list merge when both reads have repeats) and the threshold;
-// the results return to the owning lanes and one atomic per wave reserves the
+// past n) mask their fragments explicitly.  The few survivors of the wave's 8
+// fragments are queued in LDS and processed 64 at a time for the exact count
+// (popcount plus the repeat-list merge when both reads have repeats) and the
+// threshold; one atomic per flush (normally one per wave) reserves the
 // output.  When the block has survivors, the tile's repeat lists are copied
 // into the then idle chunk buffer (two coalesced runs of the compacted lists)
 // if they fit, so the merges read LDS.
@@ -464,79 +464,73 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
     __syncthreads();
   }
 
-  // ---- survivors: exact count and threshold through the wave's queue; the
-  // results go back to the owning lanes (pass bit in keep, distance in the
-  // accumulator), then one atomic per wave reserves the output
+  // ---- survivors: queued per wave as li | lj' << 7 | popcount << 14 across
+  // the fragments, then flushed: the lanes take the entries 64 apart, write the
+  // exact count and the verdict back in place (li | lj' << 7 | m << 14 |
+  // pass << 31; m <= 2048), one atomic per flush reserves the passing pairs and
+  // the same lanes write them out.
   uint32_t *queue = sQ[wv];
-  if (!CAND) {
-    static_for<8>([&](auto F) {
-      constexpr int fa = decltype(F)::value / 4, fb = decltype(F)::value % 4;
-      uint32_t &k = keep[decltype(F)::value];
-      if (__ballot(k != 0) == 0) return;  // wave-uniform
-      const int c = __popc(k);
-      const int incl = wave_incl_sum(c, lane);
-      const int qn = __shfl(incl, 63, 64);
-      int o = incl - c;
-      static_for<16>([&](auto R) {
-        constexpr int r = decltype(R)::value;
-        if (k & (1u << r)) {
-          const int li = wv * 64 + fa * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
-          queue[o++] = (uint32_t)li | ((uint32_t)(fb * 32 + r32) << 7) | ((uint32_t)acc[fa][fb][r] << 14);
-        }
-      });
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      for (int e = lane; e < qn; e += 64) {
+  int qcount = 0;  // wave-uniform
+  auto flush = [&]() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    int mine = 0;
+    for (int e = lane; e < qcount; e += 64) {
+      if (CAND) {
+        ++mine;
+      } else {
         const uint32_t q = queue[e];
-        const int li = (int)(q & 127u);
-        const int lj = kTile + (int)((q >> 7) & 127u);
+        const int li = (int)(q & 127u), lj = kTile + (int)((q >> 7) & 127u);
         uint32_t m = q >> 14;
         const int na = sRc[li], nbr = sRc[lj];
         if (na > 0 && nbr > 0)
           m += staged ? repeat_min_sum(Ps + sLo[li], na, Ps + sLo[lj], nbr)
                       : repeat_min_sum(rlist + sRo[li], na, rlist + sRo[lj], nbr);
-        const float dist = 1.0f - (float)m / (float)min(sT[li], sT[lj]);
-        queue[e] = passes(dist, thr) ? __float_as_uint(dist) : 0xFFFFFFFFu;  // (a NaN: no pass)
+        const bool pass = passes(1.0f - (float)m / (float)min(sT[li], sT[lj]), thr);
+        queue[e] = (q & 0x3FFFu) | (m << 14) | (pass ? 0x80000000u : 0u);
+        mine += pass;
       }
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      o = incl - c;
-      static_for<16>([&](auto R) {
-        constexpr int r = decltype(R)::value;
-        if (k & (1u << r)) {
-          const uint32_t v = queue[o++];
-          if (v == 0xFFFFFFFFu)
-            k &= ~(1u << r);
-          else
-            acc[fa][fb][r] = __uint_as_float(v);
+    }
+    const int incl = wave_incl_sum(mine, lane);
+    const int total = __shfl(incl, 63, 64);
+    if (total > 0) {  // wave-uniform
+      unsigned long long base = 0;
+      if (lane == 63) base = atomicAdd(counter, (unsigned long long)total);
+      int64_t pos = (int64_t)__shfl(base, 63, 64) + (incl - mine);
+      for (int e = lane; e < qcount; e += 64) {
+        const uint32_t q = queue[e];
+        if (CAND || (q >> 31)) {
+          if (pos < cap) {
+            const int li = (int)(q & 127u), lj = kTile + (int)((q >> 7) & 127u);
+            out_keys[pos] = ((uint64_t)(row0 + li) << kshift) | (uint64_t)(col0 + lj - kTile);
+            if (!CAND)
+              out_dists[pos] = 1.0f - (float)((q >> 14) & 0x1FFFu) / (float)min(sT[li], sT[lj]);
+          }
+          ++pos;
         }
-      });
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    });
-  }
-  int mine = 0;
-#pragma unroll
-  for (int f = 0; f < 8; ++f) mine += __popc(keep[f]);
-  const int incl = wave_incl_sum(mine, lane);
-  const int total = __shfl(incl, 63, 64);
-  if (total == 0) return;  // wave-uniform
-  unsigned long long base = 0;
-  if (lane == 63) base = atomicAdd(counter, (unsigned long long)total);
-  int64_t pos = (int64_t)__shfl(base, 63, 64) + (incl - mine);
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    qcount = 0;
+  };
   static_for<8>([&](auto F) {
     constexpr int fa = decltype(F)::value / 4, fb = decltype(F)::value % 4;
     const uint32_t k = keep[decltype(F)::value];
     if (__ballot(k != 0) == 0) return;  // wave-uniform
+    const int c = __popc(k);
+    const int incl = wave_incl_sum(c, lane);
+    const int qn = __shfl(incl, 63, 64);
+    if (qcount + qn > 1024) flush();
+    int o = qcount + incl - c;
     static_for<16>([&](auto R) {
       constexpr int r = decltype(R)::value;
       if (k & (1u << r)) {
-        if (pos < cap) {
-          const int li = wv * 64 + fa * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
-          out_keys[pos] = ((uint64_t)(row0 + li) << kshift) | (uint64_t)(col0 + fb * 32 + r32);
-          if (!CAND) out_dists[pos] = acc[fa][fb][r];
-        }
-        ++pos;
+        const int li = wv * 64 + fa * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
+        queue[o++] = (uint32_t)li | ((uint32_t)(fb * 32 + r32) << 7) | ((uint32_t)acc[fa][fb][r] << 14);
       }
     });
+    qcount += qn;
   });
+  if (qcount > 0) flush();
 }
 
 __global__ void unpack_keys_kernel(const uint64_t *__restrict__ keys, int64_t n, int kshift,
