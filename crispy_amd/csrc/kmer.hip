@@ -617,16 +617,26 @@ __global__ __launch_bounds__(256) void kmer_cand_kernel(
     if (c < ncand) {
       key = cand[c];
       const int i = (int)(key >> kshift), j = (int)(key & ((1ull << kshift) - 1));
-      const uint16_t *A = lists + loff[i];
-      const uint16_t *Bl = lists + loff[j];
+      // the lists start at multiples of 4 codes: read them 4 at a time, the
+      // next word already in flight (the buffer has 8 codes of slack at the end)
+      const uint64_t *A = reinterpret_cast<const uint64_t *>(lists + loff[i]);
+      const uint64_t *Bl = reinterpret_cast<const uint64_t *>(lists + loff[j]);
       const unsigned short l1 = (unsigned short)(len[i] - K + 1), l2 = (unsigned short)(len[j] - K + 1);
+      uint64_t a0 = A[0], a1 = A[1], b0 = Bl[0], b1 = Bl[1];
       unsigned short matches = 0;
       int k = 0, l = 0;
       while (k < l1 && l < l2) {
-        const unsigned short x = A[k], y = Bl[l];
+        const uint32_t x = (uint32_t)(a0 >> (16 * (k & 3))) & 0xFFFFu;
+        const uint32_t y = (uint32_t)(b0 >> (16 * (l & 3))) & 0xFFFFu;
         matches = (unsigned short)(matches + (x == y));
-        k += x <= y;
-        l += x >= y;
+        if (x <= y && (++k & 3) == 0) {
+          a0 = a1;
+          a1 = A[(k >> 2) + 1];
+        }
+        if (x >= y && (++l & 3) == 0) {
+          b0 = b1;
+          b1 = Bl[(l >> 2) + 1];
+        }
       }
       const unsigned short mn = l1 < l2 ? l1 : l2;
       dist = 1.0f - (float)matches / (float)mn;
@@ -769,7 +779,7 @@ static int build_profiles(crispy_ctx *ctx, int KP, int K) {
   }
   if (K > 6) {  // the K-mer lists for the exact merge of the candidates
     std::vector<int64_t> loff(n + 1, 0);
-    for (int32_t r = 0; r < n; ++r) loff[r + 1] = loff[r] + (rs.h_len[r] - K + 1);
+    for (int32_t r = 0; r < n; ++r) loff[r + 1] = loff[r] + ((rs.h_len[r] - K + 1 + 3) & ~3);  // 8-byte aligned
     CRISPY_TRY(ensure(ctx, km.loff, sizeof(int64_t) * (n + 1)));
     CRISPY_TRY(ensure(ctx, km.lists, sizeof(uint16_t) * (loff[n] + 8)));
     CRISPY_HIP(ctx, hipMemcpyAsync(km.loff.p, loff.data(), sizeof(int64_t) * (n + 1),

@@ -186,7 +186,7 @@ int64_t crispy_server_job_bytes(const char *bases, const int64_t *offsets, int32
   const int64_t pairs = std::min<int64_t>(all_pairs, 256 * (int64_t)n);
   const int64_t reads = 2 * (nbases + nbases / 4 + 48 * (int64_t)n);
   int64_t prof = (int64_t)n * (4 * words_per_read(K) + 48) + 6 * nbases;
-  if (K > 6) prof += 2 * nbases + 8 * (int64_t)n + 16 * std::max(guess, pairs);  // candidates + sorted copy
+  if (K > 6) prof += 2 * nbases + 16 * (int64_t)n + 16 * std::max(guess, pairs);  // lists, candidates + sorted copy
   int64_t est = reads + prof + 12 * guess + 72 * pairs + (int64_t)(8 << 20);
   const bool generic = band <= 1 || !clean || (band > 0 && maxlen / band > kSlicedMaxHalfBand);
   if (generic) est += 8 * std::min<int64_t>(pairs, 32768) * (maxlen + 1);
