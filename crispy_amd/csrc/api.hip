@@ -209,7 +209,7 @@ int crispy_ctx_create(int device, crispy_ctx **out) {
 }
 
 static void release_all(crispy_ctx *ctx) {
-  DevBuf *bufs[] = {&ctx->km.bitmap,   &ctx->km.T,         &ctx->km.hinfo,     &ctx->km.rcnt,
+  DevBuf *bufs[] = {&ctx->km.bitmap,   &ctx->km.T,         &ctx->km.rcnt,
                     &ctx->km.roff,     &ctx->km.rlist,     &ctx->km.excess,    &ctx->km.loff,
                     &ctx->km.roffc,    &ctx->km.rlistc,
                     &ctx->km.lists,    &ctx->km.cand,      &ctx->km.keys,      &ctx->km.keys_alt,

@@ -95,7 +95,7 @@ struct KmerState {
   // per-read profile for the current K
   int K = 0;
   int profile_gen = -1;
-  DevBuf bitmap, T, hinfo, rcnt, roff, rlist, excess;
+  DevBuf bitmap, T, rcnt, roff, rlist, excess;
   // the repeat lists back to back in read order (rlistc + roffc[r])
   DevBuf roffc, rlistc;
   // K = 7, 8: signed-sorted K-mer lists and the k = 6 candidate pairs

@@ -15,9 +15,9 @@
 //   * a presence bitmap of 4^K bits (padded to 16-word chunks), and
 //   * a short sorted list of (code << 16 | count-1) for codes seen twice or
 //     more ("repeats"),
-// and a 128x128 tile of pairs is an AND + popcount "GEMM" over the bitmaps
-// (v_bcnt_u32_b32 accumulates) plus a merge of the two repeat lists for
-// the pairs the popcount bound does not settle.  The filter dist <= thr and
+// and a 128x128 tile of pairs is a {0,1} matrix product of the bitmaps on the
+// matrix cores plus a merge of the two repeat lists for the pairs the
+// popcount bound does not settle.  The filter dist <= thr and
 // the compaction of passing pairs run in the same kernel; only passing pairs
 // leave the chip.  K = 7, 8 use the k = 6 tiles as a candidate filter and the
 // reference's merge on the candidates (kmer_cand_kernel).
@@ -39,73 +39,56 @@
 namespace crispy {
 
 // --------------------------------------------------------------- profiles
-// One block per read: codes of the T = L-K+1 k-mers (first base most
-// significant, as READ::formTuples kmerMain.cpp:117-154), a bitonic sort in
-// LDS, then runs -> bitmap bits, repeat entries and hinfo.
+// One block per read: the codes of its T = L-K+1 k-mers (first base most
+// significant, as READ::formTuples kmerMain.cpp:117-154) counted in an LDS
+// histogram over all 4^K codes; the bins then give the presence bitmap, the
+// repeat entries in code order and excess = T - (distinct codes).
 template <int K>
 __global__ __launch_bounds__(256) void kmer_profile_kernel(
     const uint32_t *__restrict__ pack, const int64_t *__restrict__ poff,
     const int32_t *__restrict__ len, uint32_t *__restrict__ bitmap, int32_t *__restrict__ Tout,
-    uint32_t *__restrict__ hinfo, int32_t *__restrict__ rcnt, const int64_t *__restrict__ roff,
-    uint32_t *__restrict__ rlist, int32_t *__restrict__ excess) {
+    int32_t *__restrict__ rcnt, const int64_t *__restrict__ roff, uint32_t *__restrict__ rlist,
+    int32_t *__restrict__ excess) {
   constexpr int WP = kmer_words(K);
-  __shared__ uint32_t codes[CRISPY_MAX_READ_LEN];
+  constexpr int NBINS = 1 << (2 * K);
+  constexpr int BPT = NBINS >= 256 ? NBINS / 256 : 1;  // bins per thread (within one bitmap word)
+  __shared__ uint32_t hist[NBINS];
   __shared__ uint32_t bm[WP];
   __shared__ int scan[256];
   __shared__ int s_distinct;
   const int r = blockIdx.x;
   const int tid = threadIdx.x;
-  const int L = len[r];
-  const int T = L - K + 1;  // >= 1, checked by the host
+  const int T = len[r] - K + 1;  // >= 1, checked by the host
   const uint32_t *s = pack + poff[r];
-  int P = 1;
-  while (P < T) P <<= 1;
-  for (int t = tid; t < P; t += 256) {
-    uint32_t c = 0xFFFFFFFFu;
-    if (t < T) {
-      c = 0;
-#pragma unroll
-      for (int q = 0; q < K; ++q) {
-        const int p = t + q;
-        c = (c << 2) | ((s[p >> 4] >> (2 * (p & 15))) & 3u);
-      }
-    }
-    codes[t] = c;
-  }
+  for (int v = tid; v < NBINS; v += 256) hist[v] = 0;
   for (int w = tid; w < WP; w += 256) bm[w] = 0;
   if (tid == 0) s_distinct = 0;
   __syncthreads();
-  for (int k = 2; k <= P; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int t = tid; t < P; t += 256) {
-        const int o = t ^ j;
-        if (o > t) {
-          const uint32_t a = codes[t], b = codes[o];
-          const bool up = (t & k) == 0;
-          if ((a > b) == up) {
-            codes[t] = b;
-            codes[o] = a;
-          }
-        }
-      }
-      __syncthreads();
+  for (int t = tid; t < T; t += 256) {
+    uint32_t c = 0;
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      const int p = t + q;
+      c = (c << 2) | ((s[p >> 4] >> (2 * (p & 15))) & 3u);
     }
+    atomicAdd(&hist[c], 1u);
   }
-  // runs: each thread owns the run starts inside its contiguous segment
-  const int seg = (T + 255) / 256;
-  const int lo = min(tid * seg, T), hi = min(lo + seg, T);
-  int nrep = 0, nrun = 0;
-  for (int x = lo; x < hi; ++x) {
-    const uint32_t v = codes[x];
-    if (x == 0 || codes[x - 1] != v) {
-      int c = 1;
-      while (x + c < T && codes[x + c] == v) ++c;
-      atomicOr(&bm[v >> 5], 1u << (v & 31));
+  __syncthreads();
+  // thread tid owns bins [tid * BPT, tid * BPT + BPT)
+  const int b0 = tid * BPT;
+  uint32_t bits = 0;
+  int nrep = 0, ndist = 0;
+  if (b0 < NBINS) {
+#pragma unroll
+    for (int k = 0; k < BPT; ++k) {
+      const uint32_t c = hist[b0 + k];
+      bits |= (c > 0 ? 1u : 0u) << k;
+      ndist += c > 0;
       nrep += c >= 2;
-      ++nrun;
     }
+    if (bits) atomicOr(&bm[b0 >> 5], bits << (b0 & 31));
   }
-  if (nrun) atomicAdd(&s_distinct, nrun);
+  if (ndist) atomicAdd(&s_distinct, ndist);
   scan[tid] = nrep;
   __syncthreads();
   for (int off = 1; off < 256; off <<= 1) {
@@ -114,30 +97,19 @@ __global__ __launch_bounds__(256) void kmer_profile_kernel(
     scan[tid] += add;
     __syncthreads();
   }
-  int pos = scan[tid] - nrep;
-  uint32_t *out = rlist + roff[r];
-  for (int x = lo; x < hi; ++x) {
-    const uint32_t v = codes[x];
-    if (x == 0 || codes[x - 1] != v) {
-      int c = 1;
-      while (x + c < T && codes[x + c] == v) ++c;
-      if (c >= 2) out[pos++] = (v << 16) | (uint32_t)(c - 1);
+  if (nrep) {
+    uint32_t *out = rlist + roff[r] + (scan[tid] - nrep);
+#pragma unroll
+    for (int k = 0; k < BPT; ++k) {
+      const uint32_t c = hist[b0 + k];
+      if (c >= 2) *out++ = ((uint32_t)(b0 + k) << 16) | (c - 1);
     }
   }
   if (tid == 255) rcnt[r] = scan[255];
   if (tid == 0) {
     Tout[r] = T;
     excess[r] = T - s_distinct;  // sum over codes of (count - 1)
-    const uint32_t v = codes[T - 1];
-    uint32_t h = 0;
-    if (v >= 0x8000u) {
-      int c = 1;
-      while (T - 1 - c >= 0 && codes[T - 1 - c] == v) ++c;
-      h = (v << 16) | (uint32_t)c;
-    }
-    hinfo[r] = h;
   }
-  __syncthreads();
   uint32_t *dst = bitmap + (int64_t)r * WP;
   for (int w = tid; w < WP; w += 256) dst[w] = bm[w];
 }
@@ -684,7 +656,7 @@ static int launch_profile(crispy_ctx *ctx, ReadStore &rs) {
   timer_begin(ctx, "kmer_profile", &ev);
   kmer_profile_kernel<K><<<rs.n, 256, 0, ctx->stream>>>(
       rs.pack.as<uint32_t>(), rs.poff.as<int64_t>(), rs.len.as<int32_t>(), km.bitmap.as<uint32_t>(),
-      km.T.as<int32_t>(), km.hinfo.as<uint32_t>(), km.rcnt.as<int32_t>(), km.roff.as<int64_t>(),
+      km.T.as<int32_t>(), km.rcnt.as<int32_t>(), km.roff.as<int64_t>(),
       km.rlist.as<uint32_t>(), km.excess.as<int32_t>());
   CRISPY_LAUNCHED(ctx);
   timer_end(ctx, "kmer_profile", ev, rs.n);
@@ -752,7 +724,6 @@ static int build_profiles(crispy_ctx *ctx, int KP, int K) {
   const int WP = kmer_words(KP);
   CRISPY_TRY(ensure(ctx, km.bitmap, sizeof(uint32_t) * (size_t)n * WP));
   CRISPY_TRY(ensure(ctx, km.T, sizeof(int32_t) * (n + 1)));
-  CRISPY_TRY(ensure(ctx, km.hinfo, sizeof(uint32_t) * (n + 1)));
   CRISPY_TRY(ensure(ctx, km.rcnt, sizeof(int32_t) * (n + 1)));
   CRISPY_TRY(ensure(ctx, km.excess, sizeof(int32_t) * (n + 1)));
   CRISPY_TRY(ensure(ctx, km.roff, sizeof(int64_t) * (n + 1)));
