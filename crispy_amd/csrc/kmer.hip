@@ -35,6 +35,7 @@
 #include <utility>
 
 #include "common.hpp"
+#include "sort_config.hpp"
 
 namespace crispy {
 
@@ -808,14 +809,14 @@ static int sort_keys(crispy_ctx *ctx, int64_t count) {
     return CRISPY_OK;
   }
   size_t tbytes = 0;
-  CRISPY_HIP(ctx, rocprim::radix_sort_pairs(nullptr, tbytes, km.keys.as<uint64_t>(),
+  CRISPY_HIP(ctx, rocprim::radix_sort_pairs<RadixOnesweep>(nullptr, tbytes, km.keys.as<uint64_t>(),
                                             km.keys_alt.as<uint64_t>(), km.dists.as<float>(),
                                             km.dists_alt.as<float>(), (size_t)count, 0, 2 * km.kshift,
                                             ctx->stream));
   CRISPY_TRY(ensure(ctx, ctx->temp, tbytes));
   hipEvent_t ev;
   timer_begin(ctx, "kmer_sort", &ev);
-  CRISPY_HIP(ctx, rocprim::radix_sort_pairs(ctx->temp.p, tbytes, km.keys.as<uint64_t>(),
+  CRISPY_HIP(ctx, rocprim::radix_sort_pairs<RadixOnesweep>(ctx->temp.p, tbytes, km.keys.as<uint64_t>(),
                                             km.keys_alt.as<uint64_t>(), km.dists.as<float>(),
                                             km.dists_alt.as<float>(), (size_t)count, 0, 2 * km.kshift,
                                             ctx->stream));
@@ -881,13 +882,13 @@ int crispy_kmer_run(crispy_ctx *ctx, int K, float threshold, int64_t tile_begin,
     if (ncand > 1) {
       CRISPY_TRY(ensure(ctx, km.cand_alt, sizeof(uint64_t) * ncand));
       size_t tbytes = 0;
-      CRISPY_HIP(ctx, rocprim::radix_sort_keys(nullptr, tbytes, km.cand.as<uint64_t>(),
+      CRISPY_HIP(ctx, rocprim::radix_sort_keys<RadixOnesweep>(nullptr, tbytes, km.cand.as<uint64_t>(),
                                                km.cand_alt.as<uint64_t>(), (size_t)ncand, 0, 2 * km.kshift,
                                                ctx->stream));
       CRISPY_TRY(ensure(ctx, ctx->temp, tbytes));
       hipEvent_t sev;
       timer_begin(ctx, "kmer_cand_sort", &sev);
-      CRISPY_HIP(ctx, rocprim::radix_sort_keys(ctx->temp.p, tbytes, km.cand.as<uint64_t>(),
+      CRISPY_HIP(ctx, rocprim::radix_sort_keys<RadixOnesweep>(ctx->temp.p, tbytes, km.cand.as<uint64_t>(),
                                                km.cand_alt.as<uint64_t>(), (size_t)ncand, 0, 2 * km.kshift,
                                                ctx->stream));
       timer_end(ctx, "kmer_cand_sort", sev, (double)ncand);

@@ -44,6 +44,7 @@
 #include <type_traits>
 
 #include "common.hpp"
+#include "sort_config.hpp"
 #include "nw_fused.hpp"
 
 namespace crispy {
@@ -295,11 +296,11 @@ static int nw_core(crispy_ctx *ctx, const int2 *d_pairs, int64_t np, int band) {
                                  hipMemcpyDeviceToHost, ctx->stream));
   CRISPY_HIP(ctx, hipEventRecord(ctx->hist_ev, ctx->stream));
   size_t tbytes = 0;
-  CRISPY_HIP(ctx, rocprim::radix_sort_pairs(nullptr, tbytes, nw.keys.as<uint32_t>(),
+  CRISPY_HIP(ctx, rocprim::radix_sort_pairs<RadixOnesweep>(nullptr, tbytes, nw.keys.as<uint32_t>(),
                                             nw.keys_alt.as<uint32_t>(), nw.idx.as<uint32_t>(),
                                             nw.idx_alt.as<uint32_t>(), (size_t)np, 0, kBucketKeyBits, ctx->stream));
   CRISPY_TRY(ensure(ctx, ctx->temp, tbytes));
-  CRISPY_HIP(ctx, rocprim::radix_sort_pairs(ctx->temp.p, tbytes, nw.keys.as<uint32_t>(),
+  CRISPY_HIP(ctx, rocprim::radix_sort_pairs<RadixOnesweep>(ctx->temp.p, tbytes, nw.keys.as<uint32_t>(),
                                             nw.keys_alt.as<uint32_t>(), nw.idx.as<uint32_t>(),
                                             nw.idx_alt.as<uint32_t>(), (size_t)np, 0, kBucketKeyBits, ctx->stream));
   timer_end(ctx, "nw_bucket", ev, (double)np);
@@ -571,11 +572,11 @@ int crispy_sort_by_dist(crispy_ctx *ctx, int32_t *pairs, float *dists, int64_t n
   CRISPY_LAUNCHED(ctx);
   // 1) by (i, j)   2) stable by dist  ->  (dist, i, j)
   size_t t1 = 0, t2 = 0;
-  CRISPY_HIP(ctx, rocprim::radix_sort_pairs(nullptr, t1, k0, k1, b0, b1, (size_t)n, 0, 40, ctx->stream));
-  CRISPY_HIP(ctx, rocprim::radix_sort_pairs(nullptr, t2, b1, b0, k1, k0, (size_t)n, 0, 32, ctx->stream));
+  CRISPY_HIP(ctx, rocprim::radix_sort_pairs<RadixOnesweep>(nullptr, t1, k0, k1, b0, b1, (size_t)n, 0, 40, ctx->stream));
+  CRISPY_HIP(ctx, rocprim::radix_sort_pairs<RadixOnesweep>(nullptr, t2, b1, b0, k1, k0, (size_t)n, 0, 32, ctx->stream));
   CRISPY_TRY(ensure(ctx, ctx->temp, std::max(t1, t2)));
-  CRISPY_HIP(ctx, rocprim::radix_sort_pairs(ctx->temp.p, t1, k0, k1, b0, b1, (size_t)n, 0, 40, ctx->stream));
-  CRISPY_HIP(ctx, rocprim::radix_sort_pairs(ctx->temp.p, t2, b1, b0, k1, k0, (size_t)n, 0, 32, ctx->stream));
+  CRISPY_HIP(ctx, rocprim::radix_sort_pairs<RadixOnesweep>(ctx->temp.p, t1, k0, k1, b0, b1, (size_t)n, 0, 40, ctx->stream));
+  CRISPY_HIP(ctx, rocprim::radix_sort_pairs<RadixOnesweep>(ctx->temp.p, t2, b1, b0, k1, k0, (size_t)n, 0, 32, ctx->stream));
   unpack_pair_keys_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(k0, b0, n, nw.in_pairs.as<int2>(),
                                                                  nw.dist.as<float>());
   CRISPY_LAUNCHED(ctx);
