@@ -67,6 +67,7 @@ def load_library():
         "crispy_nw_fetch": ([_vp, ctypes.c_int64, ctypes.c_int64, _i32p, _f32p], ctypes.c_int),
         "crispy_nw_dists": ([_vp, _i32p, ctypes.c_int64, ctypes.c_int, _f32p], ctypes.c_int),
         "crispy_nw_csr": ([_vp, ctypes.c_int32, _i64p, _i32p, _f32p], ctypes.c_int),
+        "crispy_nw_set_slice_width": ([_vp, ctypes.c_int], ctypes.c_int),
         "crispy_sort_by_dist": ([_vp, _i32p, _f32p, ctypes.c_int64], ctypes.c_int),
         "crispy_timing_enable": ([_vp, ctypes.c_int], ctypes.c_int),
         "crispy_timing_reset": ([_vp], ctypes.c_int),
@@ -299,6 +300,10 @@ class Context:
         d = np.zeros(count, np.float32)
         self._check(self._L.crispy_nw_fetch(self._h, first, count, _p(p, _i32p), _p(d, _f32p)))
         return p, d
+
+    def nw_slice_width(self, cells):
+        """Band cells per lane of the lane-sliced NW kernel: 0 = cost model (default), 24 or 32."""
+        self._check(self._L.crispy_nw_set_slice_width(self._h, cells))
 
     def nw_csr(self, n_rows):
         """(row_ptr, cols, vals) of the last NW result, built on the device."""

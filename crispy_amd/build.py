@@ -27,7 +27,8 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["api.hip", "kmer.hip", "nw.hip", "preprocess.hip", "nw_fam_reg0.hip", "nw_fam_reg1.hip",
-               "nw_fam_reg2.hip", "nw_fam_reg3.hip", "nw_fam_sliced.hip"]
+               "nw_fam_reg2.hip", "nw_fam_reg3.hip", "nw_fam_sliced24.hip",
+               "nw_fam_sliced32.hip"]
 HOST_SOURCES = ["host.cpp", "serve.cpp"]
 CLI = {"kmerDist": "cli_kmer.cpp", "genDist": "cli_gen.cpp", "preprocess": "cli_preprocess.cpp"}
 

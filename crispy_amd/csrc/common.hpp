@@ -115,6 +115,7 @@ struct NwState {
   DevBuf out_pairs, out_dists, scratch;
   DevBuf tasks;  // bucket table + wave tasks of the fused DP launches
   DevBuf csr_rows, csr_cols;  // CSR view of the (i, j)-ordered result
+  int slice_width = 0;  // sliced-kernel cells per lane: 0 = cost model, 24 or 32 = forced
   int64_t count = 0;
   bool valid = false;
 };

@@ -263,7 +263,8 @@ static inline int grid_for(int64_t n) { return (int)std::max<int64_t>(1, std::mi
 using FamilyLaunch = void (*)(hipStream_t, unsigned, const NwFusedArgs &);
 static const FamilyLaunch kFamilyLaunch[kNumFamilies] = {
     launch_nw_reg_family0, launch_nw_reg_family1, launch_nw_reg_family2, launch_nw_reg_family3,
-    launch_nw_sliced_family};
+    launch_nw_sliced_family24, launch_nw_sliced_family32};
+static_assert(kNumFamilies == 6 && kSliceWidths[0] == 24 && kSliceWidths[1] == 32, "one launcher per family");
 
 // Distances of np device pairs into nw.dist (indexed like the pairs).
 static int nw_core(crispy_ctx *ctx, const int2 *d_pairs, int64_t np, int band) {
@@ -330,8 +331,9 @@ static int nw_core(crispy_ctx *ctx, const int2 *d_pairs, int64_t np, int band) {
   std::vector<uint2> tasks;
   int64_t fam_first[kNumFamilies + 1] = {};
   double fam_cells[kNumFamilies] = {};
-  auto family_of = [](int b) {
-    if (b > kKbRegMax) return kNumFamilies - 1;
+  const int forced = nw.slice_width;
+  auto family_of = [forced](int b) {
+    if (b > kKbRegMax) return kNumRegFamilies + (slice_width_for(b, forced) == kSliceWidths[0] ? 0 : 1);
     int f = 0;
     while (b > kRegFamilies[f].hi) ++f;
     return f;
@@ -346,7 +348,7 @@ static int nw_core(crispy_ctx *ctx, const int2 *d_pairs, int64_t np, int band) {
     });
     for (int b : bs) {
       btab[b] = make_uint2((uint32_t)start[b], hist[b]);
-      const int per_wave = b > kKbRegMax ? 64 / ((2 * b + 1 + kSliceCells - 1) / kSliceCells) : 64;
+      const int per_wave = b > kKbRegMax ? 64 / slice_lanes(b, kSliceWidths[f - kNumRegFamilies]) : 64;
       const int64_t waves = ((int64_t)hist[b] + per_wave - 1) / per_wave;
       for (int64_t w = waves - 1; w >= 0; --w) tasks.push_back(make_uint2((uint32_t)b, (uint32_t)w));
       fam_cells[f] += (double)cells[b];
@@ -379,7 +381,7 @@ static int nw_core(crispy_ctx *ctx, const int2 *d_pairs, int64_t np, int band) {
     const int wk = (int)(std::min_element(load, load + kNumWorkers) - load);
     load[wk] += fam_cells[f];
     hipStream_t s = ctx->workers[wk];
-    const char *name = f == kNumFamilies - 1 ? "nw_band_sliced" : "nw_band_reg";
+    const char *name = f >= kNumRegFamilies ? "nw_band_sliced" : "nw_band_reg";
     const NwFusedArgs args{rs.pack.as<uint32_t>(), rs.poff.as<int64_t>(), rs.len.as<int32_t>(),
                            d_pairs, order, d_btab, d_tasks + fam_first[f], out};
     timer_begin(ctx, name, &ev, s);
@@ -502,6 +504,15 @@ int crispy_nw_fetch(crispy_ctx *ctx, int64_t first, int64_t count, int32_t *pair
     CRISPY_HIP(ctx, hipMemcpyAsync(dists, nw.out_dists.as<float>() + first, sizeof(float) * count,
                                    hipMemcpyDeviceToHost, ctx->stream));
   CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return CRISPY_OK;
+}
+
+int crispy_nw_set_slice_width(crispy_ctx *ctx, int cells) {
+  if (!ctx) return CRISPY_ERR_ARG;
+  if (cells != 0 && cells != kSliceWidths[0] && cells != kSliceWidths[1])
+    return set_err(ctx, CRISPY_ERR_ARG, "slice width %d: 0 (cost model), %d or %d", cells, kSliceWidths[0],
+                   kSliceWidths[1]);
+  ctx->nw.slice_width = cells;
   return CRISPY_OK;
 }
 

@@ -233,8 +233,8 @@ __device__ __forceinline__ void nw_reg_wave(const uint32_t *__restrict__ pack,
 constexpr int kDppWaveShl1 = 0x130;  // lane l takes lane l+1
 constexpr int kDppWaveShr1 = 0x138;  // lane l takes lane l-1
 
-// Wide bands (kB > kKbRegMax): G = ceil(W / 24) lanes per pair, lane g owns
-// band positions [24g, 24g + 24) of every row in VGPRs (same per-cell state
+// Wide bands (kB > kKbRegMax): G = ceil(W / C) lanes per pair, lane g owns
+// band positions [Cg, Cg + C) of every row in VGPRs (same per-cell state
 // and arithmetic as nw_band_reg_kernel).  Lane g processes row t - g at step
 // t, one row behind lane g - 1, so per step it needs only
 //   * "up" for its first cell: lane g-1's last cell of the same row, sent at
@@ -242,7 +242,7 @@ constexpr int kDppWaveShr1 = 0x138;  // lane l takes lane l-1
 //   * "left" for its last cell: lane g+1's first cell of the previous row,
 //     which lane g+1 computes first in this step and sends mid-step.
 // Positions >= W in the last lane are dead; the last real cell (KEND in the
-// last lane, KEND = (W - 1) mod 24) sees an out-of-band left neighbour.
+// last lane, KEND = (W - 1) mod C) sees an out-of-band left neighbour.
 template <int KEND, int MODE>
 __device__ __forceinline__ void nw_slice_cell(const int k, const int i, const uint32_t s,
                                               int &Mt_k, int &Mv_k, uint32_t &LA_k,
@@ -282,7 +282,7 @@ __device__ __forceinline__ void nw_slice_cell(const int k, const int i, const ui
 
 // One wave: 64 / G pairs starting at order[wave * (64 / G)]; `red` is the
 // wave's 64 x 6 LDS reduction buffer.
-template <int KEND>
+template <int C, int KEND>
 __device__ __forceinline__ void nw_sliced_wave(const uint32_t *__restrict__ pack,
                                                const int64_t *__restrict__ poff,
                                                const int32_t *__restrict__ len,
@@ -290,7 +290,6 @@ __device__ __forceinline__ void nw_sliced_wave(const uint32_t *__restrict__ pack
                                                const uint32_t *__restrict__ order, int32_t count,
                                                int KB, int G, int wave, int (*red)[6],
                                                float *__restrict__ out) {
-  constexpr int C = kSliceCells;
   constexpr int NMW = (C + 15) / 16;
   constexpr int NB = NMW + 1;
   const int W = 2 * KB + 1;
@@ -517,18 +516,34 @@ __global__ __launch_bounds__(64, WAVES) void nw_reg_fused_kernel(
                        (int)task.y * 64, out);
 }
 
-template <int KEND>
+template <int C, int KEND>
 __device__ __forceinline__ void sliced_dispatch(int kend, const uint32_t *pack, const int64_t *poff,
                                                 const int32_t *len, const int2 *pairs,
                                                 const uint32_t *order, int32_t count, int kb, int G,
                                                 int wave, int (*red)[6], float *out) {
-  if constexpr (KEND < kSliceCells) {
+  if constexpr (KEND < C) {
     if (kend == KEND) {
-      nw_sliced_wave<KEND>(pack, poff, len, pairs, order, count, kb, G, wave, red, out);
+      nw_sliced_wave<C, KEND>(pack, poff, len, pairs, order, count, kb, G, wave, red, out);
       return;
     }
-    sliced_dispatch<KEND + 2>(kend, pack, poff, len, pairs, order, count, kb, G, wave, red, out);
+    sliced_dispatch<C, KEND + 2>(kend, pack, poff, len, pairs, order, count, kb, G, wave, red, out);
   }
+}
+
+// Lane-sliced kernel at slice width C for the buckets assigned to it.
+template <int C>
+__global__ __launch_bounds__(64) void nw_sliced_fused_kernel(
+    const uint32_t *__restrict__ pack, const int64_t *__restrict__ poff,
+    const int32_t *__restrict__ len, const int2 *__restrict__ pairs,
+    const uint32_t *__restrict__ order, const uint2 *__restrict__ btab,
+    const uint2 *__restrict__ tasks, float *__restrict__ out) {
+  __shared__ int red[64][6];
+  const uint2 task = tasks[blockIdx.x];
+  const uint2 bk = btab[task.x];
+  const int kb = (int)task.x;
+  // KEND = (W - 1) mod C = 2kB mod C, even
+  sliced_dispatch<C, 0>((2 * kb) % C, pack, poff, len, pairs, order + bk.x, (int32_t)bk.y, kb,
+                        slice_lanes(kb, C), (int)task.y, red, out);
 }
 
 }  // namespace crispy

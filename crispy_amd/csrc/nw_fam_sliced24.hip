@@ -1,0 +1,12 @@
+// Lane-sliced family at 24 band cells per lane: the kB buckets above
+// kKbRegMax that slice_width_for() assigns to this width, in one fused launch.
+#include "nw_dp.hpp"
+
+namespace crispy {
+
+void launch_nw_sliced_family24(hipStream_t s, unsigned blocks, const NwFusedArgs &a) {
+  nw_sliced_fused_kernel<24><<<blocks, 64, 0, s>>>(a.pack, a.poff, a.len, a.pairs, a.order, a.btab, a.tasks,
+                                                  a.out);
+}
+
+}  // namespace crispy

@@ -22,8 +22,28 @@ static_assert(kRegFamilies[0].lo == 0 && kRegFamilies[0].hi + 1 == kRegFamilies[
                   kRegFamilies[1].hi + 1 == kRegFamilies[2].lo &&
                   kRegFamilies[2].hi + 1 == kRegFamilies[3].lo && kRegFamilies[3].hi == kKbRegMax,
               "the register families cover kB 0..kKbRegMax without gaps");
-constexpr int kNumFamilies = kNumRegFamilies + 1;  // + the sliced family
-constexpr int kSliceCells = 24;                    // band cells per lane, sliced kernel
+// Lane-sliced families: G = ceil(W / C) lanes per pair, C band cells per lane,
+// one family per slice width.  Which width a bucket runs at comes from
+// slice_width_for().
+constexpr int kNumSliceWidths = 2;
+constexpr int kSliceWidths[kNumSliceWidths] = {24, 32};
+constexpr int kNumFamilies = kNumRegFamilies + kNumSliceWidths;
+
+__host__ __device__ constexpr int slice_lanes(int kb, int C) { return (2 * kb + 1 + C - 1) / C; }
+
+// Cost model of one wave step of the sliced kernel: ~13.5 C VALU ops for the C
+// cells of every lane plus ~45 per step (match words, neighbour exchange,
+// window slide), shared by the 64 / G pairs of the wave.  Measured on MI355X
+// the model holds for wide bands but not for G <= 6 lanes at 24 cells (kB up
+// to ~71: 24 cells ran 3 % faster on kB 33..55 where the model favours 32), so
+// the wider slice is taken only from 7 lanes up.  `forced` (24 or 32)
+// overrides the choice.
+inline int slice_width_for(int kb, int forced) {
+  if (forced == 24 || forced == 32) return forced;
+  const int g24 = slice_lanes(kb, 24), p24 = 64 / g24, p32 = 64 / slice_lanes(kb, 32);
+  const double c24 = (13.5 * 24 + 45) / p24, c32 = (13.5 * 32 + 45) / p32;
+  return g24 >= 7 && c32 <= c24 ? 32 : 24;
+}
 
 struct NwFusedArgs {
   const uint32_t *pack;
@@ -40,6 +60,7 @@ void launch_nw_reg_family0(hipStream_t s, unsigned blocks, const NwFusedArgs &a)
 void launch_nw_reg_family1(hipStream_t s, unsigned blocks, const NwFusedArgs &a);
 void launch_nw_reg_family2(hipStream_t s, unsigned blocks, const NwFusedArgs &a);
 void launch_nw_reg_family3(hipStream_t s, unsigned blocks, const NwFusedArgs &a);
-void launch_nw_sliced_family(hipStream_t s, unsigned blocks, const NwFusedArgs &a);
+void launch_nw_sliced_family24(hipStream_t s, unsigned blocks, const NwFusedArgs &a);
+void launch_nw_sliced_family32(hipStream_t s, unsigned blocks, const NwFusedArgs &a);
 
 }  // namespace crispy

@@ -138,6 +138,11 @@ int crispy_nw_fetch(crispy_ctx *ctx, int64_t first, int64_t count, int32_t *pair
  * cols / vals (count entries, either may be NULL) the column ids and
  * distances. */
 int crispy_nw_csr(crispy_ctx *ctx, int32_t n_rows, int64_t *row_ptr, int32_t *cols, float *vals);
+/* Band cells per lane of the lane-sliced NW kernel (kB > 32): 0 = chosen per
+ * kB bucket by the library's cost model (the default), 24 or 32 = forced.
+ * The distances do not depend on it.  No counterpart in the reference: a
+ * tuning and test switch. */
+int crispy_nw_set_slice_width(crispy_ctx *ctx, int cells);
 /* Unfiltered distances for host pairs (every pair gets a distance). */
 int crispy_nw_dists(crispy_ctx *ctx, const int32_t *pairs, int64_t npairs, int band,
                     float *dists);

@@ -254,10 +254,12 @@ def test_nw_register_kernel_every_kb(ctx):
     assert np.array_equal(bits(d), bits(od))
 
 
-def test_nw_sliced_kernel_shapes(ctx):
+@pytest.mark.parametrize("width", [24, 32, 0])
+def test_nw_sliced_kernel_shapes(ctx, width):
     """Band 2 and 3 on reads of 60..2048 bp: kB from 33 to beyond 767 covers
-    every lane-sliced instance (kB mod 12), group sizes 3..64 lanes, and the
-    generic kernel above; near copies and unrelated reads, both orders."""
+    every lane-sliced instance of both slice widths (kB mod 12 and kB mod 16),
+    group sizes 3..64 lanes, and the generic kernel above; near copies and
+    unrelated reads, both orders; each width forced, then the cost model."""
     say("nw sliced shapes")
     rng = np.random.default_rng(31)
     alph = np.frombuffer(b"ACGT", np.uint8)
@@ -277,13 +279,22 @@ def test_nw_sliced_kernel_shapes(ctx):
     pairs = all_pairs(r.n)
     sel = rng.choice(len(pairs), size=min(700, len(pairs)), replace=False)
     pairs = np.concatenate([pairs[sel], pairs[sel[:200], ::-1]])
+    lens = r.lengths()
+    for band in (2, 3):  # the selection reaches every slice end of both widths
+        kb = (lens[pairs[:, 0]] + lens[pairs[:, 1]]) // (2 * band)
+        sl = kb[(kb > 32) & (kb <= 767)]
+        assert set(sl % 12) == set(range(12)) and set(sl % 16) == set(range(16))
     ctx.upload_reads(ca.NW, r)
-    for band in (2, 3):
-        d = ctx.nw_dists(pairs, band)
-        od, _, _ = O.nw_pairs(ro, pairs, band)
-        bad = np.nonzero(bits(d) != bits(od))[0]
-        assert len(bad) == 0, (band, [(int(pairs[b, 0]), int(pairs[b, 1]), float(d[b]), float(od[b]))
-                                      for b in bad[:5]])
+    ctx.nw_slice_width(width)
+    try:
+        for band in (2, 3):
+            d = ctx.nw_dists(pairs, band)
+            od, _, _ = O.nw_pairs(ro, pairs, band)
+            bad = np.nonzero(bits(d) != bits(od))[0]
+            assert len(bad) == 0, (band, [(int(pairs[b, 0]), int(pairs[b, 1]), float(d[b]), float(od[b]))
+                                          for b in bad[:5]])
+    finally:
+        ctx.nw_slice_width(0)
 
 
 def test_sort_by_dist(ctx):
