@@ -287,20 +287,29 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
     sF[q] = ok && F > 0.0f ? F : 0.0f;
   }
 
-  // loader: thread t stages row t (read row0 + t) and row 128 + t (read col0 + t);
-  // rows past n load read 0 (always valid): their pairs are masked in the epilogue
-  const uint4 *ga = reinterpret_cast<const uint4 *>(bitmap + (int64_t)(row0 + tid < n ? row0 + tid : 0) * WP);
-  const uint4 *gb = reinterpret_cast<const uint4 *>(bitmap + (int64_t)(col0 + tid < n ? col0 + tid : 0) * WP);
+  // loader: a chunk is 16 words (64 bytes) of each of the 256 reads; lane quad
+  // (t >> 2) of the block loads reads (t >> 2) + 32k, k = 0..3, of the rows and
+  // of the columns, lane t & 3 one 16-byte quarter, so a wave instruction reads
+  // 16 whole 64-byte runs.  Rows past n load read 0 (always valid): their
+  // pairs are masked in the epilogue.
+  const int lq = tid & 3, lr = tid >> 2;
+  uint32_t oa[4], ob[4];  // word offsets of this lane's quarter, chunk 0
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int ra = row0 + lr + 32 * k, rb = col0 + lr + 32 * k;
+    oa[k] = (uint32_t)(ra < n ? ra : 0) * WP + 4 * lq;
+    ob[k] = (uint32_t)(rb < n ? rb : 0) * WP + 4 * lq;
+  }
   // (built field by field: a whole-struct copy into the arrays keeps them in scratch)
-  auto ld = [](const uint4 *p) {
-    const uint4 v = *p;
+  auto ld = [bitmap](uint32_t o) {
+    const uint4 v = *reinterpret_cast<const uint4 *>(bitmap + o);
     return make_uint4(v.x, v.y, v.z, v.w);
   };
   uint4 pa[4], pb[4];
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    pa[q] = ld(ga + q);
-    pb[q] = ld(gb + q);
+  for (int k = 0; k < 4; ++k) {
+    pa[k] = ld(oa[k]);
+    pb[k] = ld(ob[k]);
   }
 
   mfma_v16f acc[2][4];
@@ -339,21 +348,17 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
 
   for (int ch = 0; ch < NCH; ++ch) {
     __syncthreads();
-    {
-      uint4 *da = reinterpret_cast<uint4 *>(&Ps[tid * kRow]);
-      uint4 *db = reinterpret_cast<uint4 *>(&Ps[(kTile + tid) * kRow]);
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        da[q] = pa[q];
-        db[q] = pb[q];
-      }
+    for (int k = 0; k < 4; ++k) {
+      *reinterpret_cast<uint4 *>(&Ps[(lr + 32 * k) * kRow + 4 * lq]) = pa[k];
+      *reinterpret_cast<uint4 *>(&Ps[(kTile + lr + 32 * k) * kRow + 4 * lq]) = pb[k];
     }
     __syncthreads();
     if (ch + 1 < NCH) {  // prefetch the next chunk while this one is consumed
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        pa[q] = ld(ga + (ch + 1) * 4 + q);
-        pb[q] = ld(gb + (ch + 1) * 4 + q);
+      for (int k = 0; k < 4; ++k) {
+        pa[k] = ld(oa[k] + (ch + 1) * kChunkWords);
+        pb[k] = ld(ob[k] + (ch + 1) * kChunkWords);
       }
     }
 #pragma unroll
@@ -842,6 +847,9 @@ int crispy_kmer_run(crispy_ctx *ctx, int K, float threshold, int64_t tile_begin,
   km.valid = false;
   ctx->nw.valid = false;
   const int32_t n = rs.n;
+  if ((int64_t)n * kmer_words(6) > (int64_t)UINT32_MAX)  // the tiles address bitmaps with 32-bit word offsets
+    return set_err(ctx, CRISPY_ERR_ARG, "%d reads: at most %lld per k-mer run", n,
+                   (long long)(UINT32_MAX / kmer_words(6)));
   const int nb = (n + kTile - 1) / kTile;
   const int64_t total_tiles = num_slots(nb);
   if (tile_end < 0 || tile_end > total_tiles) tile_end = total_tiles;
