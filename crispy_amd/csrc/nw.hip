@@ -20,9 +20,9 @@
 //   nw_band_reg_kernel<KB>  one lane per pair, the whole band row of 2kB+1
 //     cells held in VGPRs in band coordinates d = j - i + kB, updated in
 //     place in ascending d (diag = old[d], left = old[d+1], up = new[d-1]).
-//     Pairs are bucketed by kB (template) and sorted by (R, L1), R =
-//     min(L2, L1 + kB) the pair's last row with an in-band cell, so the 64
-//     lanes of a wave walk rows in lock step and stop together.  Reads are 2-bit packed; a
+//     Pairs are bucketed by kB (template) and sorted by R = min(L2, L1 + kB),
+//     the pair's last row with an in-band cell, so the 64 lanes of a wave
+//     walk rows in lock step and stop together.  Reads are 2-bit packed; a
 //     row's match bits come from XOR of a sliding 2-bit window of read 1
 //     with read 2's base replicated.  Requires ACGT-only reads.
 //   nw_band_sliced: kKbRegMax < kB <= kKbSliceMax, G lanes per pair.
@@ -58,10 +58,10 @@ constexpr uint32_t kGenericKey = (1u << kBucketKeyBits) - 1;
 static_assert(((uint32_t)kKbSliceMax << 11 | 2047u) < kGenericKey, "bucket keys fit below the generic key");
 
 // ---------------------------------------------------------------- bucketing
-// Sort key per pair: (kB << 11 | R), R = min(L2, L1 + kB) the last row with
-// an in-band cell, clamped to 2047,
-// for the register and sliced kernels; all ones for the generic kernel.  Per-bucket pair and cell counts are
-// reduced in LDS first (a handful of buckets would serialise global atomics).
+// Sort key per pair: kB << 11 | R, R = min(L2, L1 + kB) the last row with an
+// in-band cell, clamped to 2047, for the register and sliced kernels; all ones
+// for the generic kernel.  Per-bucket pair and cell counts are reduced in LDS
+// first (a handful of buckets would serialise global atomics).
 __global__ __launch_bounds__(256) void nw_bucket_kernel(
     const int2 *__restrict__ pairs, int64_t np, const int32_t *__restrict__ len,
     const uint8_t *__restrict__ clean, int band, uint32_t *__restrict__ keys,
@@ -326,7 +326,7 @@ static int nw_core(crispy_ctx *ctx, const int2 *d_pairs, int64_t np, int band) {
 
   // Task lists of the fused launches: per family, buckets by descending mean
   // cells per pair, the waves of a bucket from its longest pairs (the end of
-  // the (L2, L1) order) down.
+  // the R order) down.
   std::vector<uint2> btab(kNumBuckets, make_uint2(0, 0));
   std::vector<uint2> tasks;
   int64_t fam_first[kNumFamilies + 1] = {};
