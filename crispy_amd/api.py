@@ -76,6 +76,8 @@ def load_library():
         "crispy_preprocess_read": ([ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(_vp),
                                     ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                     ctypes.POINTER(_vp)], ctypes.c_int32),
+        "crispy_preprocess_primer_filter": ([_vp, ctypes.c_char_p, _i64p, ctypes.c_int32, ctypes.c_char_p,
+                                             ctypes.c_int32, ctypes.c_int32, _i32p], ctypes.c_int32),
         "crispy_preprocess_length_filter": ([_i64p, ctypes.c_char_p, ctypes.c_int32, ctypes.c_double,
                                              _i32p], ctypes.c_int32),
         "crispy_preprocess_order": ([_vp, ctypes.c_char_p, _i64p, _i32p, ctypes.c_int32, _i32p, _i32p,
@@ -199,6 +201,16 @@ def preprocess_length_filter(rec: Records, sd_allowed):
     reads with N in place; returns the number discarded."""
     return int(load_library().crispy_preprocess_length_filter(
         _p(rec.reads.offsets, _i64p), rec.reads.bases, rec.n, float(sd_allowed), _p(rec.freq, _i32p)))
+
+
+def read_primer(path):
+    """The primer as the reference reads it (preprocess.cpp:672-683): the
+    file's first line with trailing whitespace removed (a line of only
+    whitespace stays as it is)."""
+    with open(path, "rb") as f:
+        line = f.readline().split(b"\n")[0]
+    t = line.rstrip(b"\n\r \t\v\f\b")
+    return t if t else line
 
 
 def preprocess_write(path, rec: Records, order, freq, fastq=False):
@@ -329,6 +341,16 @@ class Context:
         return pairs, dists
 
     # -- input side
+    def preprocess_primer_filter(self, rec, primer: bytes, max_mismatches=2):
+        """-p filter (preprocess.cpp:394-560) on the device: zeroes the
+        frequency of reads without the primer at their start, in place in
+        rec.freq; returns the number discarded."""
+        r = self._L.crispy_preprocess_primer_filter(self._h, rec.reads.bases, _p(rec.reads.offsets, _i64p), rec.n,
+                                                    primer, len(primer), int(max_mismatches), _p(rec.freq, _i32p))
+        if r < 0:
+            self._check(r)
+        return int(r)
+
     def preprocess_order(self, reads: Reads, freq):
         """mergeReads on the device: (order, frequencies) of the output reads."""
         freq = np.ascontiguousarray(freq, dtype=np.int32)

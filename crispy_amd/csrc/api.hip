@@ -220,7 +220,7 @@ static void release_all(crispy_ctx *ctx) {
                     &ctx->nw.out_dists, &ctx->nw.scratch,  &ctx->nw.tasks,     &ctx->temp,
                     &ctx->pre.bases,   &ctx->pre.off,      &ctx->pre.freq,     &ctx->pre.keys,
                     &ctx->pre.keys_alt, &ctx->pre.flags,   &ctx->pre.sel,      &ctx->pre.nsel,
-                    &ctx->pre.csum,    &ctx->pre.out_order, &ctx->pre.out_freq,
+                    &ctx->pre.csum,    &ctx->pre.out_order, &ctx->pre.out_freq, &ctx->pre.primer, &ctx->pre.scratch,
                     &ctx->km.cand_alt,  &ctx->nw.csr_rows,
                     &ctx->nw.csr_cols};
   for (DevBuf *b : bufs) release(ctx, *b);

@@ -3,13 +3,14 @@
 // (<out> = <in>_Clean by default: "> label" + sequence per unique read, in
 // (length, strcmp) order; <out>.frq: "label frequency" per line).  The
 // ordering and the exact-duplicate merge run on the GPU through the C ABI.
-// The primer trimming (-p, a Smith-Waterman search against a primer) is not
-// part of this build and is rejected.
+// The primer filter (-p, a Smith-Waterman search for the primer at the start
+// of each read, -m mismatches allowed) runs on the GPU too.
 #include <sys/time.h>
 
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <string>
 #include <vector>
 
@@ -19,7 +20,7 @@ static void help() {
   std::printf("<-i inFileName>\n");
   std::printf("[-o outFileName, default: <inFileName>_Clean]\n");
   std::printf("[-f input format a(FASTA) q(FASTQ), default: a\n");
-  std::printf("[-p primerFileName] (not supported by this build)\n");
+  std::printf("[-p primerFileName]\n");
   std::printf("[-m number of mismatches to primers allowed, default 2]\n");
   std::printf("[-s standard deviation from average length allowed, default 1.0]\n");
   std::printf("[-h  help]\n");
@@ -33,8 +34,8 @@ static double now_s() {
 
 int main(int argc, char *argv[]) {
   const double t0 = now_s();
-  std::string in, out;
-  int fastq = 0, length_check = 0;
+  std::string in, out, primer_file;
+  int fastq = 0, length_check = 0, primer_check = 0, max_mismatches = 2;
   double sd_allowed = 1.0;
   // preprocess.cpp:613-684: every flag is matched anywhere in argv
   for (int i = 0; i < argc; ++i) {
@@ -48,10 +49,11 @@ int main(int argc, char *argv[]) {
     }
     if (!std::strcmp(argv[i], "-o") && has_arg) out = argv[i + 1];
     if (!std::strcmp(argv[i], "-f") && has_arg && !std::strcmp(argv[i + 1], "q")) fastq = 1;
-    if (!std::strcmp(argv[i], "-p")) {
-      std::fprintf(stderr, "Error: primer trimming (-p) is not supported by this build.\n");
-      return 2;
+    if (!std::strcmp(argv[i], "-p") && has_arg) {
+      primer_file = argv[i + 1];
+      primer_check = 1;
     }
+    if (!std::strcmp(argv[i], "-m") && has_arg) max_mismatches = std::atoi(argv[i + 1]);
     if (!std::strcmp(argv[i], "-s") && has_arg) {
       sd_allowed = std::atof(argv[i + 1]);
       length_check = 1;
@@ -66,6 +68,17 @@ int main(int argc, char *argv[]) {
     return 255;
   }
   if (out.length() < 2) out = in + "_Clean";
+  std::string primer;
+  if (primer_check) {  // preprocess.cpp:672-683: the first line, trailing whitespace removed
+    std::ifstream pf(primer_file);
+    if (!pf) {
+      std::printf("Error: cannot open primer file. EXIT...");
+      return 255;
+    }
+    std::getline(pf, primer);
+    const size_t last = primer.find_last_not_of("\n\r \t\v\f\b");
+    if (last != std::string::npos) primer.resize(last + 1);
+  }
 
   std::printf("\n--------------------------------------------------------------------\n");
   std::printf("\n                            PREPROCESS\n");
@@ -90,6 +103,16 @@ int main(int argc, char *argv[]) {
   if (rc != CRISPY_OK) {
     std::fprintf(stderr, "Error: no usable HIP device (code %d)\n", rc);
     return 255;
+  }
+  if (primer_check) {
+    const int32_t d = crispy_preprocess_primer_filter(ctx, bases, offs, n, primer.data(), (int32_t)primer.size(),
+                                                      max_mismatches, freq);
+    if (d < 0) {
+      std::fprintf(stderr, "Error: %s\n", crispy_ctx_error(ctx));
+      crispy_ctx_destroy(ctx);
+      return 255;
+    }
+    deleted += d;
   }
   std::vector<int32_t> order(n > 0 ? n : 1), ofreq(n > 0 ? n : 1);
   int32_t count = 0;

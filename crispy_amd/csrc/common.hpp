@@ -123,6 +123,7 @@ struct NwState {
 // Input side (preprocess-equivalent ordering and merge).
 struct PreState {
   DevBuf bases, off, freq, keys, keys_alt, flags, sel, nsel, csum, out_order, out_freq;
+  DevBuf primer, scratch;  // primer filter: the primer, per-lane DP rows
 };
 
 struct KStat {

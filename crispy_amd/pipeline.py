@@ -80,15 +80,18 @@ def compute_gen_dist(ctx: Context, reads: Reads, in_pair_path: str, pair_prefix:
 
 
 def preprocess(ctx: Context, in_path: str, out_path: str = None, fastq: bool = False,
-               sd_allowed: float = None):
-    """The reference preprocess main (preprocess.cpp:686-726) without primer
-    trimming: read, optional -s length filter, merge exact duplicates on the
-    device, write <out> (default <in>_Clean) and <out>.frq.  Returns the
-    number of reads written."""
+               sd_allowed: float = None, primer: bytes = None, max_mismatches: int = 2):
+    """The reference preprocess main (preprocess.cpp:686-726): read, optional
+    -s length filter, optional -p primer filter (primer as bytes, see
+    api.read_primer), merge exact duplicates on the device, write <out>
+    (default <in>_Clean) and <out>.frq.  Returns the number of reads
+    written."""
     from .api import preprocess_length_filter, preprocess_read, preprocess_write
     rec = preprocess_read(in_path, fastq)
     if sd_allowed is not None:
         preprocess_length_filter(rec, sd_allowed)
+    if primer is not None:
+        ctx.preprocess_primer_filter(rec, primer, max_mismatches)
     order, freq = ctx.preprocess_order(rec.reads, rec.freq)
     preprocess_write(out_path or (in_path + "_Clean"), rec, order, freq, fastq)
     return len(order)

@@ -187,6 +187,14 @@ int32_t crispy_preprocess_read(const char *path, int fastq, char **bases, int64_
  * frequency of outliers and of reads with 'n'/'N' to 0; returns the count. */
 int32_t crispy_preprocess_length_filter(const int64_t *offsets, const char *bases, int32_t n,
                                         double sd_allowed, int32_t *freq);
+/* The -p primer filter (processPrimer + SWLocalAlignment,
+ * preprocess.cpp:394-560) on the device, after the -s filter: every read with
+ * frequency > 0 whose first min(2 primer_len, L) bytes hold no local alignment
+ * of the primer within max_mismatches (mismatches, gaps and unaligned primer
+ * ends) gets frequency 0; returns the count, or a negative error code. */
+int32_t crispy_preprocess_primer_filter(crispy_ctx *ctx, const char *bases, const int64_t *offsets,
+                                        int32_t n, const char *primer, int32_t primer_len,
+                                        int32_t max_mismatches, int32_t *freq);
 /* mergeReads (preprocess.cpp:244-280) on the device: the reads with frequency
  * > 0 in stable (length, strcmp) order (READ::operator< :42-49), each run of
  * identical sequences collapsed onto its last read with the run's summed

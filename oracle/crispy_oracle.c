@@ -618,6 +618,47 @@ done:
     free(label.p);
 }
 
+/* processPrimer + SWLocalAlignment (preprocess.cpp:394-560): local alignment
+ * of the primer (rows) against the read's first min(2 |primer|, L) bytes
+ * (columns), +4 / -1 / gap -2, floored at 0; per cell the primer start and the
+ * mismatch count of the path (ties: 0, diagonal, up, left); the best cell is
+ * the first of the highest score with the fewest mismatches in row-major
+ * order; unaligned primer ends count as mismatches.  Returns 1 when the read
+ * is kept, 0 when processPrimer sets its frequency to 0. */
+int oracle_primer_keep(const char *primer, int lenA, const char *read, int L, int max_mm) {
+    const int lenB = L < 2 * lenA ? L : 2 * lenA;
+    const int W = lenB + 1;
+    int *S = (int *)calloc((size_t)(lenA + 1) * W, sizeof(int));
+    int *ST = (int *)calloc((size_t)(lenA + 1) * W, sizeof(int));
+    int *MM = (int *)calloc((size_t)(lenA + 1) * W, sizeof(int));
+    for (int i = 0; i <= lenA; ++i) ST[i * W] = i;
+    int maxScore = -9999, nmm = 999, startA = 0, endA = 0;
+    for (int i = 1; i <= lenA; ++i)
+        for (int j = 1; j <= lenB; ++j) {
+            const int al = primer[i - 1] == read[j - 1] ? 4 : -1;
+            const int t0 = S[(i - 1) * W + j - 1] + al, t1 = S[(i - 1) * W + j] - 2, t2 = S[i * W + j - 1] - 2;
+            int v = t0 > t1 ? t0 : t1;
+            v = v > t2 ? v : t2;
+            v = v > 0 ? v : 0;
+            int st, m;
+            if (v == 0) { st = i - 1; m = 0; }
+            else if (v == t0) { st = ST[(i - 1) * W + j - 1]; m = MM[(i - 1) * W + j - 1] + (al != 4); }
+            else if (v == t1) { st = ST[(i - 1) * W + j]; m = MM[(i - 1) * W + j] + 1; }
+            else { st = ST[i * W + j - 1]; m = MM[i * W + j - 1] + 1; }
+            S[i * W + j] = v;
+            ST[i * W + j] = st;
+            MM[i * W + j] = m;
+            if (maxScore < v || (maxScore == v && nmm > m)) { maxScore = v; startA = st; endA = i; nmm = m; }
+        }
+    free(S);
+    free(ST);
+    free(MM);
+    nmm += startA;
+    nmm += lenA > endA ? lenA - endA : endA - lenA;
+    if (nmm > max_mm) startA = endA = 0;
+    return startA != endA;
+}
+
 /* READ::operator< (length, then strcmp), ties by file position (list::sort
  * is stable) */
 static int or_rec_cmp(const void *a, const void *b) {
@@ -628,10 +669,13 @@ static int or_rec_cmp(const void *a, const void *b) {
     return (x->idx > y->idx) - (x->idx < y->idx);
 }
 
-/* Runs the preprocess path on `in` and writes `out` and `out`.frq.  sd >= 0
- * applies the -s filter.  Returns the number of reads written, -1 when a file
- * cannot be opened, -2 when a FASTA input does not start with '>'. */
-int oracle_preprocess_file(const char *in, const char *out, int fastq, double sd) {
+/* Runs the preprocess path on `in` and writes `out` and `out`.frq
+ * (preprocess.cpp:686-726).  sd >= 0 applies the -s filter; a non-NULL primer
+ * then the -p filter with at most max_mm mismatches.  Returns the number of
+ * reads written, -1 when a file cannot be opened, -2 when a FASTA input does
+ * not start with '>'. */
+int oracle_preprocess_file(const char *in, const char *out, int fastq, double sd, const char *primer,
+                           int max_mm) {
     FILE *f = fopen(in, "rb");
     if (!f) return -1;
     or_recs R = {0};
@@ -667,6 +711,10 @@ int oracle_preprocess_file(const char *in, const char *out, int fastq, double sd
             }
         }
     }
+    if (primer) /* processPrimer */
+        for (int k = 0; k < R.n; ++k)
+            if (R.r[k].freq && !oracle_primer_keep(primer, (int)strlen(primer), R.r[k].seq, (int)R.r[k].len, max_mm))
+                R.r[k].freq = 0;
     qsort(R.r, R.n, sizeof(or_rec), or_rec_cmp);
     /* mergeReads: a run of equal sequences collapses onto its last member */
     int prev = -1;

@@ -63,8 +63,11 @@ def lib():
                                            _i64p]
         L.oracle_num_threads.restype = ctypes.c_int
         L.oracle_preprocess_file.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
-                                             ctypes.c_double]
+                                             ctypes.c_double, ctypes.c_char_p, ctypes.c_int]
         L.oracle_preprocess_file.restype = ctypes.c_int
+        L.oracle_primer_keep.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
+                                         ctypes.c_int]
+        L.oracle_primer_keep.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -201,14 +204,21 @@ def gendist_files(reads: Reads, pairs: np.ndarray, band: int, thr: float, prefix
     return nf, tot.value
 
 
-def preprocess_file(in_path, out_path, fastq=False, sd=None):
-    """The reference preprocess path (preprocess.cpp:686-726, no primer step):
-    writes out_path and out_path.frq, returns the number of reads written."""
+def preprocess_file(in_path, out_path, fastq=False, sd=None, primer=None, max_mismatches=2):
+    """The reference preprocess path (preprocess.cpp:686-726): -s filter when
+    sd is given, -p filter when primer (bytes, already read from its file) is
+    given; writes out_path and out_path.frq, returns the number of reads
+    written."""
     n = lib().oracle_preprocess_file(os.fsencode(in_path), os.fsencode(out_path), int(bool(fastq)),
-                                     -1.0 if sd is None else float(sd))
+                                     -1.0 if sd is None else float(sd), primer, int(max_mismatches))
     if n < 0:
         raise IOError(f"oracle_preprocess_file({in_path}) failed: {n}")
     return n
+
+
+def primer_keep(primer, read, max_mismatches=2):
+    """processPrimer's verdict for one read (preprocess.cpp:394-560): True = kept."""
+    return bool(lib().oracle_primer_keep(primer, len(primer), read, len(read), int(max_mismatches)))
 
 
 def num_threads():

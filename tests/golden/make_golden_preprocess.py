@@ -5,7 +5,8 @@ oracle/_ref by `make -C oracle ref`.
 Run from the repository root in a container that has the reference sources:
     python tests/golden/make_golden_preprocess.py
 The fixtures are data only: the input files and the reference's outputs
-(<input>_Clean / .frq, and <input>_s1 / .frq for -s 1.0).
+(<input>_Clean / .frq, <input>_s1 / .frq for -s 1.0, and the primer filter
+runs: -p with the default -m 2, -m 0, and -m 4 with -s 1.0).
 """
 import os
 import shutil
@@ -79,6 +80,43 @@ def fastq():
     return b"".join(out)
 
 
+PRIMER = b"GTGCCAGCMGCCGCGGTAA"
+
+
+def primer_reads(n=240, seed=33):
+    """Reads that start with the primer (exact, 1-4 substitutions, an
+    insertion or deletion, shifted by a few bases, lowercase, absent), and a
+    primer file with trailing whitespace."""
+    rng = np.random.default_rng(seed)
+    alph = np.frombuffer(b"ACGT", np.uint8)
+    out = []
+    for k in range(n):
+        p = bytearray(PRIMER)
+        kind = k % 12
+        if 1 <= kind <= 4:  # kind substitutions
+            for _ in range(kind):
+                q = int(rng.integers(0, len(p)))
+                p[q] = int(alph[(list(alph).index(p[q]) + 1) % 4]) if p[q] in alph else ord("A")
+        elif kind == 5:
+            p.insert(int(rng.integers(1, len(p))), int(rng.choice(alph)))
+        elif kind == 6:
+            del p[int(rng.integers(1, len(p) - 1))]
+        elif kind == 7:
+            p = bytearray(rng.choice(alph, int(rng.integers(1, 5))).tobytes()) + p
+        elif kind == 8:
+            p = bytearray(bytes(p).lower())
+        elif kind == 9:
+            p = bytearray(rng.choice(alph, len(p)).tobytes())
+        elif kind == 10:
+            p = p[int(rng.integers(2, 8)):]
+        body = bytes(p) + rng.choice(alph, int(rng.integers(0, 60)) if k % 37 == 0 else int(rng.integers(80, 200))).tobytes()
+        if k % 5 == 0 and out:  # exact duplicates for the merge
+            out.append(out[-1].replace(b">p", b">dup", 1))
+            continue
+        out.append(b">p%d primer=%d\n%s\n" % (k, kind, body))
+    return b"".join(out)
+
+
 def run(args, cwd):
     r = subprocess.run(args, cwd=cwd, capture_output=True)
     if r.returncode not in (0, 1):  # preprocess's main returns 1 (preprocess.cpp:725)
@@ -91,7 +129,8 @@ def main():
     tmp = tempfile.mkdtemp(prefix="golden_pre_")
     try:
         inputs = {"pre_adv.fa": adversarial(), "pre_dups.fa": raw_with_duplicates(),
-                  "pre_reads.fq": fastq()}
+                  "pre_reads.fq": fastq(), "pre_primer.fa": primer_reads(),
+                  "pre_primer.txt": PRIMER + b" \t\r\nsecond line is ignored\n"}
         for name, data in inputs.items():
             with open(os.path.join(OUT, name), "wb") as f:
                 f.write(data)
@@ -99,7 +138,12 @@ def main():
         jobs = [(["-i", "pre_adv.fa"], "pre_adv.fa_Clean"),
                 (["-i", "pre_dups.fa"], "pre_dups.fa_Clean"),
                 (["-i", "pre_dups.fa", "-s", "1.0", "-o", "pre_dups.fa_s1"], "pre_dups.fa_s1"),
-                (["-i", "pre_reads.fq", "-f", "q"], "pre_reads.fq_Clean")]
+                (["-i", "pre_reads.fq", "-f", "q"], "pre_reads.fq_Clean"),
+                (["-i", "pre_primer.fa", "-p", "pre_primer.txt"], "pre_primer.fa_Clean"),
+                (["-i", "pre_primer.fa", "-p", "pre_primer.txt", "-m", "0", "-o", "pre_primer.fa_m0"],
+                 "pre_primer.fa_m0"),
+                (["-i", "pre_primer.fa", "-p", "pre_primer.txt", "-m", "4", "-s", "1.0", "-o",
+                  "pre_primer.fa_m4s1"], "pre_primer.fa_m4s1")]
         for args, out in jobs:
             run([REF] + args, tmp)
             for suffix in ("", ".frq"):

@@ -1,6 +1,7 @@
 """GPU tests of the input side: the device merge (crispy_preprocess_order)
-and bin/preprocess against the reference preprocess's outputs and a Python
-stable (length, bytes) merge."""
+and primer filter (crispy_preprocess_primer_filter), and bin/preprocess,
+against the reference preprocess's outputs, the oracle and a Python stable
+(length, bytes) merge."""
 import os
 import shutil
 import subprocess
@@ -10,6 +11,7 @@ import numpy as np
 import pytest
 
 import crispy_amd as ca
+from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
@@ -20,7 +22,10 @@ BIN = os.path.join(ROOT, "bin")
 CASES = [("pre_adv.fa", "pre_adv.fa_Clean", []),
          ("pre_dups.fa", "pre_dups.fa_Clean", []),
          ("pre_dups.fa", "pre_dups.fa_s1", ["-s", "1.0"]),
-         ("pre_reads.fq", "pre_reads.fq_Clean", ["-f", "q"])]
+         ("pre_reads.fq", "pre_reads.fq_Clean", ["-f", "q"]),
+         ("pre_primer.fa", "pre_primer.fa_Clean", ["-p", "pre_primer.txt"]),
+         ("pre_primer.fa", "pre_primer.fa_m0", ["-p", "pre_primer.txt", "-m", "0"]),
+         ("pre_primer.fa", "pre_primer.fa_m4s1", ["-p", "pre_primer.txt", "-m", "4", "-s", "1.0"])]
 
 
 def say(*a):
@@ -51,8 +56,10 @@ def py_merge(seqs, freq):
 @pytest.mark.parametrize("inp,out,flags", CASES)
 def test_cli_matches_reference(tmp_path, inp, out, flags):
     shutil.copyfile(os.path.join(PRE, inp), tmp_path / inp)
+    if "-p" in flags:
+        shutil.copyfile(os.path.join(PRE, "pre_primer.txt"), tmp_path / "pre_primer.txt")
     args = [os.path.join(BIN, "preprocess"), "-i", inp] + flags
-    if out.endswith("_s1"):
+    if not out.endswith("_Clean"):
         args += ["-o", out]
     say("preprocess", args)
     r = subprocess.run(args, cwd=tmp_path, capture_output=True, timeout=120)
@@ -84,6 +91,55 @@ def test_python_mirror(ctx, tmp_path):
     n = ca.preprocess(ctx, os.path.join(PRE, "pre_dups.fa"), out, sd_allowed=1.0)
     assert n == 224
     assert open(out, "rb").read() == open(os.path.join(PRE, "pre_dups.fa_s1"), "rb").read()
+
+
+def test_python_mirror_primer(ctx, tmp_path):
+    out = str(tmp_path / "p_Clean")
+    primer = ca.read_primer(os.path.join(PRE, "pre_primer.txt"))
+    n = ca.preprocess(ctx, os.path.join(PRE, "pre_primer.fa"), out, sd_allowed=1.0, primer=primer,
+                      max_mismatches=4)
+    assert n == 93
+    assert open(out, "rb").read() == open(os.path.join(PRE, "pre_primer.fa_m4s1"), "rb").read()
+
+
+@pytest.mark.parametrize("seed,plen,mm", [(11, 20, 2), (12, 8, 0), (13, 35, 5), (14, 1, 0), (15, 120, 9),
+                                          (16, 0, 3)])
+def test_primer_filter_random_vs_oracle(ctx, seed, plen, mm):
+    """Device -p filter against the oracle read by read: reads built from the
+    primer with substitutions, indels, shifts and case changes, unrelated and
+    empty reads, bytes outside ACGT, zero frequencies."""
+    rng = np.random.default_rng(seed)
+    alph = np.frombuffer(b"ACGTNacgtM", np.uint8)
+    primer = bytes(rng.choice(alph[:4], plen).tobytes())
+    seqs = []
+    for k in range(3000):
+        p = bytearray(primer)
+        for _ in range(int(rng.integers(0, 4))):
+            if p:
+                op = int(rng.integers(0, 3))
+                q = int(rng.integers(0, len(p)))
+                if op == 0:
+                    p[q] = int(rng.choice(alph))
+                elif op == 1:
+                    p.insert(q, int(rng.choice(alph)))
+                else:
+                    del p[q]
+        u = rng.random()
+        if u < 0.1:
+            p = bytearray(rng.choice(alph, len(p)).tobytes())
+        elif u < 0.2:
+            p = bytearray(rng.choice(alph, int(rng.integers(1, 6))).tobytes()) + p
+        elif u < 0.25:
+            p = bytearray(bytes(p).lower())
+        seq = bytes(p) + rng.choice(alph, int(rng.integers(0, 80))).tobytes()
+        seqs.append(seq[:int(rng.integers(0, len(seq) + 1))] if k % 50 == 0 else seq)
+    freq = (rng.random(len(seqs)) < 0.95).astype(np.int32) * rng.integers(1, 3, len(seqs)).astype(np.int32)
+    rec = ca.Records(ca.Reads.from_list(seqs), b"", np.zeros(len(seqs) + 1, np.int64), freq.copy())
+    d = ctx.preprocess_primer_filter(rec, primer, mm)
+    want = np.array([f if f == 0 or O.primer_keep(primer, s, mm) else 0 for s, f in zip(seqs, freq)], np.int32)
+    say(f"primer {plen} bases, -m {mm}: {d} of {int((freq != 0).sum())} discarded")
+    assert np.array_equal(rec.freq, want)
+    assert d == int(((freq != 0) & (want == 0)).sum())
 
 
 def rand_records(rng, n, maxlen, alphabet, dup_frac):

@@ -179,10 +179,21 @@ def test_preprocess_host_parts(tmp_path, inp, out, fastq, sd):
     assert open(o + ".frq", "rb").read() == open(os.path.join(PRE, out) + ".frq", "rb").read()
 
 
+def test_read_primer(tmp_path):
+    """The primer line as preprocess.cpp:672-683 reads it."""
+    f = tmp_path / "p.txt"
+    for data, want in [(b"ACGT \t\r\nGG\n", b"ACGT"), (b"ACGT", b"ACGT"), (b"", b""),
+                       (b"   \nACGT\n", b"   "), (b"AC GT\n", b"AC GT")]:
+        f.write_bytes(data)
+        assert ca.read_primer(str(f)) == want
+
+
 def test_preprocess_cli_error_paths(tmp_path):
     exe = os.path.join(BIN, "preprocess")
     assert subprocess.run([exe, "-h"], capture_output=True).returncode == 255
-    assert subprocess.run([exe, "-i", "x.fa", "-p", "primer.txt"], capture_output=True).returncode == 2
+    # a primer file that cannot be opened ends the run before the reads are read
+    assert subprocess.run([exe, "-i", "x.fa", "-p", str(tmp_path / "no_primer.txt")],
+                          capture_output=True).returncode == 255
     assert subprocess.run([exe, "-i", str(tmp_path / "missing.fa")], capture_output=True).returncode == 255
     bad = tmp_path / "bad.fa"
     bad.write_bytes(b"ACGT\n")

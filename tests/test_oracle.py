@@ -360,6 +360,40 @@ def test_oracle_preprocess_matches_reference(tmp_path, inp, out, fastq, sd):
     assert open(o + ".frq", "rb").read() == open(os.path.join(PRE, out) + ".frq", "rb").read()
 
 
+PRIMER_CASES = [("pre_primer.fa_Clean", 2, None), ("pre_primer.fa_m0", 0, None), ("pre_primer.fa_m4s1", 4, 1.0)]
+
+
+@pytest.mark.parametrize("out,max_mm,sd", PRIMER_CASES)
+def test_oracle_primer_filter_matches_reference(tmp_path, out, max_mm, sd):
+    """-p (with -m and -s) against the reference binary's outputs."""
+    primer = open(os.path.join(PRE, "pre_primer.txt"), "rb").readline().rstrip(b"\n\r \t\v\f\b")
+    o = str(tmp_path / "p_Clean")
+    O.preprocess_file(os.path.join(PRE, "pre_primer.fa"), o, False, sd, primer, max_mm)
+    assert open(o, "rb").read() == open(os.path.join(PRE, out), "rb").read()
+    assert open(o + ".frq", "rb").read() == open(os.path.join(PRE, out) + ".frq", "rb").read()
+
+
+def test_oracle_primer_closed_forms():
+    """Cases the alignment fixes by hand: an exact primer anywhere in the
+    window aligns with 0 mismatches; k substitutions in the middle cost k; a
+    primer cut short by c bases costs c (unaligned end); the window is
+    2 |primer| bytes, and a window without a single matching byte leaves the
+    all-zero first cell as the best, a one-base alignment whose unaligned
+    primer end costs |primer| - 1; empty primer or read always fails."""
+    P = b"GTGCCAGCAGCCGCGGTAA"
+    tail = b"T" * 30
+    assert O.primer_keep(P, P + tail, 0)
+    assert O.primer_keep(P, b"ACG" + P + tail, 0)
+    sub2 = P[:6] + b"T" + P[7:12] + b"T" + P[13:]
+    assert O.primer_keep(P, sub2 + tail, 2) and not O.primer_keep(P, sub2 + tail, 1)
+    cut3 = P[3:] + tail
+    assert O.primer_keep(P, cut3, 3) and not O.primer_keep(P, cut3, 2)
+    far = b"N" * (2 * len(P)) + P  # the primer lies past the window
+    assert O.primer_keep(P, far, len(P) - 1) and not O.primer_keep(P, far, len(P) - 2)
+    assert not O.primer_keep(b"", P, 999) and not O.primer_keep(P, b"", 999)
+    assert O.primer_keep(b"A", b"A", 0) and O.primer_keep(b"A", b"C", 0)
+
+
 def test_oracle_preprocess_merge_is_stable_last_wins(tmp_path):
     """Closed-form check: duplicates collapse onto their last occurrence with
     the run length as frequency, output in (length, bytes) order."""
