@@ -166,6 +166,64 @@ def input_side(args):
     shutil.rmtree(tmp)
 
 
+def plain_vs_fused(args):
+    """The one-launch-per-stage plain GPU form (the reference's two executables
+    chained through files, kmerDist.cpp -> .kpair -> genDist: the k-mer pairs
+    come back to the host, then NW runs over 589,824-pair chunks of them with
+    a fetch after each, genCUDA.cu:156-226) against the fused form
+    (crispy_nw_run_on_kmer: the k-mer result stays on the device) on the same
+    reads; both results must be identical."""
+    import crispy_amd as ca
+    from crispy_amd import synth
+    seqs, _, cfg = synth.make(args.workload)
+    reads = ca.Reads.from_list(seqs)
+    n = reads.n
+    K, band = (args.k or cfg["K"]), cfg["band"]
+    kthr, nthr = ca.kmer_default_threshold(n), ca.nw_default_threshold(n)
+    ctx = ca.Context(0)
+    ctx.upload_reads(ca.KMER, reads)
+    ctx.upload_reads(ca.NW, reads)
+    chunk = 589824
+
+    def plain():
+        ctx.kmer_run(K, kthr)
+        kp, _ = ctx.kmer_fetch(dists=False)
+        outs = []
+        for c0 in range(0, len(kp), chunk):
+            ctx.nw_run(kp[c0:c0 + chunk], band, nthr)
+            outs.append(ctx.nw_fetch())
+        return (np.concatenate([o[0] for o in outs]), np.concatenate([o[1] for o in outs])) if outs else None
+
+    def fused():
+        ctx.kmer_run(K, kthr)
+        ctx.nw_run_on_kmer(band, nthr)
+        return ctx.nw_fetch()
+
+    res = {}
+    for name, fn in (("plain", plain), ("fused", fused)):
+        for _ in range(args.warmup):
+            out = fn()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            out = fn()
+        ctx.sync()
+        res[name] = ((time.perf_counter() - t0) / args.steps * 1e3, out)
+        log(f"{name}: {res[name][0]:.2f} ms/step (incl. fetching the NW result)")
+    same = (np.array_equal(res["plain"][1][0], res["fused"][1][0]) and
+            np.array_equal(res["plain"][1][1].view(np.uint32), res["fused"][1][1].view(np.uint32)))
+    ctx.close()
+    pairs = n * (n - 1) // 2
+    print(json.dumps({"metric": "k-mer pairs/s, fused vs one-launch-per-stage form",
+                      "unit": "pairs/s", "workload": args.workload, "n_reads": n, "k": K, "band": band,
+                      "plain_ms_per_step": res["plain"][0], "fused_ms_per_step": res["fused"][0],
+                      "plain_pairs_per_s": pairs / (res["plain"][0] * 1e-3),
+                      "fused_pairs_per_s": pairs / (res["fused"][0] * 1e-3),
+                      "speedup": res["plain"][0] / res["fused"][0], "identical_results": same,
+                      "nw_passing_pairs": int(len(res["fused"][1][1])), "steps": args.steps,
+                      "warmup": args.warmup}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,12 +236,17 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 1 GiB read + 1 GiB write elementwise kernel (3x), to scale "
                          "the FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--plain-vs-fused", action="store_true",
+                    help="time the one-launch-per-stage form (k-mer pairs to the host, NW in "
+                         "589,824-pair chunks) against the fused form on the workload")
     ap.add_argument("--input-side", action="store_true",
                     help="time the preprocess-equivalent merge (device) against the oracle's "
                          "preprocess on a raw FASTA with duplicates, instead of the step")
     args = ap.parse_args()
     if args.input_side:
         return input_side(args)
+    if args.plain_vs_fused:
+        return plain_vs_fused(args)
     if args.copy_calibration:
         import torch
         a = torch.ones(1 << 30, dtype=torch.uint8, device="cuda")

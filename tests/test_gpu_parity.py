@@ -389,37 +389,48 @@ def test_python_mirror_matches_cli(ctx, tmp_path):
         assert open(tmp_path / f"py.{ext}", "rb").read() == open(f"{fa}.{ext}", "rb").read()
 
 
-def test_full_size_reads20k(ctx):
-    """The 20,000-read workload at full size: exact against the oracle on
-    a row subset, plus invariants of the whole output."""
-    say("reads20k full size")
-    seqs, _, cfg = synth.make("reads20k")
+@pytest.mark.parametrize("workload,K,rows", [("reads20k", 6, 40), ("reads100k", 6, 24), ("reads100k", 8, 16),
+                                           ("reads250k", 6, 12)])
+def test_full_size(ctx, workload, K, rows):
+    """The benchmark workloads at full size (20k, 100k and 250k reads): exact
+    against the oracle on the first and the last `rows` rows of the k-mer
+    matrix and on a sample of NW pairs, invariants of the whole output (i < j,
+    (i, j) order, thresholds, NW pairs a subset of the k-mer pairs), and the
+    same output from a second run."""
+    say(f"{workload} full size, k={K}")
+    seqs, _, cfg = synth.make(workload)
+    band = cfg["band"]
     r = ca.Reads.from_list(seqs)
     ro = O.Reads.from_list(seqs)
     kthr = ca.kmer_default_threshold(r.n)
     nthr = ca.nw_default_threshold(r.n)
     ctx.upload_reads(ca.KMER, r)
     ctx.upload_reads(ca.NW, r)
-    c = ctx.kmer_run(6, kthr)
+    c = ctx.kmer_run(K, kthr)
     p, d = ctx.kmer_fetch()
     say("kmer pairs", c)
+    assert c == len(d) > 0
     assert np.all(p[:, 0] < p[:, 1]) and p[:, 1].max() < r.n
     key = p[:, 0].astype(np.int64) * r.n + p[:, 1]
     assert np.all(np.diff(key) > 0)
     assert all(O.passes(float(x), kthr) for x in d[:: max(1, len(d) // 5000)])
-    # rows [0, 40) and [19000, 19040) exactly
-    for r0, r1 in ((0, 40), (19000, 19040)):
-        op, od = O.kmer_all(ro, 6, kthr, nthreads=8, rows=(r0, r1))
+    for r0, r1 in ((0, rows), (r.n - rows, r.n)):
+        op, od = O.kmer_all(ro, K, kthr, nthreads=O.num_threads(), rows=(r0, r1))
         sel = (p[:, 0] >= r0) & (p[:, 0] < r1)
+        say(f"rows [{r0}, {r1}): {int(sel.sum())} pairs")
         assert np.array_equal(canon(p[sel], d[sel]), canon(op, od))
-    nc = ctx.nw_run_on_kmer(10, nthr)
+    nc = ctx.nw_run_on_kmer(band, nthr)
     np_, nd = ctx.nw_fetch()
     say("nw pairs", nc)
-    # every kept pair came from the k-mer list and passes; a sample agrees with the oracle
+    knum = p[:, 0].astype(np.int64) * r.n + p[:, 1]
+    nnum = np_[:, 0].astype(np.int64) * r.n + np_[:, 1]
+    assert np.all(np.diff(nnum) > 0) and np.all(np.isin(nnum, knum))
+    assert all(O.passes(float(x), nthr) for x in nd[:: max(1, len(nd) // 5000)])
+    # a sample agrees with the oracle, kept exactly when it passes
     rng = np.random.default_rng(9)
-    samp = rng.choice(len(p), size=min(3000, len(p)), replace=False)
-    od, _, _ = O.nw_pairs(ro, p[samp], 10)
-    full = ctx.nw_dists(p[samp], 10)
+    samp = rng.choice(len(p), size=min(2000, len(p)), replace=False)
+    od, _, _ = O.nw_pairs(ro, p[samp], band)
+    full = ctx.nw_dists(p[samp], band)
     assert np.array_equal(bits(full), bits(od))
     kept = {(int(a), int(b)): float(x) for (a, b), x in zip(np_, nd)}
     for (a, b), x in zip(p[samp], od):
@@ -427,3 +438,10 @@ def test_full_size_reads20k(ctx):
             assert kept[(int(a), int(b))] == float(x)
         else:
             assert (int(a), int(b)) not in kept
+    # a second run gives the same bytes
+    ctx.kmer_run(K, kthr)
+    p2, d2 = ctx.kmer_fetch()
+    ctx.nw_run_on_kmer(band, nthr)
+    np2, nd2 = ctx.nw_fetch()
+    assert np.array_equal(p2, p) and np.array_equal(bits(d2), bits(d))
+    assert np.array_equal(np2, np_) and np.array_equal(bits(nd2), bits(nd))
