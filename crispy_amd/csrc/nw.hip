@@ -33,6 +33,8 @@
 //     reads with other bytes (N, lowercase, ...).
 #include <hip/hip_runtime.h>
 
+#include <chrono>
+
 #include <cstring>
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -504,6 +506,39 @@ int crispy_nw_fetch(crispy_ctx *ctx, int64_t first, int64_t count, int32_t *pair
     CRISPY_HIP(ctx, hipMemcpyAsync(dists, nw.out_dists.as<float>() + first, sizeof(float) * count,
                                    hipMemcpyDeviceToHost, ctx->stream));
   CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return CRISPY_OK;
+}
+
+int crispy_nw_autotune(crispy_ctx *ctx, int band, int repeats, int *chosen) {
+  if (!ctx || repeats < 1) return CRISPY_ERR_ARG;
+  if (!ctx->km.valid) return set_err(ctx, CRISPY_ERR_STATE, "no k-mer result to tune on");
+  if (ctx->reads[CRISPY_READS_NW].n != ctx->reads[CRISPY_READS_KMER].n)
+    return set_err(ctx, CRISPY_ERR_STATE, "NW read slot does not hold the k-mer reads");
+  CRISPY_HIP(ctx, hipSetDevice(ctx->device));
+  NwState &nw = ctx->nw;
+  const int64_t np = ctx->km.count;
+  const int2 *d = kmer_result_pairs(ctx);
+  const int policies[] = {0, kSliceWidths[0], kSliceWidths[1]};
+  double best = 1e300;
+  int pick = nw.slice_width;
+  for (int pol : policies) {
+    nw.slice_width = pol;
+    double fastest = 1e300;
+    for (int r = 0; r <= repeats; ++r) {  // the first run warms up
+      CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      const auto t0 = std::chrono::steady_clock::now();
+      CRISPY_TRY(nw_core(ctx, d, np, band));
+      CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      if (r > 0) fastest = std::min(fastest, s);
+    }
+    if (fastest < best) {
+      best = fastest;
+      pick = pol;
+    }
+  }
+  nw.slice_width = pick;
+  if (chosen) *chosen = pick;
   return CRISPY_OK;
 }
 

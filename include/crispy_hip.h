@@ -143,6 +143,13 @@ int crispy_nw_csr(crispy_ctx *ctx, int32_t n_rows, int64_t *row_ptr, int32_t *co
  * The distances do not depend on it.  No counterpart in the reference: a
  * tuning and test switch. */
 int crispy_nw_set_slice_width(crispy_ctx *ctx, int cells);
+/* Autotuner for that choice: on the context's current k-mer result, times the
+ * NW DP (bucketing, sort and fused launches; fastest of `repeats` runs after a
+ * warm-up) under the cost model and under each forced width, keeps the
+ * fastest setting for the context and returns it in *chosen (0, 24 or 32).
+ * The NW result of the context is left as it was.  No counterpart in the
+ * reference. */
+int crispy_nw_autotune(crispy_ctx *ctx, int band, int repeats, int *chosen);
 /* Unfiltered distances for host pairs (every pair gets a distance). */
 int crispy_nw_dists(crispy_ctx *ctx, const int32_t *pairs, int64_t npairs, int band,
                     float *dists);

@@ -297,6 +297,32 @@ def test_nw_sliced_kernel_shapes(ctx, width):
         ctx.nw_slice_width(0)
 
 
+def test_nw_autotune(ctx):
+    """The autotuner picks one of the settings, the context keeps it, the
+    result is the same under every setting, and an earlier NW result stays
+    fetchable."""
+    seqs, _, _ = synth.make("reads1k")
+    rng = np.random.default_rng(4)
+    seqs = [s + s[: int(rng.integers(100, 400))] for s in seqs[:600]]  # 300-700 bp: kB 33..117 at band 3
+    r = ca.Reads.from_list(seqs)
+    ctx.upload_reads(ca.KMER, r)
+    ctx.upload_reads(ca.NW, r)
+    ctx.kmer_run(6, 0.5)
+    ctx.nw_run_on_kmer(3, 0.4)
+    before = ctx.nw_fetch()
+    try:
+        w = ctx.nw_autotune(3, repeats=2)
+        say("autotune chose", w)
+        assert w in (0, 24, 32)
+        after = ctx.nw_fetch()
+        assert np.array_equal(after[0], before[0]) and np.array_equal(bits(after[1]), bits(before[1]))
+        ctx.nw_run_on_kmer(3, 0.4)
+        tuned = ctx.nw_fetch()
+        assert np.array_equal(tuned[0], before[0]) and np.array_equal(bits(tuned[1]), bits(before[1]))
+    finally:
+        ctx.nw_slice_width(0)
+
+
 def test_sort_by_dist(ctx):
     rng = np.random.default_rng(5)
     n = 100000
