@@ -170,9 +170,10 @@ def plain_vs_fused(args):
     """The one-launch-per-stage plain GPU form (the reference's two executables
     chained through files, kmerDist.cpp -> .kpair -> genDist: the k-mer pairs
     come back to the host, then NW runs over 589,824-pair chunks of them with
-    a fetch after each, genCUDA.cu:156-226) against the fused form
-    (crispy_nw_run_on_kmer: the k-mer result stays on the device) on the same
-    reads; both results must be identical."""
+    a fetch after each, genCUDA.cu:156-226; NW DP kernels launched one by
+    one) against the fused form (crispy_nw_run_on_kmer: the k-mer result stays
+    on the device, the DP phase replays its captured graph) on the same reads;
+    both results must be identical."""
     import crispy_amd as ca
     from crispy_amd import synth
     seqs, _, cfg = synth.make(args.workload)
@@ -186,6 +187,7 @@ def plain_vs_fused(args):
     chunk = 589824
 
     def plain():
+        ctx.nw_graphs(False)
         ctx.kmer_run(K, kthr)
         kp, _ = ctx.kmer_fetch(dists=False)
         outs = []
@@ -195,6 +197,7 @@ def plain_vs_fused(args):
         return (np.concatenate([o[0] for o in outs]), np.concatenate([o[1] for o in outs])) if outs else None
 
     def fused():
+        ctx.nw_graphs(True)
         ctx.kmer_run(K, kthr)
         ctx.nw_run_on_kmer(band, nthr)
         return ctx.nw_fetch()
@@ -232,6 +235,8 @@ def main():
     ap.add_argument("--workload", default="reads20k")
     ap.add_argument("--k", type=int, default=None, help="override the workload's k")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch the NW DP phase kernel by kernel instead of replaying its captured graph")
     ap.add_argument("--cpu-budget", type=float, default=24.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 1 GiB read + 1 GiB write elementwise kernel (3x), to scale "
@@ -279,6 +284,7 @@ def main():
     ctx = ca.Context(0)
     ctx.upload_reads(ca.KMER, reads)
     ctx.upload_reads(ca.NW, reads)
+    ctx.nw_graphs(not args.no_graphs)
 
     def step():
         kc = ctx.kmer_run(K, kthr)
@@ -327,7 +333,9 @@ def main():
                     "frac": ach / VALU_PEAK_TOPS,
                     "work": f"{NW_OPS_PER_CELL} int ops x in-band DP cells ({cells:.4e}/step)",
                     "phase_ms": phase_ms, "avg_launch_ms": phase_ms,
-                    "family_launches_per_step": dp_launches / args.steps,
+                    "dp_form": ("kernel launches" if args.no_graphs else
+                                "captured HIP graph: plan kernel + one launch per kernel family"),
+                    "family_launches_per_step": dp_launches / args.steps if args.no_graphs else None,
                     "traffic": measured_traffic(args.workload),
                     "traffic_source": f"profiles/kmer_nw_step_{args.workload}_traffic.md"}
     else:

@@ -41,6 +41,7 @@ int ensure(crispy_ctx *ctx, DevBuf &b, size_t bytes) {
 }
 
 int ensure_host(crispy_ctx *ctx, size_t bytes) {
+  CRISPY_HIP(ctx, hipEventSynchronize(ctx->stage_ev));
   if (ctx->h_stage_bytes >= bytes) return CRISPY_OK;
   if (ctx->h_stage) {
     CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -194,9 +195,9 @@ int crispy_ctx_create(int device, crispy_ctx **out) {
   ctx->device = device;
   bool ok = hipSetDevice(device) == hipSuccess && hipStreamCreate(&ctx->stream) == hipSuccess &&
             hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming) == hipSuccess &&
-            hipEventCreateWithFlags(&ctx->hist_ev, hipEventDisableTiming) == hipSuccess &&
-            hipHostMalloc(&ctx->h_hist, kNumBuckets * (sizeof(unsigned int) + sizeof(unsigned long long)),
-                          hipHostMallocDefault) == hipSuccess;
+            hipEventCreateWithFlags(&ctx->stage_ev, hipEventDisableTiming) == hipSuccess &&
+            hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+            hipHostMalloc(&ctx->h_hist, 64, hipHostMallocDefault) == hipSuccess;
   for (int w = 0; ok && w < kNumWorkers; ++w)
     ok = hipStreamCreate(&ctx->workers[w]) == hipSuccess &&
          hipEventCreateWithFlags(&ctx->join_ev[w], hipEventDisableTiming) == hipSuccess;
@@ -209,6 +210,7 @@ int crispy_ctx_create(int device, crispy_ctx **out) {
 }
 
 static void release_all(crispy_ctx *ctx) {
+  nw_release_graphs(ctx);
   DevBuf *bufs[] = {&ctx->km.bitmap,   &ctx->km.T,         &ctx->km.rcnt,
                     &ctx->km.roff,     &ctx->km.rlist,     &ctx->km.excess,    &ctx->km.loff,
                     &ctx->km.roffc,    &ctx->km.rlistc,
@@ -217,7 +219,8 @@ static void release_all(crispy_ctx *ctx) {
                     &ctx->nw.in_pairs, &ctx->nw.keys,      &ctx->nw.keys_alt,  &ctx->nw.idx,
                     &ctx->nw.idx_alt,  &ctx->nw.dist,      &ctx->nw.hist,      &ctx->nw.cells,
                     &ctx->nw.flags,    &ctx->nw.sel,       &ctx->nw.nsel,      &ctx->nw.out_pairs,
-                    &ctx->nw.out_dists, &ctx->nw.scratch,  &ctx->nw.tasks,     &ctx->temp,
+                    &ctx->nw.out_dists, &ctx->nw.scratch,  &ctx->nw.plan,      &ctx->nw.stat,
+                    &ctx->temp,
                     &ctx->pre.bases,   &ctx->pre.off,      &ctx->pre.freq,     &ctx->pre.keys,
                     &ctx->pre.keys_alt, &ctx->pre.flags,   &ctx->pre.sel,      &ctx->pre.nsel,
                     &ctx->pre.csum,    &ctx->pre.out_order, &ctx->pre.out_freq, &ctx->pre.primer, &ctx->pre.scratch,
@@ -249,7 +252,7 @@ void crispy_ctx_destroy(crispy_ctx *ctx) {
     if (ctx->join_ev[w]) (void)hipEventDestroy(ctx->join_ev[w]);
   }
   if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
-  if (ctx->hist_ev) (void)hipEventDestroy(ctx->hist_ev);
+  if (ctx->stage_ev) (void)hipEventDestroy(ctx->stage_ev);
   if (ctx->h_hist) (void)hipHostFree(ctx->h_hist);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -299,7 +302,7 @@ int crispy_reads_upload(crispy_ctx *ctx, int slot, const char *bases, const int6
   rs.n = n;
   rs.h_len.assign(n, 0);
   std::vector<int64_t> poff(n + 1, 0);
-  int32_t maxlen = 0;
+  int32_t maxlen = 0, minlen = CRISPY_MAX_READ_LEN;
   if (n > 0 && offsets[0] != 0) return set_err(ctx, CRISPY_ERR_ARG, "offsets[0] must be 0");
   for (int32_t r = 0; r < n; ++r) {
     const int64_t L = offsets[r + 1] - offsets[r];
@@ -307,10 +310,12 @@ int crispy_reads_upload(crispy_ctx *ctx, int slot, const char *bases, const int6
       return set_err(ctx, CRISPY_ERR_ARG, "unsupported read length %lld (read %d)", (long long)L, r);
     rs.h_len[r] = (int32_t)L;
     maxlen = std::max<int32_t>(maxlen, (int32_t)L);
+    minlen = std::min<int32_t>(minlen, (int32_t)L);
     poff[r + 1] = poff[r] + (L + 15) / 16 + 1;  // one zero pad word per read
   }
   rs.nbases = n ? offsets[n] : 0;
   rs.maxlen = maxlen;
+  rs.minlen = n ? minlen : 0;
   const int64_t nwords = poff[n];
   CRISPY_TRY(ensure(ctx, rs.bases, (size_t)rs.nbases + 16));
   CRISPY_TRY(ensure(ctx, rs.off, sizeof(int64_t) * (n + 1)));
@@ -353,7 +358,7 @@ int crispy_timing_reset(crispy_ctx *ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   drain_timers(ctx);
   ctx->stats.clear();
-  return CRISPY_OK;
+  return nw_timer_reset(ctx);
 }
 
 int crispy_timing_query(crispy_ctx *ctx, const char *kernel, double *total_ms, int64_t *launches,
@@ -364,7 +369,9 @@ int crispy_timing_query(crispy_ctx *ctx, const char *kernel, double *total_ms, i
   const KStat s = it == ctx->stats.end() ? KStat{} : it->second;
   if (total_ms) *total_ms = s.ms;
   if (launches) *launches = s.launches;
-  if (work_units) *work_units = s.units;
+  double units = s.units;
+  CRISPY_TRY(nw_timer_units(ctx, kernel, &units));
+  if (work_units) *work_units = units;
   return CRISPY_OK;
 }
 

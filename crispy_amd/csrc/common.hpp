@@ -84,6 +84,7 @@ struct ReadStore {
   int32_t n = 0;
   int64_t nbases = 0;
   int32_t maxlen = 0;
+  int32_t minlen = 0;
   int all_clean = 1;
   DevBuf bases, off, len, pack, poff, clean;
   std::vector<int32_t> h_len;
@@ -113,9 +114,13 @@ struct KmerState {
 struct NwState {
   DevBuf in_pairs, keys, keys_alt, idx, idx_alt, dist, hist, cells, flags, sel, nsel;
   DevBuf out_pairs, out_dists, scratch;
-  DevBuf tasks;  // bucket table + wave tasks of the fused DP launches
+  DevBuf plan;  // NwPlan of the DP phase (bucket table, family task lists, counters)
+  DevBuf stat;  // cells per family (generic last) summed over timed runs
   DevBuf csr_rows, csr_cols;  // CSR view of the (i, j)-ordered result
   int slice_width = 0;  // sliced-kernel cells per lane: 0 = cost model, 24 or 32 = forced
+  bool graphs = true;   // DP phase through a captured graph (false: launched one by one)
+  hipGraphExec_t dp_exec = nullptr;  // the captured DP phase and what it was captured with
+  std::vector<uintptr_t> dp_key;
   int64_t count = 0;
   bool valid = false;
 };
@@ -142,6 +147,7 @@ constexpr int kNumWorkers = 4;
 
 struct crispy_ctx {
   int device = 0;
+  int cus = 0;  // compute units of the device
   hipStream_t stream = nullptr;
   // Side streams (default priority) for independent launches of one phase;
   // forked from and joined back into `stream` with events.
@@ -156,8 +162,8 @@ struct crispy_ctx {
   DevBuf temp;  // rocPRIM temporary storage
   void *h_stage = nullptr;  // pinned host staging for small async uploads
   size_t h_stage_bytes = 0;
-  void *h_hist = nullptr;      // pinned NW bucket histogram (cells, then pairs)
-  hipEvent_t hist_ev = nullptr;  // the histogram has reached h_hist
+  hipEvent_t stage_ev = nullptr;  // the last upload from h_stage has been read
+  void *h_hist = nullptr;  // pinned slot for small device read-backs
   bool timing = false;
   std::vector<PendingEvent> pending;
   std::vector<hipEvent_t> pool;
@@ -170,13 +176,19 @@ namespace crispy {
 int set_err(crispy_ctx *ctx, int code, const char *fmt, ...);
 int ensure(crispy_ctx *ctx, DevBuf &b, size_t bytes);
 void release(crispy_ctx *ctx, DevBuf &b);
-// Grow-only pinned host staging buffer (ctx->h_stage); the main stream is
-// synchronised before an old buffer is freed.
+// Grow-only pinned host staging buffer (ctx->h_stage) that the host may
+// write: waits for the last upload from it (recorded in stage_ev) and
+// synchronises the main stream before an old buffer is freed.
 int ensure_host(crispy_ctx *ctx, size_t bytes);
 void timer_begin(crispy_ctx *ctx, const char *name, hipEvent_t *a, hipStream_t s = nullptr);
 void timer_end(crispy_ctx *ctx, const char *name, hipEvent_t a, double units, hipStream_t s = nullptr);
 int fork_workers(crispy_ctx *ctx);
 int join_workers(crispy_ctx *ctx);
+// NW timers: *units = the cells behind an NW DP timer name (summed on the
+// device while timing is on); other names leave *units alone.
+int nw_timer_units(crispy_ctx *ctx, const char *name, double *units);
+int nw_timer_reset(crispy_ctx *ctx);
+void nw_release_graphs(crispy_ctx *ctx);
 
 }  // namespace crispy
 

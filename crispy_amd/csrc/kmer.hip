@@ -721,12 +721,16 @@ static int build_profiles(crispy_ctx *ctx, int KP, int K) {
   KmerState &km = ctx->km;
   // Rebuilt on every run: the per-read profiles are part of the stage's work.
   const int32_t n = rs.n;
-  std::vector<int64_t> roff(n + 1, 0);
-  for (int32_t r = 0; r < n; ++r) {
+  for (int32_t r = 0; r < n; ++r)
     if (rs.h_len[r] - K + 1 < 1)
       return set_err(ctx, CRISPY_ERR_ARG, "read %d is shorter than K=%d (length %d)", r, K, rs.h_len[r]);
-    roff[r + 1] = roff[r] + (rs.h_len[r] - KP + 1) / 2 + 1;
-  }
+  // list offsets (and, for K > 6, the K-mer list offsets) staged in pinned
+  // memory, so their uploads need no wait
+  const size_t offs = sizeof(int64_t) * (n + 1);
+  CRISPY_TRY(ensure_host(ctx, K > 6 ? 2 * offs : offs));
+  int64_t *roff = static_cast<int64_t *>(ctx->h_stage);
+  roff[0] = 0;
+  for (int32_t r = 0; r < n; ++r) roff[r + 1] = roff[r] + (rs.h_len[r] - KP + 1) / 2 + 1;
   const int WP = kmer_words(KP);
   CRISPY_TRY(ensure(ctx, km.bitmap, sizeof(uint32_t) * (size_t)n * WP));
   CRISPY_TRY(ensure(ctx, km.T, sizeof(int32_t) * (n + 1)));
@@ -734,8 +738,7 @@ static int build_profiles(crispy_ctx *ctx, int KP, int K) {
   CRISPY_TRY(ensure(ctx, km.excess, sizeof(int32_t) * (n + 1)));
   CRISPY_TRY(ensure(ctx, km.roff, sizeof(int64_t) * (n + 1)));
   CRISPY_TRY(ensure(ctx, km.rlist, sizeof(uint32_t) * (roff[n] + 1)));
-  CRISPY_HIP(ctx, hipMemcpyAsync(km.roff.p, roff.data(), sizeof(int64_t) * (n + 1),
-                                 hipMemcpyHostToDevice, ctx->stream));
+  CRISPY_HIP(ctx, hipMemcpyAsync(km.roff.p, roff, offs, hipMemcpyHostToDevice, ctx->stream));
   CRISPY_TRY(kProfile[KP](ctx, rs));
   CRISPY_TRY(ensure(ctx, km.roffc, sizeof(int64_t) * (n + 1)));
   CRISPY_TRY(ensure(ctx, km.rlistc, sizeof(uint32_t) * (roff[n] + 1)));
@@ -755,12 +758,12 @@ static int build_profiles(crispy_ctx *ctx, int KP, int K) {
     timer_end(ctx, "kmer_profile", ev, n);
   }
   if (K > 6) {  // the K-mer lists for the exact merge of the candidates
-    std::vector<int64_t> loff(n + 1, 0);
+    int64_t *loff = roff + (n + 1);
+    loff[0] = 0;
     for (int32_t r = 0; r < n; ++r) loff[r + 1] = loff[r] + ((rs.h_len[r] - K + 1 + 3) & ~3);  // 8-byte aligned
-    CRISPY_TRY(ensure(ctx, km.loff, sizeof(int64_t) * (n + 1)));
+    CRISPY_TRY(ensure(ctx, km.loff, offs));
     CRISPY_TRY(ensure(ctx, km.lists, sizeof(uint16_t) * (loff[n] + 8)));
-    CRISPY_HIP(ctx, hipMemcpyAsync(km.loff.p, loff.data(), sizeof(int64_t) * (n + 1),
-                                   hipMemcpyHostToDevice, ctx->stream));
+    CRISPY_HIP(ctx, hipMemcpyAsync(km.loff.p, loff, offs, hipMemcpyHostToDevice, ctx->stream));
     hipEvent_t ev;
     timer_begin(ctx, "kmer_profile", &ev);
     if (K == 7)
@@ -773,10 +776,8 @@ static int build_profiles(crispy_ctx *ctx, int KP, int K) {
                                                      km.lists.as<uint16_t>());
     CRISPY_LAUNCHED(ctx);
     timer_end(ctx, "kmer_profile", ev, n);
-    CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));  // loff is a local
   }
-  // roff must stay alive until the copy has run
-  CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  CRISPY_HIP(ctx, hipEventRecord(ctx->stage_ev, ctx->stream));
   km.K = K;
   km.profile_gen = rs.generation;
   return CRISPY_OK;
@@ -926,7 +927,7 @@ int crispy_kmer_run(crispy_ctx *ctx, int K, float threshold, int64_t tile_begin,
                                                         km.kshift, km.pairs.as<int2>());
     CRISPY_LAUNCHED(ctx);
   }
-  CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  // no wait here: every consumer of the result is ordered on the same stream
   km.valid = true;
   if (count) *count = km.count;
   return CRISPY_OK;

@@ -268,6 +268,168 @@ static const FamilyLaunch kFamilyLaunch[kNumFamilies] = {
     launch_nw_sliced_family24, launch_nw_sliced_family32};
 static_assert(kNumFamilies == 6 && kSliceWidths[0] == 24 && kSliceWidths[1] == 32, "one launcher per family");
 
+// ------------------------------------------------------------ DP phase plan
+// One block of 1024 threads, thread b for bucket b: the plan of the DP phase
+// from the bucket histogram (btab by an exclusive scan of the pair counts;
+// per family its buckets by descending mean cells per pair, ties by bucket,
+// and their first tasks).  With `stat` non-null the per-family cells (generic
+// last) are added to it for the timers.
+__global__ __launch_bounds__(1024) void nw_plan_kernel(const unsigned int *__restrict__ hist,
+                                                       const unsigned long long *__restrict__ cells,
+                                                       int forced, NwPlan *__restrict__ plan,
+                                                       unsigned long long *__restrict__ stat) {
+  static_assert(kNumBuckets <= 1024, "one thread per bucket");
+  __shared__ uint32_t s_wsum[16];
+  __shared__ int s_live[kNumBuckets];
+  __shared__ int s_nlive;
+  __shared__ double s_mean[kNumBuckets];
+  __shared__ int s_fam[kNumBuckets];
+  __shared__ uint32_t s_waves[kNumBuckets];
+  __shared__ uint32_t s_ntasks[kNumFamilies], s_nbk[kNumFamilies];
+  __shared__ unsigned long long s_cells[kNumFamilies + 1];
+  const int b = threadIdx.x, lane = b & 63, wv = b >> 6;
+  if (b == 0) s_nlive = 0;
+  if (b < kNumFamilies) s_ntasks[b] = s_nbk[b] = 0;
+  if (b <= kNumFamilies) s_cells[b] = 0;
+  const uint32_t h = b < kNumBuckets ? hist[b] : 0u;
+  // exclusive scan of h: within the wave, then over the 16 wave sums
+  uint32_t x = h;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_wsum[wv] = x;
+  __syncthreads();
+  uint32_t before = 0;
+  for (int w = 0; w < wv; ++w) before += s_wsum[w];
+  const uint32_t start = before + x - h;
+  int f = -1;
+  uint32_t waves = 0;
+  double mean = 0;
+  if (b < kNumBuckets) {
+    plan->btab[b] = make_uint2(start, h);
+    if (b < kGenericBucket && h) {
+      f = nw_family_of(b, forced);
+      const uint32_t per = (uint32_t)nw_pairs_per_wave(b, f);
+      waves = (h + per - 1) / per;
+      mean = (double)cells[b] / (double)h;
+      s_live[atomicAdd(&s_nlive, 1)] = b;
+    }
+    s_fam[b] = f;
+    s_waves[b] = waves;
+    s_mean[b] = mean;
+  }
+  __syncthreads();
+  if (f >= 0) {
+    uint32_t rank = 0, first = 0;
+    for (int k = 0; k < s_nlive; ++k) {
+      const int c = s_live[k];
+      if (s_fam[c] == f && (s_mean[c] > mean || (s_mean[c] == mean && c < b))) {
+        ++rank;
+        first += s_waves[c];
+      }
+    }
+    plan->list[f][rank] = make_uint2((uint32_t)b, first);
+    atomicAdd(&s_ntasks[f], waves);
+    atomicAdd(&s_nbk[f], 1u);
+    atomicAdd(&s_cells[f], cells[b]);
+  }
+  if (b == kGenericBucket) s_cells[kNumFamilies] = cells[b];
+  __syncthreads();
+  if (b < kNumFamilies) {
+    plan->ntasks[b] = s_ntasks[b];
+    plan->nbuckets[b] = s_nbk[b];
+  }
+  if (stat && b <= kNumFamilies) stat[b] += s_cells[b];
+}
+
+// Grid of family f for np pairs whose kB lies in [kb_lo, kb_hi]: an upper
+// bound on the family's tasks, np over the fewest pairs per wave among the
+// family's kB in that range plus one partial wave per bucket, rounded up to
+// m * 2^e with m < 16 (similar pair counts launch alike and share a captured
+// graph; at most 1/8 of the blocks are spare).  0 if no kB of the range
+// belongs to the family.
+static unsigned family_grid(int f, int64_t np, int kb_lo, int kb_hi, int forced) {
+  int64_t per = 0, buckets = 0;
+  for (int kb = std::max(kb_lo, 0); kb <= std::min(kb_hi, kKbSliceMax); ++kb) {
+    if (nw_family_of(kb, forced) != f) continue;
+    const int64_t p = nw_pairs_per_wave(kb, f);
+    per = per ? std::min(per, p) : p;
+    ++buckets;
+  }
+  if (!buckets) return 0;
+  int64_t g = (np + per - 1) / per + buckets;
+  int e = 0;
+  while ((g >> e) >= 16) ++e;
+  g = (((g - 1) >> e) + 1) << e;
+  return (unsigned)std::min<int64_t>(g, 0x7FFFFFFF);
+}
+
+// The DP phase's launches after the bucket sort: the plan, then every family
+// that the reads' kB range allows on the worker streams (family f on worker
+// f mod 4).  No host read-back and nothing sized by the histogram, so the
+// sequence can be captured into a graph.  Per-family timers only when not
+// capturing.
+static int enqueue_dp(crispy_ctx *ctx, const NwFusedArgs &args, const unsigned (&grid)[kNumFamilies],
+                      bool capturing) {
+  NwState &nw = ctx->nw;
+  nw_plan_kernel<<<1, 1024, 0, ctx->stream>>>(nw.hist.as<unsigned int>(), nw.cells.as<unsigned long long>(),
+                                              nw.slice_width, nw.plan.as<NwPlan>(),
+                                              ctx->timing ? nw.stat.as<unsigned long long>() : nullptr);
+  CRISPY_LAUNCHED(ctx);
+  CRISPY_TRY(fork_workers(ctx));
+  for (int f = 0; f < kNumFamilies; ++f) {
+    if (!grid[f]) continue;
+    hipStream_t s = ctx->workers[f % kNumWorkers];
+    const char *name = f >= kNumRegFamilies ? "nw_band_sliced" : "nw_band_reg";
+    hipEvent_t ev = nullptr;
+    if (!capturing) timer_begin(ctx, name, &ev, s);
+    kFamilyLaunch[f](s, grid[f], args);
+    CRISPY_LAUNCHED(ctx);
+    if (!capturing) timer_end(ctx, name, ev, 0.0, s);
+  }
+  return join_workers(ctx);
+}
+
+static void drop_dp_graph(NwState &nw) {
+  if (nw.dp_exec) (void)hipGraphExecDestroy(nw.dp_exec);
+  nw.dp_exec = nullptr;
+  nw.dp_key.clear();
+}
+
+// Launches the DP phase through its captured graph, capturing (and
+// instantiating) it first when anything it was captured with has changed.
+static int launch_dp_graph(crispy_ctx *ctx, const NwFusedArgs &args, const unsigned (&grid)[kNumFamilies]) {
+  NwState &nw = ctx->nw;
+  std::vector<uintptr_t> key = {
+      (uintptr_t)args.pack, (uintptr_t)args.poff, (uintptr_t)args.len, (uintptr_t)args.pairs,
+      (uintptr_t)args.order, (uintptr_t)args.plan, (uintptr_t)args.out, (uintptr_t)nw.hist.p,
+      (uintptr_t)nw.cells.p, (uintptr_t)(ctx->timing ? nw.stat.p : nullptr), (uintptr_t)nw.slice_width};
+  for (unsigned g : grid) key.push_back(g);
+  if (!nw.dp_exec || key != nw.dp_key) {
+    drop_dp_graph(nw);
+    CRISPY_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_dp(ctx, args, grid, true);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc != CRISPY_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return set_err(ctx, CRISPY_ERR_HIP, "capturing the NW DP phase: %s", hipGetErrorString(e));
+    const hipError_t ie = hipGraphInstantiate(&nw.dp_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) {
+      nw.dp_exec = nullptr;
+      return set_err(ctx, CRISPY_ERR_HIP, "instantiating the NW DP graph: %s", hipGetErrorString(ie));
+    }
+    nw.dp_key = key;
+  }
+  CRISPY_HIP(ctx, hipGraphLaunch(nw.dp_exec, ctx->stream));
+  return CRISPY_OK;
+}
+
 // Distances of np device pairs into nw.dist (indexed like the pairs).
 static int nw_core(crispy_ctx *ctx, const int2 *d_pairs, int64_t np, int band) {
   ReadStore &rs = ctx->reads[CRISPY_READS_NW];
@@ -280,6 +442,7 @@ static int nw_core(crispy_ctx *ctx, const int2 *d_pairs, int64_t np, int band) {
   CRISPY_TRY(ensure(ctx, nw.dist, sizeof(float) * (np + 1)));
   CRISPY_TRY(ensure(ctx, nw.hist, sizeof(unsigned int) * kNumBuckets));
   CRISPY_TRY(ensure(ctx, nw.cells, sizeof(unsigned long long) * kNumBuckets));
+  CRISPY_TRY(ensure(ctx, nw.plan, sizeof(NwPlan)));
   if (np == 0) return CRISPY_OK;
   CRISPY_HIP(ctx, hipMemsetAsync(nw.hist.p, 0, sizeof(unsigned int) * kNumBuckets, ctx->stream));
   CRISPY_HIP(ctx, hipMemsetAsync(nw.cells.p, 0, sizeof(unsigned long long) * kNumBuckets, ctx->stream));
@@ -289,15 +452,6 @@ static int nw_core(crispy_ctx *ctx, const int2 *d_pairs, int64_t np, int band) {
       d_pairs, np, rs.len.as<int32_t>(), rs.clean.as<uint8_t>(), band, nw.keys.as<uint32_t>(),
       nw.idx.as<uint32_t>(), nw.hist.as<unsigned int>(), nw.cells.as<unsigned long long>());
   CRISPY_LAUNCHED(ctx);
-  // The histogram is final here: bring it to the host now and build the task
-  // table there while the device sorts the pairs.
-  unsigned long long *cells = reinterpret_cast<unsigned long long *>(ctx->h_hist);
-  unsigned int *hist = reinterpret_cast<unsigned int *>(cells + kNumBuckets);
-  CRISPY_HIP(ctx, hipMemcpyAsync(hist, nw.hist.p, sizeof(unsigned int) * kNumBuckets, hipMemcpyDeviceToHost,
-                                 ctx->stream));
-  CRISPY_HIP(ctx, hipMemcpyAsync(cells, nw.cells.p, sizeof(unsigned long long) * kNumBuckets,
-                                 hipMemcpyDeviceToHost, ctx->stream));
-  CRISPY_HIP(ctx, hipEventRecord(ctx->hist_ev, ctx->stream));
   size_t tbytes = 0;
   CRISPY_HIP(ctx, rocprim::radix_sort_pairs<RadixOnesweep>(nullptr, tbytes, nw.keys.as<uint32_t>(),
                                             nw.keys_alt.as<uint32_t>(), nw.idx.as<uint32_t>(),
@@ -307,101 +461,46 @@ static int nw_core(crispy_ctx *ctx, const int2 *d_pairs, int64_t np, int band) {
                                             nw.keys_alt.as<uint32_t>(), nw.idx.as<uint32_t>(),
                                             nw.idx_alt.as<uint32_t>(), (size_t)np, 0, kBucketKeyBits, ctx->stream));
   timer_end(ctx, "nw_bucket", ev, (double)np);
-  CRISPY_HIP(ctx, hipEventSynchronize(ctx->hist_ev));
   const uint32_t *order = nw.idx_alt.as<uint32_t>();
   float *out = nw.dist.as<float>();
-  // Bucket b's pairs are order[start[b] .. start[b] + hist[b]).
-  int64_t start[kNumBuckets];
-  int64_t pos = 0;
-  std::vector<int> todo;
-  double total_cells = 0;
-  for (int b = 0; b < kNumBuckets; ++b) {
-    start[b] = pos;
-    pos += hist[b];
-    if (hist[b]) todo.push_back(b);
-    total_cells += (double)cells[b];
-  }
-  const int64_t ng = hist[kGenericBucket];
-  const int gblocks = (int)((std::min<int64_t>(ng, 32768) + 255) / 256);
-  if (ng > 0)  // before the fork: ensure() may free and reallocate
-    CRISPY_TRY(ensure(ctx, nw.scratch, sizeof(int2) * (size_t)gblocks * 256 * (rs.maxlen + 1)));
-
-  // Task lists of the fused launches: per family, buckets by descending mean
-  // cells per pair, the waves of a bucket from its longest pairs (the end of
-  // the R order) down.
-  std::vector<uint2> btab(kNumBuckets, make_uint2(0, 0));
-  std::vector<uint2> tasks;
-  int64_t fam_first[kNumFamilies + 1] = {};
-  double fam_cells[kNumFamilies] = {};
-  const int forced = nw.slice_width;
-  auto family_of = [forced](int b) {
-    if (b > kKbRegMax) return kNumRegFamilies + (slice_width_for(b, forced) == kSliceWidths[0] ? 0 : 1);
-    int f = 0;
-    while (b > kRegFamilies[f].hi) ++f;
-    return f;
-  };
-  for (int f = 0; f < kNumFamilies; ++f) {
-    fam_first[f] = (int64_t)tasks.size();
-    std::vector<int> bs;
-    for (int b : todo)
-      if (b != kGenericBucket && family_of(b) == f) bs.push_back(b);
-    std::sort(bs.begin(), bs.end(), [&](int a, int b) {
-      return (double)cells[a] / hist[a] > (double)cells[b] / hist[b];
-    });
-    for (int b : bs) {
-      btab[b] = make_uint2((uint32_t)start[b], hist[b]);
-      const int per_wave = b > kKbRegMax ? 64 / slice_lanes(b, kSliceWidths[f - kNumRegFamilies]) : 64;
-      const int64_t waves = ((int64_t)hist[b] + per_wave - 1) / per_wave;
-      for (int64_t w = waves - 1; w >= 0; --w) tasks.push_back(make_uint2((uint32_t)b, (uint32_t)w));
-      fam_cells[f] += (double)cells[b];
-    }
-  }
-  fam_first[kNumFamilies] = (int64_t)tasks.size();
-  const size_t tab_bytes = sizeof(uint2) * kNumBuckets;
-  const size_t task_bytes = sizeof(uint2) * tasks.size();
-  CRISPY_TRY(ensure(ctx, nw.tasks, tab_bytes + task_bytes));
-  CRISPY_TRY(ensure_host(ctx, tab_bytes + task_bytes));
-  std::memcpy(ctx->h_stage, btab.data(), tab_bytes);
-  if (task_bytes) std::memcpy((char *)ctx->h_stage + tab_bytes, tasks.data(), task_bytes);
-  CRISPY_HIP(ctx, hipMemcpyAsync(nw.tasks.p, ctx->h_stage, tab_bytes + task_bytes,
-                                 hipMemcpyHostToDevice, ctx->stream));
-  const uint2 *d_btab = nw.tasks.as<uint2>();
-  const uint2 *d_tasks = d_btab + kNumBuckets;
+  NwPlan *plan = nw.plan.as<NwPlan>();
+  if (ctx->timing) CRISPY_TRY(ensure(ctx, nw.stat, sizeof(unsigned long long) * (kNumFamilies + 1)));
+  const NwFusedArgs args{rs.pack.as<uint32_t>(), rs.poff.as<int64_t>(), rs.len.as<int32_t>(),
+                         d_pairs, order, plan, out};
+  // kB = (L1 + L2) / (2 band) of any pair lies in [minlen / band, maxlen / band]
+  const int kb_lo = band >= 2 ? rs.minlen / band : 0, kb_hi = band >= 2 ? rs.maxlen / band : -1;
+  unsigned grid[kNumFamilies];
+  for (int f = 0; f < kNumFamilies; ++f) grid[f] = family_grid(f, np, kb_lo, kb_hi, nw.slice_width);
 
   hipEvent_t phase;
   timer_begin(ctx, "nw_dp_phase", &phase);
-  CRISPY_TRY(fork_workers(ctx));
-  // families by descending cells, each to the worker stream with the fewest
-  // cells so far (longest processing time first)
-  int fam_order[kNumFamilies];
-  for (int f = 0; f < kNumFamilies; ++f) fam_order[f] = f;
-  std::sort(fam_order, fam_order + kNumFamilies, [&](int a, int b) { return fam_cells[a] > fam_cells[b]; });
-  double load[kNumWorkers] = {};
-  for (int f : fam_order) {
-    const int64_t nt = fam_first[f + 1] - fam_first[f];
-    if (nt == 0) continue;
-    const int wk = (int)(std::min_element(load, load + kNumWorkers) - load);
-    load[wk] += fam_cells[f];
-    hipStream_t s = ctx->workers[wk];
-    const char *name = f >= kNumRegFamilies ? "nw_band_sliced" : "nw_band_reg";
-    const NwFusedArgs args{rs.pack.as<uint32_t>(), rs.poff.as<int64_t>(), rs.len.as<int32_t>(),
-                           d_pairs, order, d_btab, d_tasks + fam_first[f], out};
-    timer_begin(ctx, name, &ev, s);
-    kFamilyLaunch[f](s, (unsigned)nt, args);
-    CRISPY_LAUNCHED(ctx);
-    timer_end(ctx, name, ev, fam_cells[f], s);
+  if (nw.graphs)
+    CRISPY_TRY(launch_dp_graph(ctx, args, grid));
+  else
+    CRISPY_TRY(enqueue_dp(ctx, args, grid, false));
+  // Pairs the fused kernels cannot take (full NW, non-ACGT bytes, kB past
+  // kKbSliceMax) exist only if the reads allow them; then their count comes
+  // back from the plan and the generic kernel runs after the families.
+  const bool maybe_generic = band < 2 || !rs.all_clean || rs.maxlen / band > kKbSliceMax;
+  if (maybe_generic) {
+    uint2 *h_g = reinterpret_cast<uint2 *>(ctx->h_hist);
+    CRISPY_HIP(ctx, hipMemcpyAsync(h_g, &plan->btab[kGenericBucket], sizeof(uint2), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+    CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const uint2 gb = *h_g;
+    const int64_t ng = gb.y;
+    if (ng > 0) {
+      const int gblocks = (int)((std::min<int64_t>(ng, 32768) + 255) / 256);
+      CRISPY_TRY(ensure(ctx, nw.scratch, sizeof(int2) * (size_t)gblocks * 256 * (rs.maxlen + 1)));
+      timer_begin(ctx, "nw_generic", &ev);
+      nw_generic_kernel<<<gblocks, 256, 0, ctx->stream>>>(rs.bases.as<uint8_t>(), rs.off.as<int64_t>(),
+                                                         rs.len.as<int32_t>(), d_pairs, order + gb.x,
+                                                         (int32_t)ng, band, nw.scratch.as<int2>(), out);
+      CRISPY_LAUNCHED(ctx);
+      timer_end(ctx, "nw_generic", ev, 0.0);
+    }
   }
-  if (ng > 0) {
-    hipStream_t s = ctx->workers[std::min_element(load, load + kNumWorkers) - load];
-    timer_begin(ctx, "nw_generic", &ev, s);
-    nw_generic_kernel<<<gblocks, 256, 0, s>>>(rs.bases.as<uint8_t>(), rs.off.as<int64_t>(),
-                                             rs.len.as<int32_t>(), d_pairs, order + start[kGenericBucket],
-                                             (int32_t)ng, band, nw.scratch.as<int2>(), out);
-    CRISPY_LAUNCHED(ctx);
-    timer_end(ctx, "nw_generic", ev, (double)cells[kGenericBucket], s);
-  }
-  CRISPY_TRY(join_workers(ctx));
-  timer_end(ctx, "nw_dp_phase", phase, total_cells);
+  timer_end(ctx, "nw_dp_phase", phase, 0.0);
   return CRISPY_OK;
 }
 
@@ -451,11 +550,44 @@ static int upload_pairs(crispy_ctx *ctx, const int32_t *pairs, int64_t np) {
   return CRISPY_OK;
 }
 
+int nw_timer_units(crispy_ctx *ctx, const char *name, double *units) {
+  NwState &nw = ctx->nw;
+  int lo, hi;  // families [lo, hi); kNumFamilies is the generic kernel
+  if (!std::strcmp(name, "nw_band_reg")) lo = 0, hi = kNumRegFamilies;
+  else if (!std::strcmp(name, "nw_band_sliced")) lo = kNumRegFamilies, hi = kNumFamilies;
+  else if (!std::strcmp(name, "nw_generic")) lo = kNumFamilies, hi = kNumFamilies + 1;
+  else if (!std::strcmp(name, "nw_dp_phase")) lo = 0, hi = kNumFamilies + 1;
+  else return CRISPY_OK;
+  *units = 0;
+  if (!nw.stat.p) return CRISPY_OK;
+  unsigned long long *h = reinterpret_cast<unsigned long long *>(ctx->h_hist);
+  CRISPY_HIP(ctx, hipMemcpyAsync(h, nw.stat.p, sizeof(unsigned long long) * (kNumFamilies + 1),
+                                 hipMemcpyDeviceToHost, ctx->stream));
+  CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  for (int f = lo; f < hi; ++f) *units += (double)h[f];
+  return CRISPY_OK;
+}
+
+int nw_timer_reset(crispy_ctx *ctx) {
+  if (!ctx->nw.stat.p) return CRISPY_OK;
+  CRISPY_HIP(ctx, hipMemsetAsync(ctx->nw.stat.p, 0, ctx->nw.stat.bytes, ctx->stream));
+  CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return CRISPY_OK;
+}
+
+void nw_release_graphs(crispy_ctx *ctx) { drop_dp_graph(ctx->nw); }
+
 }  // namespace crispy
 
 using namespace crispy;
 
 extern "C" {
+
+int crispy_nw_set_graphs(crispy_ctx *ctx, int enable) {
+  if (!ctx) return CRISPY_ERR_ARG;
+  ctx->nw.graphs = enable != 0;
+  return CRISPY_OK;
+}
 
 int crispy_nw_run(crispy_ctx *ctx, const int32_t *pairs, int64_t npairs, int band, float threshold,
                   int64_t *count) {

@@ -483,11 +483,39 @@ __device__ __forceinline__ void nw_sliced_wave(const uint32_t *__restrict__ pack
 }
 
 // ------------------------------------------------------------ fused launches
-// One launch per kernel family instead of one per bucket: block = one wave,
-// tasks[blockIdx.x] = (bucket kB, wave index in the bucket), btab[kB] = (first
-// position in `order`, pairs in the bucket).  The host lists the tasks longest
+// One launch per kernel family instead of one per bucket: block = one wave =
+// one task of the family's list in the device plan.  The tasks run longest
 // first, so the grid drains like a longest-processing-time schedule and small
 // buckets do not each pay a launch (and a full-wave DP latency) in series.
+// The grid is an upper bound on the tasks, set before the histogram exists;
+// blocks past the plan's task count exit at once, so the phase needs no host
+// read-back.  A block's bucket: the last list entry whose first task is <= its
+// task (the entries ascend), found by one ballot per 64 entries.
+struct NwTask {
+  int kb;     // bucket
+  int wave;   // wave index within the bucket
+  uint2 bk;   // btab entry of the bucket
+};
+
+__device__ __forceinline__ bool nw_task(const NwPlan *__restrict__ plan, const int f, NwTask &task) {
+  const uint32_t t = blockIdx.x;
+  if (t >= plan->ntasks[f]) return false;
+  const int nbk = (int)plan->nbuckets[f];
+  const uint2 *list = plan->list[f];
+  int k = 0;
+  for (int base = 0; base < nbk; base += 64) {
+    const int e = base + (int)(threadIdx.x & 63);
+    k += __popcll(__ballot(e < nbk && list[e].y <= t));
+  }
+  const uint2 e = list[k - 1];
+  task.kb = (int)e.x;
+  task.bk = plan->btab[e.x];
+  const uint32_t per = (uint32_t)nw_pairs_per_wave(task.kb, f);
+  const uint32_t waves = (task.bk.y + per - 1) / per;
+  task.wave = (int)(waves - 1 - (t - e.y));
+  return true;
+}
+
 template <int KB, int HI>
 __device__ __forceinline__ void reg_dispatch(int kb, const uint32_t *pack, const int64_t *poff,
                                              const int32_t *len, const int2 *pairs,
@@ -502,18 +530,17 @@ __device__ __forceinline__ void reg_dispatch(int kb, const uint32_t *pack, const
   }
 }
 
-// Register kernel for buckets LO..HI; WAVES is the smallest occupancy target
-// among them (nw_reg_waves).
-template <int LO, int HI, int WAVES>
+// Register kernel for buckets LO..HI (family F); WAVES is the smallest
+// occupancy target among them (nw_reg_waves).
+template <int LO, int HI, int WAVES, int F>
 __global__ __launch_bounds__(64, WAVES) void nw_reg_fused_kernel(
     const uint32_t *__restrict__ pack, const int64_t *__restrict__ poff,
     const int32_t *__restrict__ len, const int2 *__restrict__ pairs,
-    const uint32_t *__restrict__ order, const uint2 *__restrict__ btab,
-    const uint2 *__restrict__ tasks, float *__restrict__ out) {
-  const uint2 task = tasks[blockIdx.x];
-  const uint2 bk = btab[task.x];
-  reg_dispatch<LO, HI>((int)task.x, pack, poff, len, pairs, order + bk.x, (int32_t)bk.y,
-                       (int)task.y * 64, out);
+    const uint32_t *__restrict__ order, const NwPlan *__restrict__ plan, float *__restrict__ out) {
+  NwTask task;
+  if (!nw_task(plan, F, task)) return;
+  reg_dispatch<LO, HI>(task.kb, pack, poff, len, pairs, order + task.bk.x, (int32_t)task.bk.y,
+                       task.wave * 64, out);
 }
 
 template <int C, int KEND>
@@ -530,20 +557,19 @@ __device__ __forceinline__ void sliced_dispatch(int kend, const uint32_t *pack, 
   }
 }
 
-// Lane-sliced kernel at slice width C for the buckets assigned to it.
-template <int C>
+// Lane-sliced kernel at slice width C (family F) for the buckets assigned to it.
+template <int C, int F>
 __global__ __launch_bounds__(64) void nw_sliced_fused_kernel(
     const uint32_t *__restrict__ pack, const int64_t *__restrict__ poff,
     const int32_t *__restrict__ len, const int2 *__restrict__ pairs,
-    const uint32_t *__restrict__ order, const uint2 *__restrict__ btab,
-    const uint2 *__restrict__ tasks, float *__restrict__ out) {
+    const uint32_t *__restrict__ order, const NwPlan *__restrict__ plan, float *__restrict__ out) {
+  static_assert(kSliceWidths[F - kNumRegFamilies] == C, "family F runs slice width C");
   __shared__ int red[64][6];
-  const uint2 task = tasks[blockIdx.x];
-  const uint2 bk = btab[task.x];
-  const int kb = (int)task.x;
+  NwTask task;
+  if (!nw_task(plan, F, task)) return;
   // KEND = (W - 1) mod C = 2kB mod C, even
-  sliced_dispatch<C, 0>((2 * kb) % C, pack, poff, len, pairs, order + bk.x, (int32_t)bk.y, kb,
-                        slice_lanes(kb, C), (int)task.y, red, out);
+  sliced_dispatch<C, 0>((2 * task.kb) % C, pack, poff, len, pairs, order + task.bk.x, (int32_t)task.bk.y,
+                        task.kb, slice_lanes(task.kb, C), task.wave, red, out);
 }
 
 }  // namespace crispy

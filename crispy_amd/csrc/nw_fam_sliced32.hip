@@ -5,8 +5,8 @@
 namespace crispy {
 
 void launch_nw_sliced_family32(hipStream_t s, unsigned blocks, const NwFusedArgs &a) {
-  nw_sliced_fused_kernel<32><<<blocks, 64, 0, s>>>(a.pack, a.poff, a.len, a.pairs, a.order, a.btab, a.tasks,
-                                                  a.out);
+  nw_sliced_fused_kernel<32, kNumRegFamilies + 1><<<blocks, 64, 0, s>>>(a.pack, a.poff, a.len, a.pairs, a.order,
+      a.plan, a.out);
 }
 
 }  // namespace crispy

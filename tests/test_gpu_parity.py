@@ -323,6 +323,50 @@ def test_nw_autotune(ctx):
         ctx.nw_slice_width(0)
 
 
+def test_nw_graph_replays(ctx):
+    """The DP phase replayed from its captured graph equals the same kernels
+    launched one by one, on every replay: two inputs alternate (each switch
+    captures again), with eager work and synchronisations between replays,
+    timing toggled (captured again), register and sliced families, and reads
+    with non-ACGT bytes (generic kernel after the graph)."""
+    say("nw graph replays")
+    seqs, _, _ = synth.make("reads1k")
+    rng = np.random.default_rng(8)
+    seqs = [s + s[: int(rng.integers(0, 400))] for s in seqs[:500]]  # 200-700 bp
+    for k in range(0, 500, 97):
+        t = bytearray(seqs[k])
+        t[int(rng.integers(0, len(t)))] = ord("N")
+        seqs[k] = bytes(t)
+    r = ca.Reads.from_list(seqs)
+    ro = O.Reads.from_list(seqs)
+    ctx.upload_reads(ca.KMER, r)
+    ctx.upload_reads(ca.NW, r)
+    extra = rng.integers(0, r.n, (3000, 2)).astype(np.int32)
+    try:
+        for band in (10, 3):
+            ctx.nw_graphs(False)
+            ctx.kmer_run(6, 0.5)
+            ctx.nw_run_on_kmer(band, 0.45)
+            ref_a = ctx.nw_fetch()
+            ref_b = ctx.nw_dists(extra, band)
+            od, _, _ = O.nw_pairs(ro, extra, band)
+            assert np.array_equal(bits(ref_b), bits(od))
+            ctx.nw_graphs(True)
+            for rep in range(4):
+                ctx.timing(rep == 2)
+                ctx.nw_run_on_kmer(band, 0.45)
+                got = ctx.nw_fetch()
+                assert np.array_equal(got[0], ref_a[0]) and np.array_equal(bits(got[1]), bits(ref_a[1])), (band, rep)
+                ctx.kmer_fetch(dists=False)  # eager work between replays
+                ctx.sync()
+                d = ctx.nw_dists(extra, band)
+                assert np.array_equal(bits(d), bits(ref_b)), (band, rep)
+            say("band", band, "pairs", len(ref_a[1]))
+    finally:
+        ctx.timing(False)
+        ctx.nw_graphs(True)
+
+
 def test_sort_by_dist(ctx):
     rng = np.random.default_rng(5)
     n = 100000
