@@ -143,7 +143,7 @@ struct PendingEvent {
   double units;
 };
 
-constexpr int kNumWorkers = 4;
+constexpr int kNumWorkers = 2;  // with the main stream, within the 4 hardware queues a process gets
 
 struct crispy_ctx {
   int device = 0;

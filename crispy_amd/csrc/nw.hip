@@ -368,7 +368,8 @@ static unsigned family_grid(int f, int64_t np, int kb_lo, int kb_hi, int forced)
 
 // The DP phase's launches after the bucket sort: the plan, then every family
 // that the reads' kB range allows on the worker streams (family f on worker
-// f mod 4).  No host read-back and nothing sized by the histogram, so the
+// f mod 2: neighbouring families, which common length spreads put to work
+// together, run side by side).  No host read-back and nothing sized by the histogram, so the
 // sequence can be captured into a graph.  Per-family timers only when not
 // capturing.
 static int enqueue_dp(crispy_ctx *ctx, const NwFusedArgs &args, const unsigned (&grid)[kNumFamilies],
