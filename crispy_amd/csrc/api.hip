@@ -217,7 +217,7 @@ static void release_all(crispy_ctx *ctx) {
                     &ctx->km.lists,    &ctx->km.cand,      &ctx->km.keys,      &ctx->km.keys_alt,
                     &ctx->km.dists,    &ctx->km.dists_alt, &ctx->km.pairs,     &ctx->km.counter,
                     &ctx->nw.in_pairs, &ctx->nw.keys,      &ctx->nw.keys_alt,  &ctx->nw.idx,
-                    &ctx->nw.idx_alt,  &ctx->nw.dist,      &ctx->nw.hist,      &ctx->nw.cells,
+                    &ctx->nw.order,    &ctx->nw.dist,      &ctx->nw.hist,      &ctx->nw.cells,
                     &ctx->nw.flags,    &ctx->nw.sel,       &ctx->nw.nsel,      &ctx->nw.out_pairs,
                     &ctx->nw.out_dists, &ctx->nw.scratch,  &ctx->nw.plan,      &ctx->nw.stat,
                     &ctx->temp,

@@ -111,16 +111,22 @@ struct KmerState {
   bool valid = false;
 };
 
+// A captured graph of a launch sequence and the arguments it was captured
+// with (pointers, sizes, scalars); captured again when they change.
+struct GraphSegment {
+  hipGraphExec_t exec = nullptr;
+  std::vector<uintptr_t> key;
+};
+
 struct NwState {
-  DevBuf in_pairs, keys, keys_alt, idx, idx_alt, dist, hist, cells, flags, sel, nsel;
+  DevBuf in_pairs, keys, keys_alt, idx, order, dist, hist, cells, flags, sel, nsel;
   DevBuf out_pairs, out_dists, scratch;
   DevBuf plan;  // NwPlan of the DP phase (bucket table, family task lists, counters)
   DevBuf stat;  // cells per family (generic last) summed over timed runs
   DevBuf csr_rows, csr_cols;  // CSR view of the (i, j)-ordered result
   int slice_width = 0;  // sliced-kernel cells per lane: 0 = cost model, 24 or 32 = forced
-  bool graphs = true;   // DP phase through a captured graph (false: launched one by one)
-  hipGraphExec_t dp_exec = nullptr;  // the captured DP phase and what it was captured with
-  std::vector<uintptr_t> dp_key;
+  bool graphs = true;   // DP phase and filter through captured graphs (false: launched one by one)
+  GraphSegment g_dp, g_filter;
   int64_t count = 0;
   bool valid = false;
 };

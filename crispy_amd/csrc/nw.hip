@@ -38,8 +38,7 @@
 #include <cstring>
 
 #include <rocprim/device/device_radix_sort.hpp>
-#include <rocprim/device/device_select.hpp>
-#include <rocprim/iterator/counting_iterator.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -62,8 +61,11 @@ static_assert(((uint32_t)kKbSliceMax << 11 | 2047u) < kGenericKey, "bucket keys 
 // ---------------------------------------------------------------- bucketing
 // Sort key per pair: kB << 11 | R, R = min(L2, L1 + kB) the last row with an
 // in-band cell, clamped to 2047, for the register and sliced kernels; all ones
-// for the generic kernel.  Per-bucket pair and cell counts are reduced in LDS
-// first (a handful of buckets would serialise global atomics).
+// for the generic kernel.  The keys are radix-sorted with the pair indices as
+// values; the sort is stable, so equal keys keep the (i, j) order of the
+// input and a wave's pairs mostly share read i (one L1).  Per-bucket pair and
+// cell counts are reduced in LDS first (a handful of buckets would serialise
+// global atomics).
 __global__ __launch_bounds__(256) void nw_bucket_kernel(
     const int2 *__restrict__ pairs, int64_t np, const int32_t *__restrict__ len,
     const uint8_t *__restrict__ clean, int band, uint32_t *__restrict__ keys,
@@ -195,21 +197,26 @@ __device__ __forceinline__ bool nw_passes(float dist, float thr) {
 }
 
 __global__ void nw_flag_kernel(const float *__restrict__ dist, int64_t np, float thr,
-                               uint8_t *__restrict__ flags) {
+                               uint32_t *__restrict__ flags) {
   for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np;
        p += (int64_t)gridDim.x * blockDim.x)
-    flags[p] = nw_passes(dist[p], thr) ? 1 : 0;
+    flags[p] = nw_passes(dist[p], thr) ? 1u : 0u;
 }
 
-__global__ void nw_gather_kernel(const uint32_t *__restrict__ sel, const unsigned int *__restrict__ nsel,
-                                 const int2 *__restrict__ pairs, const float *__restrict__ dist,
-                                 int2 *__restrict__ out_pairs, float *__restrict__ out_dists) {
-  const int64_t n = *nsel;
-  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n;
-       k += (int64_t)gridDim.x * blockDim.x) {
-    const uint32_t p = sel[k];
-    out_pairs[k] = pairs[p];
-    out_dists[k] = dist[p];
+// The passing pairs in input order: pos = exclusive scan of the flags; the
+// count goes to *nsel.
+__global__ void nw_compact_kernel(const uint32_t *__restrict__ flags, const uint32_t *__restrict__ pos,
+                                  int64_t np, const int2 *__restrict__ pairs, const float *__restrict__ dist,
+                                  int2 *__restrict__ out_pairs, float *__restrict__ out_dists,
+                                  unsigned int *__restrict__ nsel) {
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t f = flags[p], q = pos[p];
+    if (f) {
+      out_pairs[q] = pairs[p];
+      out_dists[q] = dist[p];
+    }
+    if (p == np - 1) *nsel = q + f;
   }
 }
 
@@ -393,61 +400,88 @@ static int enqueue_dp(crispy_ctx *ctx, const NwFusedArgs &args, const unsigned (
   return join_workers(ctx);
 }
 
-static void drop_dp_graph(NwState &nw) {
-  if (nw.dp_exec) (void)hipGraphExecDestroy(nw.dp_exec);
-  nw.dp_exec = nullptr;
-  nw.dp_key.clear();
+static void drop_segment(GraphSegment &g) {
+  if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  g.exec = nullptr;
+  g.key.clear();
 }
 
-// Launches the DP phase through its captured graph, capturing (and
-// instantiating) it first when anything it was captured with has changed.
-static int launch_dp_graph(crispy_ctx *ctx, const NwFusedArgs &args, const unsigned (&grid)[kNumFamilies]) {
-  NwState &nw = ctx->nw;
-  std::vector<uintptr_t> key = {
-      (uintptr_t)args.pack, (uintptr_t)args.poff, (uintptr_t)args.len, (uintptr_t)args.pairs,
-      (uintptr_t)args.order, (uintptr_t)args.plan, (uintptr_t)args.out, (uintptr_t)nw.hist.p,
-      (uintptr_t)nw.cells.p, (uintptr_t)(ctx->timing ? nw.stat.p : nullptr), (uintptr_t)nw.slice_width};
-  for (unsigned g : grid) key.push_back(g);
-  if (!nw.dp_exec || key != nw.dp_key) {
-    drop_dp_graph(nw);
+// Runs enqueue(capturing) on the main stream through the segment's graph:
+// captured and instantiated first when the key differs from the one it was
+// captured with.  A captured graph must hold kernels and event dependencies
+// only (memset nodes have given wrong results on replays), which is checked.
+template <class Enqueue>
+static int run_segment(crispy_ctx *ctx, GraphSegment &g, std::vector<uintptr_t> key, const char *what,
+                       Enqueue &&enqueue) {
+  if (!ctx->nw.graphs) return enqueue(false);
+  if (!g.exec || key != g.key) {
+    drop_segment(g);
     CRISPY_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    const int rc = enqueue_dp(ctx, args, grid, true);
-    hipGraph_t g = nullptr;
-    const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
-    if (rc != CRISPY_OK) {
-      if (g) (void)hipGraphDestroy(g);
-      return rc;
+    const int rc = enqueue(true);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    if (rc != CRISPY_OK || e != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc != CRISPY_OK ? rc : set_err(ctx, CRISPY_ERR_HIP, "capturing %s: %s", what, hipGetErrorString(e));
     }
-    if (e != hipSuccess) return set_err(ctx, CRISPY_ERR_HIP, "capturing the NW DP phase: %s", hipGetErrorString(e));
-    const hipError_t ie = hipGraphInstantiate(&nw.dp_exec, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
+    size_t nn = 0;
+    hipError_t ge = hipGraphGetNodes(graph, nullptr, &nn);
+    std::vector<hipGraphNode_t> nodes(nn);
+    if (ge == hipSuccess && nn) ge = hipGraphGetNodes(graph, nodes.data(), &nn);
+    for (size_t k = 0; ge == hipSuccess && k < nn; ++k) {
+      hipGraphNodeType t;
+      ge = hipGraphNodeGetType(nodes[k], &t);
+      if (ge == hipSuccess && t == hipGraphNodeTypeMemset) {
+        (void)hipGraphDestroy(graph);
+        return set_err(ctx, CRISPY_ERR_HIP, "capturing %s: the graph holds a memset node", what);
+      }
+    }
+    if (ge != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      return set_err(ctx, CRISPY_ERR_HIP, "inspecting the %s graph: %s", what, hipGetErrorString(ge));
+    }
+    const hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
     if (ie != hipSuccess) {
-      nw.dp_exec = nullptr;
-      return set_err(ctx, CRISPY_ERR_HIP, "instantiating the NW DP graph: %s", hipGetErrorString(ie));
+      g.exec = nullptr;
+      return set_err(ctx, CRISPY_ERR_HIP, "instantiating the %s graph: %s", what, hipGetErrorString(ie));
     }
-    nw.dp_key = key;
+    g.key = std::move(key);
   }
-  CRISPY_HIP(ctx, hipGraphLaunch(nw.dp_exec, ctx->stream));
+  CRISPY_HIP(ctx, hipGraphLaunch(g.exec, ctx->stream));
   return CRISPY_OK;
 }
 
-// Distances of np device pairs into nw.dist (indexed like the pairs).
+// Distances of np device pairs into nw.dist (indexed like the pairs): the
+// bucketing (keys, radix sort), the DP phase (plan and family kernels; a
+// captured graph with graphs on), then the generic kernel if the reads can
+// have such pairs.
 static int nw_core(crispy_ctx *ctx, const int2 *d_pairs, int64_t np, int band) {
   ReadStore &rs = ctx->reads[CRISPY_READS_NW];
   NwState &nw = ctx->nw;
   if (np > 0xFFFFFFFFll) return set_err(ctx, CRISPY_ERR_ARG, "too many pairs in one call");
+  // kB = (L1 + L2) / (2 band) of any pair lies in [minlen / band, maxlen / band]
+  const int kb_lo = band >= 2 ? rs.minlen / band : 0;
+  const int kb_hi = band >= 2 ? rs.maxlen / band : -1;
   CRISPY_TRY(ensure(ctx, nw.keys, sizeof(uint32_t) * (np + 1)));
   CRISPY_TRY(ensure(ctx, nw.keys_alt, sizeof(uint32_t) * (np + 1)));
   CRISPY_TRY(ensure(ctx, nw.idx, sizeof(uint32_t) * (np + 1)));
-  CRISPY_TRY(ensure(ctx, nw.idx_alt, sizeof(uint32_t) * (np + 1)));
+  CRISPY_TRY(ensure(ctx, nw.order, sizeof(uint32_t) * (np + 1)));
   CRISPY_TRY(ensure(ctx, nw.dist, sizeof(float) * (np + 1)));
   CRISPY_TRY(ensure(ctx, nw.hist, sizeof(unsigned int) * kNumBuckets));
   CRISPY_TRY(ensure(ctx, nw.cells, sizeof(unsigned long long) * kNumBuckets));
   CRISPY_TRY(ensure(ctx, nw.plan, sizeof(NwPlan)));
+  if (ctx->timing) CRISPY_TRY(ensure(ctx, nw.stat, sizeof(unsigned long long) * (kNumFamilies + 1)));
   if (np == 0) return CRISPY_OK;
+  uint32_t *order = nw.order.as<uint32_t>();
+  float *out = nw.dist.as<float>();
+  NwPlan *plan = nw.plan.as<NwPlan>();
+
+  // bucketing: launched one by one (the radix sort clears its own state
+  // with memsets, which stay out of graphs)
+  hipEvent_t ev;
   CRISPY_HIP(ctx, hipMemsetAsync(nw.hist.p, 0, sizeof(unsigned int) * kNumBuckets, ctx->stream));
   CRISPY_HIP(ctx, hipMemsetAsync(nw.cells.p, 0, sizeof(unsigned long long) * kNumBuckets, ctx->stream));
-  hipEvent_t ev;
   timer_begin(ctx, "nw_bucket", &ev);
   nw_bucket_kernel<<<(int)std::min<int64_t>((np + 255) / 256, 1024), 256, 0, ctx->stream>>>(
       d_pairs, np, rs.len.as<int32_t>(), rs.clean.as<uint8_t>(), band, nw.keys.as<uint32_t>(),
@@ -455,30 +489,27 @@ static int nw_core(crispy_ctx *ctx, const int2 *d_pairs, int64_t np, int band) {
   CRISPY_LAUNCHED(ctx);
   size_t tbytes = 0;
   CRISPY_HIP(ctx, rocprim::radix_sort_pairs<RadixOnesweep>(nullptr, tbytes, nw.keys.as<uint32_t>(),
-                                            nw.keys_alt.as<uint32_t>(), nw.idx.as<uint32_t>(),
-                                            nw.idx_alt.as<uint32_t>(), (size_t)np, 0, kBucketKeyBits, ctx->stream));
+                                            nw.keys_alt.as<uint32_t>(), nw.idx.as<uint32_t>(), order,
+                                            (size_t)np, 0, kBucketKeyBits, ctx->stream));
   CRISPY_TRY(ensure(ctx, ctx->temp, tbytes));
   CRISPY_HIP(ctx, rocprim::radix_sort_pairs<RadixOnesweep>(ctx->temp.p, tbytes, nw.keys.as<uint32_t>(),
-                                            nw.keys_alt.as<uint32_t>(), nw.idx.as<uint32_t>(),
-                                            nw.idx_alt.as<uint32_t>(), (size_t)np, 0, kBucketKeyBits, ctx->stream));
+                                            nw.keys_alt.as<uint32_t>(), nw.idx.as<uint32_t>(), order,
+                                            (size_t)np, 0, kBucketKeyBits, ctx->stream));
   timer_end(ctx, "nw_bucket", ev, (double)np);
-  const uint32_t *order = nw.idx_alt.as<uint32_t>();
-  float *out = nw.dist.as<float>();
-  NwPlan *plan = nw.plan.as<NwPlan>();
-  if (ctx->timing) CRISPY_TRY(ensure(ctx, nw.stat, sizeof(unsigned long long) * (kNumFamilies + 1)));
+
   const NwFusedArgs args{rs.pack.as<uint32_t>(), rs.poff.as<int64_t>(), rs.len.as<int32_t>(),
                          d_pairs, order, plan, out};
-  // kB = (L1 + L2) / (2 band) of any pair lies in [minlen / band, maxlen / band]
-  const int kb_lo = band >= 2 ? rs.minlen / band : 0, kb_hi = band >= 2 ? rs.maxlen / band : -1;
   unsigned grid[kNumFamilies];
   for (int f = 0; f < kNumFamilies; ++f) grid[f] = family_grid(f, np, kb_lo, kb_hi, nw.slice_width);
-
+  std::vector<uintptr_t> key = {
+      (uintptr_t)args.pack, (uintptr_t)args.poff, (uintptr_t)args.len, (uintptr_t)args.pairs,
+      (uintptr_t)args.order, (uintptr_t)args.plan, (uintptr_t)args.out, (uintptr_t)nw.hist.p,
+      (uintptr_t)nw.cells.p, (uintptr_t)(ctx->timing ? nw.stat.p : nullptr), (uintptr_t)nw.slice_width};
+  for (unsigned g : grid) key.push_back(g);
   hipEvent_t phase;
   timer_begin(ctx, "nw_dp_phase", &phase);
-  if (nw.graphs)
-    CRISPY_TRY(launch_dp_graph(ctx, args, grid));
-  else
-    CRISPY_TRY(enqueue_dp(ctx, args, grid, false));
+  CRISPY_TRY(run_segment(ctx, nw.g_dp, std::move(key), "NW DP phase",
+                         [&](bool capturing) { return enqueue_dp(ctx, args, grid, capturing); }));
   // Pairs the fused kernels cannot take (full NW, non-ACGT bytes, kB past
   // kKbSliceMax) exist only if the reads allow them; then their count comes
   // back from the plan and the generic kernel runs after the families.
@@ -505,36 +536,47 @@ static int nw_core(crispy_ctx *ctx, const int2 *d_pairs, int64_t np, int band) {
   return CRISPY_OK;
 }
 
-// Keeps the pairs whose distance passes, in input order.
+// Keeps the pairs whose distance passes, in input order (flags, their
+// exclusive scan, a compaction; a captured graph with graphs on).
 static int nw_filter(crispy_ctx *ctx, const int2 *d_pairs, int64_t np, float thr, int64_t *count) {
   NwState &nw = ctx->nw;
-  CRISPY_TRY(ensure(ctx, nw.flags, np + 1));
+  CRISPY_TRY(ensure(ctx, nw.flags, sizeof(uint32_t) * (np + 1)));
   CRISPY_TRY(ensure(ctx, nw.sel, sizeof(uint32_t) * (np + 1)));
   CRISPY_TRY(ensure(ctx, nw.nsel, sizeof(unsigned int)));
   CRISPY_TRY(ensure(ctx, nw.out_pairs, sizeof(int2) * (np + 1)));
   CRISPY_TRY(ensure(ctx, nw.out_dists, sizeof(float) * (np + 1)));
   *count = 0;
   if (np == 0) return CRISPY_OK;
+  size_t tbytes = 0;
+  CRISPY_HIP(ctx, rocprim::exclusive_scan(nullptr, tbytes, nw.flags.as<uint32_t>(), nw.sel.as<uint32_t>(), 0u,
+                                          (size_t)np, rocprim::plus<uint32_t>(), ctx->stream));
+  CRISPY_TRY(ensure(ctx, ctx->temp, tbytes));
+  uint32_t thr_bits;
+  std::memcpy(&thr_bits, &thr, sizeof thr_bits);
   hipEvent_t ev;
   timer_begin(ctx, "nw_filter", &ev);
-  nw_flag_kernel<<<grid_for(np), 256, 0, ctx->stream>>>(nw.dist.as<float>(), np, thr, nw.flags.as<uint8_t>());
-  CRISPY_LAUNCHED(ctx);
-  size_t tbytes = 0;
-  rocprim::counting_iterator<uint32_t> it(0);
-  CRISPY_HIP(ctx, rocprim::select(nullptr, tbytes, it, nw.flags.as<uint8_t>(), nw.sel.as<uint32_t>(),
-                                  nw.nsel.as<unsigned int>(), (size_t)np, ctx->stream));
-  CRISPY_TRY(ensure(ctx, ctx->temp, tbytes));
-  CRISPY_HIP(ctx, rocprim::select(ctx->temp.p, tbytes, it, nw.flags.as<uint8_t>(), nw.sel.as<uint32_t>(),
-                                  nw.nsel.as<unsigned int>(), (size_t)np, ctx->stream));
-  nw_gather_kernel<<<grid_for(np), 256, 0, ctx->stream>>>(nw.sel.as<uint32_t>(), nw.nsel.as<unsigned int>(),
-                                                           d_pairs, nw.dist.as<float>(),
-                                                           nw.out_pairs.as<int2>(), nw.out_dists.as<float>());
-  CRISPY_LAUNCHED(ctx);
+  CRISPY_TRY(run_segment(
+      ctx, nw.g_filter,
+      {(uintptr_t)d_pairs, (uintptr_t)np, thr_bits, (uintptr_t)nw.dist.p, (uintptr_t)nw.flags.p,
+       (uintptr_t)nw.sel.p, (uintptr_t)nw.nsel.p, (uintptr_t)nw.out_pairs.p, (uintptr_t)nw.out_dists.p,
+       (uintptr_t)ctx->temp.p, (uintptr_t)tbytes},
+      "NW filter", [&](bool) -> int {
+        nw_flag_kernel<<<grid_for(np), 256, 0, ctx->stream>>>(nw.dist.as<float>(), np, thr, nw.flags.as<uint32_t>());
+        CRISPY_LAUNCHED(ctx);
+        size_t tb = tbytes;
+        CRISPY_HIP(ctx, rocprim::exclusive_scan(ctx->temp.p, tb, nw.flags.as<uint32_t>(), nw.sel.as<uint32_t>(),
+                                                0u, (size_t)np, rocprim::plus<uint32_t>(), ctx->stream));
+        nw_compact_kernel<<<grid_for(np), 256, 0, ctx->stream>>>(
+            nw.flags.as<uint32_t>(), nw.sel.as<uint32_t>(), np, d_pairs, nw.dist.as<float>(),
+            nw.out_pairs.as<int2>(), nw.out_dists.as<float>(), nw.nsel.as<unsigned int>());
+        CRISPY_LAUNCHED(ctx);
+        return CRISPY_OK;
+      }));
   timer_end(ctx, "nw_filter", ev, (double)np);
-  unsigned int c = 0;
-  CRISPY_HIP(ctx, hipMemcpyAsync(&c, nw.nsel.p, sizeof c, hipMemcpyDeviceToHost, ctx->stream));
+  unsigned int *h_c = reinterpret_cast<unsigned int *>(ctx->h_hist);
+  CRISPY_HIP(ctx, hipMemcpyAsync(h_c, nw.nsel.p, sizeof *h_c, hipMemcpyDeviceToHost, ctx->stream));
   CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  *count = c;
+  *count = *h_c;
   return CRISPY_OK;
 }
 
@@ -576,7 +618,10 @@ int nw_timer_reset(crispy_ctx *ctx) {
   return CRISPY_OK;
 }
 
-void nw_release_graphs(crispy_ctx *ctx) { drop_dp_graph(ctx->nw); }
+void nw_release_graphs(crispy_ctx *ctx) {
+  drop_segment(ctx->nw.g_dp);
+  drop_segment(ctx->nw.g_filter);
+}
 
 }  // namespace crispy
 
