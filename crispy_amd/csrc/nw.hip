@@ -407,15 +407,17 @@ static void drop_segment(GraphSegment &g) {
 }
 
 // Runs enqueue(capturing) on the main stream through the segment's graph:
-// captured and instantiated first when the key differs from the one it was
-// captured with.  A captured graph must hold kernels and event dependencies
-// only (memset nodes have given wrong results on replays), which is checked.
+// captured again when the key differs from the one it was captured with, and
+// then either applied to the instantiated graph in place (same launches, new
+// arguments: hipGraphExecUpdate) or instantiated anew.  A captured graph must
+// hold kernels and event dependencies only (memset nodes have given wrong
+// results on replays), which is checked.
 template <class Enqueue>
 static int run_segment(crispy_ctx *ctx, GraphSegment &g, std::vector<uintptr_t> key, const char *what,
                        Enqueue &&enqueue) {
   if (!ctx->nw.graphs) return enqueue(false);
   if (!g.exec || key != g.key) {
-    drop_segment(g);
+    g.key.clear();
     CRISPY_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     const int rc = enqueue(true);
     hipGraph_t graph = nullptr;
@@ -440,7 +442,17 @@ static int run_segment(crispy_ctx *ctx, GraphSegment &g, std::vector<uintptr_t> 
       (void)hipGraphDestroy(graph);
       return set_err(ctx, CRISPY_ERR_HIP, "inspecting the %s graph: %s", what, hipGetErrorString(ge));
     }
-    const hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    bool updated = false;
+    if (g.exec) {
+      hipGraphNode_t bad = nullptr;
+      hipGraphExecUpdateResult ur = hipGraphExecUpdateError;
+      updated = hipGraphExecUpdate(g.exec, graph, &bad, &ur) == hipSuccess && ur == hipGraphExecUpdateSuccess;
+      if (!updated) {
+        (void)hipGetLastError();
+        drop_segment(g);
+      }
+    }
+    const hipError_t ie = updated ? hipSuccess : hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (ie != hipSuccess) {
       g.exec = nullptr;
