@@ -187,7 +187,7 @@ def plain_vs_fused(args):
     chunk = 589824
 
     def plain():
-        ctx.nw_graphs(False)
+        ctx.graphs(False)
         ctx.kmer_run(K, kthr)
         kp, _ = ctx.kmer_fetch(dists=False)
         outs = []
@@ -197,7 +197,7 @@ def plain_vs_fused(args):
         return (np.concatenate([o[0] for o in outs]), np.concatenate([o[1] for o in outs])) if outs else None
 
     def fused():
-        ctx.nw_graphs(True)
+        ctx.graphs(True)
         ctx.kmer_run(K, kthr)
         ctx.nw_run_on_kmer(band, nthr)
         return ctx.nw_fetch()
@@ -236,7 +236,7 @@ def main():
     ap.add_argument("--k", type=int, default=None, help="override the workload's k")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-graphs", action="store_true",
-                    help="launch the NW DP phase kernel by kernel instead of replaying its captured graph")
+                    help="launch every kernel chain kernel by kernel instead of replaying captured graphs")
     ap.add_argument("--cpu-budget", type=float, default=24.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 1 GiB read + 1 GiB write elementwise kernel (3x), to scale "
@@ -284,7 +284,7 @@ def main():
     ctx = ca.Context(0)
     ctx.upload_reads(ca.KMER, reads)
     ctx.upload_reads(ca.NW, reads)
-    ctx.nw_graphs(not args.no_graphs)
+    ctx.graphs(not args.no_graphs)
 
     def step():
         kc = ctx.kmer_run(K, kthr)

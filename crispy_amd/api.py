@@ -68,7 +68,7 @@ def load_library():
         "crispy_nw_dists": ([_vp, _i32p, ctypes.c_int64, ctypes.c_int, _f32p], ctypes.c_int),
         "crispy_nw_csr": ([_vp, ctypes.c_int32, _i64p, _i32p, _f32p], ctypes.c_int),
         "crispy_nw_set_slice_width": ([_vp, ctypes.c_int], ctypes.c_int),
-        "crispy_nw_set_graphs": ([_vp, ctypes.c_int], ctypes.c_int),
+        "crispy_set_graphs": ([_vp, ctypes.c_int], ctypes.c_int),
         "crispy_nw_autotune": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "crispy_sort_by_dist": ([_vp, _i32p, _f32p, ctypes.c_int64], ctypes.c_int),
         "crispy_timing_enable": ([_vp, ctypes.c_int], ctypes.c_int),
@@ -319,10 +319,11 @@ class Context:
         """Band cells per lane of the lane-sliced NW kernel: 0 = cost model (default), 24 or 32."""
         self._check(self._L.crispy_nw_set_slice_width(self._h, cells))
 
-    def nw_graphs(self, enable):
-        """NW DP phase as one captured HIP graph (True, the default) or as the
-        same kernels launched one by one (False); the distances are the same."""
-        self._check(self._L.crispy_nw_set_graphs(self._h, 1 if enable else 0))
+    def graphs(self, enable):
+        """Kernel chains (k-mer profile and tile passes, NW DP phase, NW
+        filter) replayed from captured HIP graphs (True, the default) or
+        launched one by one (False); the results are the same."""
+        self._check(self._L.crispy_set_graphs(self._h, 1 if enable else 0))
 
     def nw_autotune(self, band, repeats=3):
         """Times the NW DP on the current k-mer result under the slice-width

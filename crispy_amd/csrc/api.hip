@@ -210,7 +210,7 @@ int crispy_ctx_create(int device, crispy_ctx **out) {
 }
 
 static void release_all(crispy_ctx *ctx) {
-  nw_release_graphs(ctx);
+  release_graphs(ctx);
   DevBuf *bufs[] = {&ctx->km.bitmap,   &ctx->km.T,         &ctx->km.rcnt,
                     &ctx->km.roff,     &ctx->km.rlist,     &ctx->km.excess,    &ctx->km.loff,
                     &ctx->km.roffc,    &ctx->km.rlistc,

@@ -92,6 +92,13 @@ struct ReadStore {
   int generation = 0;  // bumped on every upload (invalidates cached profiles)
 };
 
+// A captured graph of a launch sequence and the arguments it was captured
+// with (pointers, sizes, scalars); captured again when they change.
+struct GraphSegment {
+  hipGraphExec_t exec = nullptr;
+  std::vector<uintptr_t> key;
+};
+
 struct KmerState {
   // per-read profile for the current K
   int K = 0;
@@ -109,13 +116,7 @@ struct KmerState {
   int64_t cap = 0;
   int64_t count = 0;
   bool valid = false;
-};
-
-// A captured graph of a launch sequence and the arguments it was captured
-// with (pointers, sizes, scalars); captured again when they change.
-struct GraphSegment {
-  hipGraphExec_t exec = nullptr;
-  std::vector<uintptr_t> key;
+  GraphSegment g_profile, g_tile;  // captured profile pass and tile pass
 };
 
 struct NwState {
@@ -125,8 +126,7 @@ struct NwState {
   DevBuf stat;  // cells per family (generic last) summed over timed runs
   DevBuf csr_rows, csr_cols;  // CSR view of the (i, j)-ordered result
   int slice_width = 0;  // sliced-kernel cells per lane: 0 = cost model, 24 or 32 = forced
-  bool graphs = true;   // DP phase and filter through captured graphs (false: launched one by one)
-  GraphSegment g_dp, g_filter;
+  GraphSegment g_dp, g_filter;  // captured DP phase and filter
   int64_t count = 0;
   bool valid = false;
 };
@@ -171,6 +171,7 @@ struct crispy_ctx {
   hipEvent_t stage_ev = nullptr;  // the last upload from h_stage has been read
   void *h_hist = nullptr;  // pinned slot for small device read-backs
   bool timing = false;
+  bool graphs = true;  // kernel chains through captured graphs (false: launched one by one)
   std::vector<PendingEvent> pending;
   std::vector<hipEvent_t> pool;
   std::map<std::string, KStat> stats;
@@ -194,7 +195,7 @@ int join_workers(crispy_ctx *ctx);
 // device while timing is on); other names leave *units alone.
 int nw_timer_units(crispy_ctx *ctx, const char *name, double *units);
 int nw_timer_reset(crispy_ctx *ctx);
-void nw_release_graphs(crispy_ctx *ctx);
+void release_graphs(crispy_ctx *ctx);
 
 }  // namespace crispy
 

@@ -35,6 +35,7 @@
 #include <utility>
 
 #include "common.hpp"
+#include "graph_segment.hpp"
 #include "sort_config.hpp"
 
 namespace crispy {
@@ -658,45 +659,62 @@ __global__ __launch_bounds__(256) void compact_repeats_kernel(const int32_t *__r
 template <int K>
 static int launch_profile(crispy_ctx *ctx, ReadStore &rs) {
   KmerState &km = ctx->km;
-  hipEvent_t ev;
-  timer_begin(ctx, "kmer_profile", &ev);
   kmer_profile_kernel<K><<<rs.n, 256, 0, ctx->stream>>>(
       rs.pack.as<uint32_t>(), rs.poff.as<int64_t>(), rs.len.as<int32_t>(), km.bitmap.as<uint32_t>(),
       km.T.as<int32_t>(), km.rcnt.as<int32_t>(), km.roff.as<int64_t>(),
       km.rlist.as<uint32_t>(), km.excess.as<int32_t>());
   CRISPY_LAUNCHED(ctx);
-  timer_end(ctx, "kmer_profile", ev, rs.n);
   return CRISPY_OK;
 }
 
+// Zeroes a result counter (a kernel, not a memset: it is captured with the
+// tile pass).
+__global__ void zero_counter_kernel(unsigned long long *c) { *c = 0ull; }
+
+// Read pairs i < j < n inside tile slots [tb, tb + nt): the tile pass's units.
+static double tile_units(int32_t n, int nb, int64_t tb, int64_t nt) {
+  if (tb == 0 && nt == num_slots(nb)) return (double)n * (n - 1) / 2;
+  double units = 0;
+  for (int64_t t = tb; t < tb + nt; ++t) {
+    int bi, bj;
+    if (!slot_coords(t, nb, bi, bj)) continue;
+    const int64_t rh = std::min<int64_t>(kTile, n - (int64_t)bi * kTile);
+    const int64_t chh = std::min<int64_t>(kTile, n - (int64_t)bj * kTile);
+    units += bi == bj ? (double)rh * (rh - 1) / 2 : (double)rh * chh;
+  }
+  return units;
+}
+
+// The tile pass: zero the result counter, then the tile kernel over slots
+// [tb, tb + nt) (a captured graph with graphs on).
 template <int K, bool CAND>
 static int launch_tiles(crispy_ctx *ctx, int32_t n, int nb, int64_t tb, int64_t nt, float thr,
                         int tshift, uint64_t *out_keys, float *out_dists, int64_t cap) {
   KmerState &km = ctx->km;
   const int64_t per_xcd = (nt + 7) / 8;
   const int64_t grid = per_xcd * 8;
+  const float cut = 1.0f - thr - 2e-5f;
+  uint32_t thr_bits;
+  std::memcpy(&thr_bits, &thr, sizeof thr_bits);
   hipEvent_t ev;
   timer_begin(ctx, "kmer_tile", &ev);
-  kmer_tile_mfma_kernel<K, CAND><<<(unsigned)grid, 128, 0, ctx->stream>>>(
-      km.bitmap.as<uint32_t>(), km.T.as<int32_t>(), km.rcnt.as<int32_t>(), km.roffc.as<int64_t>(),
-      km.rlistc.as<uint32_t>(), km.excess.as<int32_t>(), n, nb, tb, nt, per_xcd, thr, 1.0f - thr - 2e-5f,
-      tshift, km.kshift, km.counter.as<unsigned long long>(), out_keys, out_dists, cap);
-  CRISPY_LAUNCHED(ctx);
-  double units = 0;  // read pairs i < j < n inside the selected tiles
-  if (ctx->timing) {
-    if (tb == 0 && nt == num_slots(nb)) {
-      units = (double)n * (n - 1) / 2;
-    } else {
-      for (int64_t t = tb; t < tb + nt; ++t) {
-        int bi, bj;
-        if (!slot_coords(t, nb, bi, bj)) continue;
-        const int64_t rh = std::min<int64_t>(kTile, n - (int64_t)bi * kTile);
-        const int64_t chh = std::min<int64_t>(kTile, n - (int64_t)bj * kTile);
-        units += bi == bj ? (double)rh * (rh - 1) / 2 : (double)rh * chh;
-      }
-    }
-  }
-  timer_end(ctx, "kmer_tile", ev, units);
+  CRISPY_TRY(run_segment(
+      ctx, km.g_tile,
+      {(uintptr_t)K, (uintptr_t)CAND, (uintptr_t)km.bitmap.p, (uintptr_t)km.T.p, (uintptr_t)km.rcnt.p,
+       (uintptr_t)km.roffc.p, (uintptr_t)km.rlistc.p, (uintptr_t)km.excess.p, (uintptr_t)n, (uintptr_t)nb,
+       (uintptr_t)tb, (uintptr_t)nt, thr_bits, (uintptr_t)tshift, (uintptr_t)km.kshift, (uintptr_t)km.counter.p,
+       (uintptr_t)out_keys, (uintptr_t)out_dists, (uintptr_t)cap},
+      "k-mer tile pass", [&](bool) -> int {
+        zero_counter_kernel<<<1, 1, 0, ctx->stream>>>(km.counter.as<unsigned long long>());
+        CRISPY_LAUNCHED(ctx);
+        kmer_tile_mfma_kernel<K, CAND><<<(unsigned)grid, 128, 0, ctx->stream>>>(
+            km.bitmap.as<uint32_t>(), km.T.as<int32_t>(), km.rcnt.as<int32_t>(), km.roffc.as<int64_t>(),
+            km.rlistc.as<uint32_t>(), km.excess.as<int32_t>(), n, nb, tb, nt, per_xcd, thr, cut, tshift,
+            km.kshift, km.counter.as<unsigned long long>(), out_keys, out_dists, cap);
+        CRISPY_LAUNCHED(ctx);
+        return CRISPY_OK;
+      }));
+  timer_end(ctx, "kmer_tile", ev, ctx->timing ? tile_units(n, nb, tb, nt) : 0.0);
   return CRISPY_OK;
 }
 
@@ -739,24 +757,8 @@ static int build_profiles(crispy_ctx *ctx, int KP, int K) {
   CRISPY_TRY(ensure(ctx, km.roff, sizeof(int64_t) * (n + 1)));
   CRISPY_TRY(ensure(ctx, km.rlist, sizeof(uint32_t) * (roff[n] + 1)));
   CRISPY_HIP(ctx, hipMemcpyAsync(km.roff.p, roff, offs, hipMemcpyHostToDevice, ctx->stream));
-  CRISPY_TRY(kProfile[KP](ctx, rs));
   CRISPY_TRY(ensure(ctx, km.roffc, sizeof(int64_t) * (n + 1)));
   CRISPY_TRY(ensure(ctx, km.rlistc, sizeof(uint32_t) * (roff[n] + 1)));
-  {
-    size_t tbytes = 0;
-    CRISPY_HIP(ctx, rocprim::exclusive_scan(nullptr, tbytes, km.rcnt.as<int32_t>(), km.roffc.as<int64_t>(),
-                                            (int64_t)0, (size_t)n, rocprim::plus<int64_t>(), ctx->stream));
-    CRISPY_TRY(ensure(ctx, ctx->temp, tbytes));
-    hipEvent_t ev;
-    timer_begin(ctx, "kmer_profile", &ev);
-    CRISPY_HIP(ctx, rocprim::exclusive_scan(ctx->temp.p, tbytes, km.rcnt.as<int32_t>(), km.roffc.as<int64_t>(),
-                                            (int64_t)0, (size_t)n, rocprim::plus<int64_t>(), ctx->stream));
-    compact_repeats_kernel<<<(int)std::min<int64_t>((n + 3) / 4, 4096), 256, 0, ctx->stream>>>(
-        km.rcnt.as<int32_t>(), km.roff.as<int64_t>(), km.rlist.as<uint32_t>(), km.roffc.as<int64_t>(),
-        km.rlistc.as<uint32_t>(), n);
-    CRISPY_LAUNCHED(ctx);
-    timer_end(ctx, "kmer_profile", ev, n);
-  }
   if (K > 6) {  // the K-mer lists for the exact merge of the candidates
     int64_t *loff = roff + (n + 1);
     loff[0] = 0;
@@ -764,27 +766,50 @@ static int build_profiles(crispy_ctx *ctx, int KP, int K) {
     CRISPY_TRY(ensure(ctx, km.loff, offs));
     CRISPY_TRY(ensure(ctx, km.lists, sizeof(uint16_t) * (loff[n] + 8)));
     CRISPY_HIP(ctx, hipMemcpyAsync(km.loff.p, loff, offs, hipMemcpyHostToDevice, ctx->stream));
-    hipEvent_t ev;
-    timer_begin(ctx, "kmer_profile", &ev);
-    if (K == 7)
-      kmer_list_kernel<7><<<n, 256, 0, ctx->stream>>>(rs.pack.as<uint32_t>(), rs.poff.as<int64_t>(),
-                                                     rs.len.as<int32_t>(), km.loff.as<int64_t>(),
-                                                     km.lists.as<uint16_t>());
-    else
-      kmer_list_kernel<8><<<n, 256, 0, ctx->stream>>>(rs.pack.as<uint32_t>(), rs.poff.as<int64_t>(),
-                                                     rs.len.as<int32_t>(), km.loff.as<int64_t>(),
-                                                     km.lists.as<uint16_t>());
-    CRISPY_LAUNCHED(ctx);
-    timer_end(ctx, "kmer_profile", ev, n);
   }
+  size_t tbytes = 0;
+  CRISPY_HIP(ctx, rocprim::exclusive_scan(nullptr, tbytes, km.rcnt.as<int32_t>(), km.roffc.as<int64_t>(),
+                                          (int64_t)0, (size_t)n, rocprim::plus<int64_t>(), ctx->stream));
+  CRISPY_TRY(ensure(ctx, ctx->temp, tbytes));
+  // The profile pass (a captured graph with graphs on): profiles, the
+  // repeat lists compacted in read order, and for K > 6 the K-mer lists.
+  hipEvent_t ev;
+  timer_begin(ctx, "kmer_profile", &ev);
+  CRISPY_TRY(run_segment(
+      ctx, km.g_profile,
+      {(uintptr_t)KP, (uintptr_t)K, (uintptr_t)n, (uintptr_t)rs.pack.p, (uintptr_t)rs.poff.p, (uintptr_t)rs.len.p,
+       (uintptr_t)km.bitmap.p, (uintptr_t)km.T.p, (uintptr_t)km.rcnt.p, (uintptr_t)km.roff.p, (uintptr_t)km.rlist.p,
+       (uintptr_t)km.excess.p, (uintptr_t)km.roffc.p, (uintptr_t)km.rlistc.p, (uintptr_t)ctx->temp.p,
+       (uintptr_t)tbytes, (uintptr_t)(K > 6 ? km.loff.p : nullptr), (uintptr_t)(K > 6 ? km.lists.p : nullptr)},
+      "k-mer profile pass", [&](bool) -> int {
+        CRISPY_TRY(kProfile[KP](ctx, rs));
+        size_t tb = tbytes;
+        CRISPY_HIP(ctx, rocprim::exclusive_scan(ctx->temp.p, tb, km.rcnt.as<int32_t>(), km.roffc.as<int64_t>(),
+                                                (int64_t)0, (size_t)n, rocprim::plus<int64_t>(), ctx->stream));
+        compact_repeats_kernel<<<(int)std::min<int64_t>((n + 3) / 4, 4096), 256, 0, ctx->stream>>>(
+            km.rcnt.as<int32_t>(), km.roff.as<int64_t>(), km.rlist.as<uint32_t>(), km.roffc.as<int64_t>(),
+            km.rlistc.as<uint32_t>(), n);
+        CRISPY_LAUNCHED(ctx);
+        if (K == 7)
+          kmer_list_kernel<7><<<n, 256, 0, ctx->stream>>>(rs.pack.as<uint32_t>(), rs.poff.as<int64_t>(),
+                                                         rs.len.as<int32_t>(), km.loff.as<int64_t>(),
+                                                         km.lists.as<uint16_t>());
+        else if (K == 8)
+          kmer_list_kernel<8><<<n, 256, 0, ctx->stream>>>(rs.pack.as<uint32_t>(), rs.poff.as<int64_t>(),
+                                                         rs.len.as<int32_t>(), km.loff.as<int64_t>(),
+                                                         km.lists.as<uint16_t>());
+        CRISPY_LAUNCHED(ctx);
+        return CRISPY_OK;
+      }));
+  timer_end(ctx, "kmer_profile", ev, n);
   CRISPY_HIP(ctx, hipEventRecord(ctx->stage_ev, ctx->stream));
   km.K = K;
   km.profile_gen = rs.generation;
   return CRISPY_OK;
 }
 
-// Runs `launch` (which appends to km.counter) and grows the output once if
-// it overflowed `cap`; returns the final count.
+// Runs `launch` (which zeroes km.counter, then appends to it) and grows the
+// output once if it overflowed `cap`; returns the final count.
 template <class F>
 static int run_counted(crispy_ctx *ctx, int64_t &cap, DevBuf &keys, DevBuf *dists, int64_t *count,
                        F &&launch) {
@@ -793,7 +818,6 @@ static int run_counted(crispy_ctx *ctx, int64_t &cap, DevBuf &keys, DevBuf *dist
   for (int attempt = 0; attempt < 2; ++attempt) {
     CRISPY_TRY(ensure(ctx, keys, sizeof(uint64_t) * cap));
     if (dists) CRISPY_TRY(ensure(ctx, *dists, sizeof(float) * cap));
-    CRISPY_HIP(ctx, hipMemsetAsync(km.counter.p, 0, sizeof(unsigned long long), ctx->stream));
     CRISPY_TRY(launch());
     unsigned long long c = 0;
     CRISPY_HIP(ctx, hipMemcpyAsync(&c, km.counter.p, sizeof c, hipMemcpyDeviceToHost, ctx->stream));
@@ -904,6 +928,8 @@ int crispy_kmer_run(crispy_ctx *ctx, int K, float threshold, int64_t tile_begin,
       cand = km.cand_alt.as<uint64_t>();
     }
     CRISPY_TRY(run_counted(ctx, km.cap, km.keys, &km.dists, &km.count, [&]() -> int {
+      zero_counter_kernel<<<1, 1, 0, ctx->stream>>>(km.counter.as<unsigned long long>());
+      CRISPY_LAUNCHED(ctx);
       if (ncand == 0) return CRISPY_OK;
       hipEvent_t ev;
       timer_begin(ctx, "kmer_cand", &ev);

@@ -46,6 +46,7 @@
 
 #include "common.hpp"
 #include "sort_config.hpp"
+#include "graph_segment.hpp"
 #include "nw_fused.hpp"
 
 namespace crispy {
@@ -400,70 +401,6 @@ static int enqueue_dp(crispy_ctx *ctx, const NwFusedArgs &args, const unsigned (
   return join_workers(ctx);
 }
 
-static void drop_segment(GraphSegment &g) {
-  if (g.exec) (void)hipGraphExecDestroy(g.exec);
-  g.exec = nullptr;
-  g.key.clear();
-}
-
-// Runs enqueue(capturing) on the main stream through the segment's graph:
-// captured again when the key differs from the one it was captured with, and
-// then either applied to the instantiated graph in place (same launches, new
-// arguments: hipGraphExecUpdate) or instantiated anew.  A captured graph must
-// hold kernels and event dependencies only (memset nodes have given wrong
-// results on replays), which is checked.
-template <class Enqueue>
-static int run_segment(crispy_ctx *ctx, GraphSegment &g, std::vector<uintptr_t> key, const char *what,
-                       Enqueue &&enqueue) {
-  if (!ctx->nw.graphs) return enqueue(false);
-  if (!g.exec || key != g.key) {
-    g.key.clear();
-    CRISPY_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    const int rc = enqueue(true);
-    hipGraph_t graph = nullptr;
-    const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
-    if (rc != CRISPY_OK || e != hipSuccess) {
-      if (graph) (void)hipGraphDestroy(graph);
-      return rc != CRISPY_OK ? rc : set_err(ctx, CRISPY_ERR_HIP, "capturing %s: %s", what, hipGetErrorString(e));
-    }
-    size_t nn = 0;
-    hipError_t ge = hipGraphGetNodes(graph, nullptr, &nn);
-    std::vector<hipGraphNode_t> nodes(nn);
-    if (ge == hipSuccess && nn) ge = hipGraphGetNodes(graph, nodes.data(), &nn);
-    for (size_t k = 0; ge == hipSuccess && k < nn; ++k) {
-      hipGraphNodeType t;
-      ge = hipGraphNodeGetType(nodes[k], &t);
-      if (ge == hipSuccess && t == hipGraphNodeTypeMemset) {
-        (void)hipGraphDestroy(graph);
-        return set_err(ctx, CRISPY_ERR_HIP, "capturing %s: the graph holds a memset node", what);
-      }
-    }
-    if (ge != hipSuccess) {
-      (void)hipGraphDestroy(graph);
-      return set_err(ctx, CRISPY_ERR_HIP, "inspecting the %s graph: %s", what, hipGetErrorString(ge));
-    }
-    bool updated = false;
-    if (g.exec) {
-      hipGraphNode_t bad = nullptr;
-      hipGraphExecUpdateResult ur = hipGraphExecUpdateError;
-      updated = hipGraphExecUpdate(g.exec, graph, &bad, &ur) == hipSuccess && ur == hipGraphExecUpdateSuccess;
-      if (!updated) {
-        (void)hipGetLastError();
-        drop_segment(g);
-      }
-    }
-    const hipError_t ie = updated ? hipSuccess : hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    if (ie != hipSuccess) {
-      g.exec = nullptr;
-      return set_err(ctx, CRISPY_ERR_HIP, "instantiating the %s graph: %s", what, hipGetErrorString(ie));
-    }
-    g.key = std::move(key);
-  }
-  CRISPY_HIP(ctx, hipGraphLaunch(g.exec, ctx->stream));
-  return CRISPY_OK;
-}
-
 // Distances of np device pairs into nw.dist (indexed like the pairs): the
 // bucketing (keys, radix sort), the DP phase (plan and family kernels; a
 // captured graph with graphs on), then the generic kernel if the reads can
@@ -630,9 +567,11 @@ int nw_timer_reset(crispy_ctx *ctx) {
   return CRISPY_OK;
 }
 
-void nw_release_graphs(crispy_ctx *ctx) {
+void release_graphs(crispy_ctx *ctx) {
   drop_segment(ctx->nw.g_dp);
   drop_segment(ctx->nw.g_filter);
+  drop_segment(ctx->km.g_profile);
+  drop_segment(ctx->km.g_tile);
 }
 
 }  // namespace crispy
@@ -641,9 +580,9 @@ using namespace crispy;
 
 extern "C" {
 
-int crispy_nw_set_graphs(crispy_ctx *ctx, int enable) {
+int crispy_set_graphs(crispy_ctx *ctx, int enable) {
   if (!ctx) return CRISPY_ERR_ARG;
-  ctx->nw.graphs = enable != 0;
+  ctx->graphs = enable != 0;
   return CRISPY_OK;
 }
 

@@ -143,12 +143,15 @@ int crispy_nw_csr(crispy_ctx *ctx, int32_t n_rows, int64_t *row_ptr, int32_t *co
  * The distances do not depend on it.  No counterpart in the reference: a
  * tuning and test switch. */
 int crispy_nw_set_slice_width(crispy_ctx *ctx, int cells);
-/* NW DP phase launch form: 1 (the default) = the plan kernel and the six
- * persistent family kernels replay as one captured HIP graph (captured again
- * only when a buffer, the slice-width policy, timing or a grid size changes);
- * 0 = the same kernels launched one by one.  The distances do not depend on
- * it.  No counterpart in the reference: a launch-form switch. */
-int crispy_nw_set_graphs(crispy_ctx *ctx, int enable);
+/* Launch form of the context's kernel chains: 1 (the default) = the k-mer
+ * profile pass, the k-mer tile pass, the NW DP phase (plan kernel + family
+ * kernels) and the NW filter each replay a captured HIP graph, captured on
+ * first use and updated (or captured again) when a buffer, a size or a
+ * setting changes; 0 = the same kernels launched one by one.  Results do not
+ * depend on it.  With graphs on, the context's stream must not itself be
+ * under capture by the caller.  No counterpart in the reference: a
+ * launch-form switch. */
+int crispy_set_graphs(crispy_ctx *ctx, int enable);
 /* Autotuner for that choice: on the context's current k-mer result, times the
  * NW DP (bucketing, sort and fused launches; fastest of `repeats` runs after a
  * warm-up) under the cost model and under each forced width, keeps the

@@ -324,11 +324,12 @@ def test_nw_autotune(ctx):
 
 
 def test_nw_graph_replays(ctx):
-    """The DP phase replayed from its captured graph equals the same kernels
-    launched one by one, on every replay: two inputs alternate (each switch
-    captures again), with eager work and synchronisations between replays,
-    timing toggled (captured again), register and sliced families, and reads
-    with non-ACGT bytes (generic kernel after the graph)."""
+    """The k-mer passes and the NW DP phase and filter replayed from their
+    captured graphs equal the same kernels launched one by one, on every
+    replay: two NW inputs alternate (each switch updates the graph), with
+    eager work and synchronisations between replays, timing toggled (new
+    arguments), register and sliced families, and reads with non-ACGT bytes
+    (generic kernel after the graph)."""
     say("nw graph replays")
     seqs, _, _ = synth.make("reads1k")
     rng = np.random.default_rng(8)
@@ -344,16 +345,20 @@ def test_nw_graph_replays(ctx):
     extra = rng.integers(0, r.n, (3000, 2)).astype(np.int32)
     try:
         for band in (10, 3):
-            ctx.nw_graphs(False)
+            ctx.graphs(False)
             ctx.kmer_run(6, 0.5)
+            ref_k = ctx.kmer_fetch()
             ctx.nw_run_on_kmer(band, 0.45)
             ref_a = ctx.nw_fetch()
             ref_b = ctx.nw_dists(extra, band)
             od, _, _ = O.nw_pairs(ro, extra, band)
             assert np.array_equal(bits(ref_b), bits(od))
-            ctx.nw_graphs(True)
+            ctx.graphs(True)
             for rep in range(4):
                 ctx.timing(rep == 2)
+                ctx.kmer_run(6, 0.5)
+                k = ctx.kmer_fetch()
+                assert np.array_equal(k[0], ref_k[0]) and np.array_equal(bits(k[1]), bits(ref_k[1])), (band, rep)
                 ctx.nw_run_on_kmer(band, 0.45)
                 got = ctx.nw_fetch()
                 assert np.array_equal(got[0], ref_a[0]) and np.array_equal(bits(got[1]), bits(ref_a[1])), (band, rep)
@@ -364,7 +369,7 @@ def test_nw_graph_replays(ctx):
             say("band", band, "pairs", len(ref_a[1]))
     finally:
         ctx.timing(False)
-        ctx.nw_graphs(True)
+        ctx.graphs(True)
 
 
 def test_sort_by_dist(ctx):
