@@ -199,10 +199,14 @@ __host__ __device__ inline int64_t num_slots(int nb) {
 // count-1 over a read's codes), so a pair with
 //   popcount + min(excess_i, excess_j) < min(F_i, F_j),  F = (T - tshift) * cut - 1
 // cannot pass (F rises with T, so min(F_i, F_j) = F(min(T_i, T_j)); the margin
-// in cut covers the fp32 roundings).  The test is branch-free: 5 VALU ops per
-// pair (the minima of the non-negative F and excess as unsigned minima of
-// their bit patterns), the sign of the difference shifted into a 16-bit reject
-// mask per fragment.  F <= 0 is stored as 0: such a pair survives either way.
+// in cut covers the fp32 roundings).  The tile tests the slightly weaker
+//   popcount < min(G_i, G_j),  G = F - excess
+// (if F_i <= F_j: min(F) - min(E) = F_i - min(E) >= F_i - E_i >= min(G); the
+// same with i, j swapped), which needs one per-read value: 3 VALU ops per pair
+// (the minimum of the non-negative G as the unsigned minimum of its bit
+// patterns, the difference, its sign shifted into a 16-bit reject mask per
+// fragment).  G <= 0 is stored as 0: such a pair survives either way.  A
+// positive F - E is exact in fp32 (E is an integer below F).
 // Diagonal tiles (i < j) and the tiles of the last row or column block (reads
 // past n) mask their fragments explicitly.  The few survivors of the wave's 8
 // fragments are queued in LDS and processed 64 at a time for the exact count
@@ -257,8 +261,7 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
   constexpr int kRow = kChunkWords + 4;  // padded row stride of a staged chunk (words)
   constexpr int kStageWords = 2 * kTile * kRow;
   __shared__ uint32_t Ps[kStageWords];  // bitmap chunk in the main loop, then repeat lists
-  __shared__ __align__(16) float sF[2 * kTile];
-  __shared__ __align__(16) float sEf[2 * kTile];
+  __shared__ __align__(16) float sG[2 * kTile];
   __shared__ int32_t sT[2 * kTile];
   __shared__ int32_t sRc[2 * kTile];
   __shared__ int64_t sRo[2 * kTile];
@@ -283,9 +286,9 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
     sT[q] = T;
     sRc[q] = ok ? rcnt[r] : 0;
     sRo[q] = ok ? roff[r] : 0;
-    sEf[q] = ok ? (float)excess[r] : 0.0f;
-    const float F = fmaf((float)(T - tshift), cut, -1.0f);
-    sF[q] = ok && F > 0.0f ? F : 0.0f;
+    // G = F - E (exact in fp32 when positive: E is an integer below F), >= 0
+    const float G = fmaf((float)(T - tshift), cut, -1.0f) - (float)(ok ? excess[r] : 0);
+    sG[q] = ok && G > 0.0f ? G : 0.0f;
   }
 
   // loader: a chunk is 16 words (64 bytes) of each of the 256 reads; lane quad
@@ -387,20 +390,17 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
   static_for<2>([&](auto FA) {
     constexpr int fa = decltype(FA)::value;
     const int rb = wv * 64 + fa * 32 + 4 * g;  // rows rb + 8q + t hold accumulators 4q + t
-    lds_f4 fr[4], er[4];
+    lds_f4 gr[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      fr[q] = *reinterpret_cast<const lds_f4 *>(&sF[rb + 8 * q]);
-      er[q] = *reinterpret_cast<const lds_f4 *>(&sEf[rb + 8 * q]);
-    }
+    for (int q = 0; q < 4; ++q) gr[q] = *reinterpret_cast<const lds_f4 *>(&sG[rb + 8 * q]);
     static_for<4>([&](auto FB) {
       constexpr int fb = decltype(FB)::value;
       const int lj = kTile + fb * 32 + r32;
-      const float fc = sF[lj], ec = sEf[lj];
+      const float gc = sG[lj];
       uint32_t rej = 0;
       static_for<16>([&](auto R) {
         constexpr int r = 15 - decltype(R)::value;  // bit r of the mask ends up holding pair r
-        const float d = (acc[fa][fb][r] + umin_f(er[r >> 2][r & 3], ec)) - umin_f(fr[r >> 2][r & 3], fc);
+        const float d = acc[fa][fb][r] - umin_f(gr[r >> 2][r & 3], gc);
         rej = __builtin_amdgcn_alignbit(rej, __float_as_uint(d), 31);  // rej << 1 | sign(d)
       });
       uint32_t k = ~rej & 0xFFFFu;
