@@ -53,11 +53,23 @@ namespace crispy {
 
 const int2 *kmer_result_pairs(crispy_ctx *ctx);
 
-// Bucket keys kB << 11 | R (21 bits: kB <= kKbSliceMax < 1024, R <= 2047);
-// the generic kernel's pairs sort last.
-constexpr int kBucketKeyBits = 21;
-constexpr uint32_t kGenericKey = (1u << kBucketKeyBits) - 1;
-static_assert(((uint32_t)kKbSliceMax << 11 | 2047u) < kGenericKey, "bucket keys fit below the generic key");
+// Bucket keys (kB - kB_lo) << rbits | R with R < 2^rbits, kB_lo the smallest
+// kB the reads allow; key_bits = kb_bits + rbits where 2^kb_bits exceeds the
+// span of possible kB by at least one, so the all-ones key of the generic
+// kernel's pairs sorts after every other (and as few radix passes as the
+// read lengths need run: 2 instead of 3 for 150-550 bp reads at band 10).
+struct BucketKey {
+  int kb_lo, rbits, bits;
+};
+
+static BucketKey bucket_key(int kb_lo, int kb_top, int maxlen) {
+  BucketKey k{kb_lo, 1, 0};
+  while ((1 << k.rbits) <= std::min(maxlen, 2047)) ++k.rbits;
+  int kb_bits = 0;
+  while ((1 << kb_bits) < std::max(kb_top - kb_lo, 0) + 2) ++kb_bits;
+  k.bits = kb_bits + k.rbits;
+  return k;
+}
 
 // ---------------------------------------------------------------- bucketing
 // Sort key per pair: kB << 11 | R, R = min(L2, L1 + kB) the last row with an
@@ -69,9 +81,10 @@ static_assert(((uint32_t)kKbSliceMax << 11 | 2047u) < kGenericKey, "bucket keys 
 // global atomics).
 __global__ __launch_bounds__(256) void nw_bucket_kernel(
     const int2 *__restrict__ pairs, int64_t np, const int32_t *__restrict__ len,
-    const uint8_t *__restrict__ clean, int band, uint32_t *__restrict__ keys,
+    const uint8_t *__restrict__ clean, int band, BucketKey bk, uint32_t *__restrict__ keys,
     uint32_t *__restrict__ idx, unsigned int *__restrict__ hist,
     unsigned long long *__restrict__ cells) {
+  const uint32_t generic_key = (1u << bk.bits) - 1;
   __shared__ unsigned int s_hist[kNumBuckets];
   __shared__ unsigned long long s_cells[kNumBuckets];
   for (int b = threadIdx.x; b < kNumBuckets; b += blockDim.x) {
@@ -83,7 +96,7 @@ __global__ __launch_bounds__(256) void nw_bucket_kernel(
        p += (int64_t)gridDim.x * blockDim.x) {
     const int2 pr = pairs[p];
     const int L1 = len[pr.x], L2 = len[pr.y];
-    uint32_t key = kGenericKey;
+    uint32_t key = generic_key;
     int bucket = kGenericBucket;
     int64_t nc;
     if (band >= 2) {
@@ -92,7 +105,7 @@ __global__ __launch_bounds__(256) void nw_bucket_kernel(
       if (kb <= kKbSliceMax && clean[pr.x] && clean[pr.y]) {
         // rows past L1 + kB hold no in-band cell: the DP ends at min(L2, L1 + kB)
         const int rows = min(L2, L1 + kb);
-        key = ((uint32_t)kb << 11) | (uint32_t)min(rows, 2047);
+        key = ((uint32_t)(kb - bk.kb_lo) << bk.rbits) | (uint32_t)min(rows, 2047);
         bucket = kb;
       }
     } else {
@@ -428,22 +441,23 @@ static int nw_core(crispy_ctx *ctx, const int2 *d_pairs, int64_t np, int band) {
 
   // bucketing: launched one by one (the radix sort clears its own state
   // with memsets, which stay out of graphs)
+  const BucketKey bkey = bucket_key(kb_lo, std::min(kb_hi, kKbSliceMax), rs.maxlen);
   hipEvent_t ev;
   CRISPY_HIP(ctx, hipMemsetAsync(nw.hist.p, 0, sizeof(unsigned int) * kNumBuckets, ctx->stream));
   CRISPY_HIP(ctx, hipMemsetAsync(nw.cells.p, 0, sizeof(unsigned long long) * kNumBuckets, ctx->stream));
   timer_begin(ctx, "nw_bucket", &ev);
   nw_bucket_kernel<<<(int)std::min<int64_t>((np + 255) / 256, 1024), 256, 0, ctx->stream>>>(
-      d_pairs, np, rs.len.as<int32_t>(), rs.clean.as<uint8_t>(), band, nw.keys.as<uint32_t>(),
+      d_pairs, np, rs.len.as<int32_t>(), rs.clean.as<uint8_t>(), band, bkey, nw.keys.as<uint32_t>(),
       nw.idx.as<uint32_t>(), nw.hist.as<unsigned int>(), nw.cells.as<unsigned long long>());
   CRISPY_LAUNCHED(ctx);
   size_t tbytes = 0;
   CRISPY_HIP(ctx, rocprim::radix_sort_pairs<RadixOnesweep>(nullptr, tbytes, nw.keys.as<uint32_t>(),
                                             nw.keys_alt.as<uint32_t>(), nw.idx.as<uint32_t>(), order,
-                                            (size_t)np, 0, kBucketKeyBits, ctx->stream));
+                                            (size_t)np, 0, bkey.bits, ctx->stream));
   CRISPY_TRY(ensure(ctx, ctx->temp, tbytes));
   CRISPY_HIP(ctx, rocprim::radix_sort_pairs<RadixOnesweep>(ctx->temp.p, tbytes, nw.keys.as<uint32_t>(),
                                             nw.keys_alt.as<uint32_t>(), nw.idx.as<uint32_t>(), order,
-                                            (size_t)np, 0, kBucketKeyBits, ctx->stream));
+                                            (size_t)np, 0, bkey.bits, ctx->stream));
   timer_end(ctx, "nw_bucket", ev, (double)np);
 
   const NwFusedArgs args{rs.pack.as<uint32_t>(), rs.poff.as<int64_t>(), rs.len.as<int32_t>(),
