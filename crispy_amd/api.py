@@ -69,6 +69,7 @@ def load_library():
         "crispy_nw_csr": ([_vp, ctypes.c_int32, _i64p, _i32p, _f32p], ctypes.c_int),
         "crispy_nw_set_slice_width": ([_vp, ctypes.c_int], ctypes.c_int),
         "crispy_set_graphs": ([_vp, ctypes.c_int], ctypes.c_int),
+        "crispy_kmer_set_exact_pass": ([_vp, ctypes.c_int], ctypes.c_int),
         "crispy_nw_autotune": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "crispy_sort_by_dist": ([_vp, _i32p, _f32p, ctypes.c_int64], ctypes.c_int),
         "crispy_timing_enable": ([_vp, ctypes.c_int], ctypes.c_int),
@@ -318,6 +319,13 @@ class Context:
     def nw_slice_width(self, cells):
         """Band cells per lane of the lane-sliced NW kernel: 0 = cost model (default), 24 or 32."""
         self._check(self._L.crispy_nw_set_slice_width(self._h, cells))
+
+    def kmer_exact_pass(self, mode):
+        """K <= 6: exact counts of the pairs the tile's bound keeps in the tile
+        kernel (0 or False), in a second kernel over the tile's candidates
+        (1 or True), or chosen by the pair count (-1, the default); the
+        results are the same."""
+        self._check(self._L.crispy_kmer_set_exact_pass(self._h, int(mode)))
 
     def graphs(self, enable):
         """Kernel chains (k-mer profile and tile passes, NW DP phase, NW

@@ -132,6 +132,38 @@ __device__ __forceinline__ uint32_t repeat_min_sum(P A, int na, P B, int nb) {
   return sum;
 }
 
+// Candidate entries of the tile pass: the pair key i << kshift | j (at most
+// 40 bits) and, above it, the pair's presence popcount (at most 4096).
+constexpr int kCandPopShift = 44;
+// Pairs from which the K <= 6 exact counts run as a pass of their own (auto mode).
+constexpr int64_t kExactPassPairs = 1000000000;
+constexpr uint64_t kCandKeyMask = (1ull << 40) - 1;
+
+// repeat_min_sum over global memory with the next entry of each list loaded a
+// step ahead, so an advance does not wait for its load.
+__device__ __forceinline__ uint32_t repeat_min_sum_prefetch(const uint32_t *__restrict__ A, int na,
+                                                            const uint32_t *__restrict__ B, int nb) {
+  uint32_t sum = 0;
+  int x = 0, y = 0;
+  uint32_t ea = A[0], eb = B[0];
+  uint32_t ea1 = na > 1 ? A[1] : 0u, eb1 = nb > 1 ? B[1] : 0u;
+  while (x < na && y < nb) {
+    const uint32_t va = ea >> 16, vb = eb >> 16;
+    if (va == vb) sum += min(ea & 0xFFFFu, eb & 0xFFFFu);
+    if (va <= vb) {
+      ++x;
+      ea = ea1;
+      ea1 = x + 1 < na ? A[x + 1] : 0u;
+    }
+    if (va >= vb) {
+      ++y;
+      eb = eb1;
+      eb1 = y + 1 < nb ? B[y + 1] : 0u;
+    }
+  }
+  return sum;
+}
+
 // kmerMain.cpp:180 / kmerCPU.cpp:45
 __device__ __forceinline__ bool passes(float dist, float thr) {
   const float diff = dist - thr;
@@ -480,7 +512,8 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
         if (CAND || (q >> 31)) {
           if (pos < cap) {
             const int li = (int)(q & 127u), lj = kTile + (int)((q >> 7) & 127u);
-            out_keys[pos] = ((uint64_t)(row0 + li) << kshift) | (uint64_t)(col0 + lj - kTile);
+            out_keys[pos] = ((uint64_t)(row0 + li) << kshift) | (uint64_t)(col0 + lj - kTile) |
+                            (CAND ? (uint64_t)((q >> 14) & 0x1FFFu) << kCandPopShift : 0ull);
             if (!CAND)
               out_dists[pos] = 1.0f - (float)((q >> 14) & 0x1FFFu) / (float)min(sT[li], sT[lj]);
           }
@@ -594,7 +627,7 @@ __global__ __launch_bounds__(256) void kmer_cand_kernel(
     float dist = 0.f;
     uint64_t key = 0;
     if (c < ncand) {
-      key = cand[c];
+      key = cand[c] & kCandKeyMask;
       const int i = (int)(key >> kshift), j = (int)(key & ((1ull << kshift) - 1));
       // the lists start at multiples of 4 codes: read them 4 at a time, the
       // next word already in flight (the buffer has 8 codes of slack at the end)
@@ -619,6 +652,50 @@ __global__ __launch_bounds__(256) void kmer_cand_kernel(
       }
       const unsigned short mn = l1 < l2 ? l1 : l2;
       dist = 1.0f - (float)matches / (float)mn;
+      pass = passes(dist, thr);
+    }
+    const uint64_t ball = __ballot(pass);
+    const int total = __popcll(ball);
+    unsigned long long basep = 0;
+    if (total > 0) {
+      if (lane == 0) basep = atomicAdd(counter, (unsigned long long)total);
+      basep = __shfl(basep, 0, 64);
+    }
+    if (pass) {
+      const int64_t pos = (int64_t)basep + __popcll(ball & ((1ull << lane) - 1));
+      if (pos < cap) {
+        out_keys[pos] = key;
+        out_dists[pos] = dist;
+      }
+    }
+  }
+}
+
+// Exact count of the K <= 6 candidates that the tile pass wrote out (pair key
+// plus its presence popcount in the high bits): matches = popcount + the
+// repeat-list merge (the same sum as in the tile's own survivor pass), the
+// reference distance and predicate, the passing pairs appended.
+__global__ __launch_bounds__(256) void kmer_exact_kernel(
+    const uint64_t *__restrict__ cand, int64_t ncand, const int32_t *__restrict__ Tn,
+    const int32_t *__restrict__ rcnt, const int64_t *__restrict__ roffc, const uint32_t *__restrict__ rlistc,
+    float thr, int kshift, unsigned long long *__restrict__ counter, uint64_t *__restrict__ out_keys,
+    float *__restrict__ out_dists, int64_t cap) {
+  const int lane = threadIdx.x & 63;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t base0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63);
+  for (int64_t b = base0; b < ncand; b += stride) {
+    const int64_t c = b + lane;
+    bool pass = false;
+    float dist = 0.f;
+    uint64_t key = 0;
+    if (c < ncand) {
+      const uint64_t e = cand[c];
+      key = e & kCandKeyMask;
+      const int i = (int)(key >> kshift), j = (int)(key & ((1ull << kshift) - 1));
+      uint32_t m = (uint32_t)(e >> kCandPopShift);
+      const int na = rcnt[i], nb = rcnt[j];
+      if (na > 0 && nb > 0) m += repeat_min_sum_prefetch(rlistc + roffc[i], na, rlistc + roffc[j], nb);
+      dist = 1.0f - (float)m / (float)min(Tn[i], Tn[j]);
       pass = passes(dist, thr);
     }
     const uint64_t ball = __ballot(pass);
@@ -731,6 +808,13 @@ static const TileFn kTiles[7] = {nullptr,
                                  launch_tiles<4, false>,
                                  launch_tiles<5, false>,
                                  launch_tiles<6, false>};
+static const TileFn kTilesCand[7] = {nullptr,
+                                     launch_tiles<1, true>,
+                                     launch_tiles<2, true>,
+                                     launch_tiles<3, true>,
+                                     launch_tiles<4, true>,
+                                     launch_tiles<5, true>,
+                                     launch_tiles<6, true>};
 
 // Profiles of the bitmap word size KP (KP = K for K <= 6, 6 above); every
 // read must hold at least one K-mer.
@@ -862,6 +946,13 @@ extern "C" {
 
 int64_t crispy_kmer_num_tiles(int32_t n) { return num_slots((n + kTile - 1) / kTile); }
 
+int crispy_kmer_set_exact_pass(crispy_ctx *ctx, int mode) {
+  if (!ctx) return CRISPY_ERR_ARG;
+  if (mode < -1 || mode > 1) return set_err(ctx, CRISPY_ERR_ARG, "exact pass mode %d: -1, 0 or 1", mode);
+  ctx->km.exact_pass = mode;
+  return CRISPY_OK;
+}
+
 int crispy_kmer_run(crispy_ctx *ctx, int K, float threshold, int64_t tile_begin, int64_t tile_end,
                     int64_t *count) {
   if (!ctx) return CRISPY_ERR_ARG;
@@ -897,10 +988,39 @@ int crispy_kmer_run(crispy_ctx *ctx, int K, float threshold, int64_t tile_begin,
   const int64_t pairs_in_range = std::min<int64_t>(nt * kTile * kTile, (int64_t)n * (n - 1) / 2);
   const int64_t guess = std::min<int64_t>((int64_t)1 << 22, std::max<int64_t>(4096, pairs_in_range));
   km.cap = std::max(km.cap, guess);
-  if (K <= 6) {
+  // K <= 6: exact counts in the tile, or in a pass of their own over the
+  // tile's candidates.  Measured on MI355X: the lean candidate tile plus the
+  // pass wins from ~10^9 pairs up (reads100k 9.8 -> 9.1 ms, reads250k 50.2
+  // -> 44.0 ms per k-mer run); below, the pass's latency-bound merges cost more
+  // than the tile saves (reads20k 0.78 -> 0.97 ms).
+  const bool separate = km.exact_pass == 1 || (km.exact_pass == -1 && (int64_t)n * (n - 1) / 2 >= kExactPassPairs);
+  if (K <= 6 && !separate) {
     CRISPY_TRY(run_counted(ctx, km.cap, km.keys, &km.dists, &km.count, [&]() {
       return kTiles[K](ctx, n, nb, tile_begin, nt, threshold, 0, km.keys.as<uint64_t>(),
                        km.dists.as<float>(), km.cap);
+    }));
+  } else if (K <= 6) {  // the tile pass writes candidates, a second pass counts them exactly
+    km.cand_cap = std::max(km.cand_cap, guess);
+    int64_t ncand = 0;
+    CRISPY_TRY(run_counted(ctx, km.cand_cap, km.cand, nullptr, &ncand, [&]() {
+      return kTilesCand[K](ctx, n, nb, tile_begin, nt, threshold, 0, km.cand.as<uint64_t>(), nullptr,
+                           km.cand_cap);
+    }));
+    km.candidates = ncand;
+    CRISPY_TRY(run_counted(ctx, km.cap, km.keys, &km.dists, &km.count, [&]() -> int {
+      zero_counter_kernel<<<1, 1, 0, ctx->stream>>>(km.counter.as<unsigned long long>());
+      CRISPY_LAUNCHED(ctx);
+      if (ncand == 0) return CRISPY_OK;
+      hipEvent_t ev;
+      timer_begin(ctx, "kmer_cand", &ev);
+      const int blocks = (int)std::min<int64_t>((ncand + 255) / 256, 8192);
+      kmer_exact_kernel<<<blocks, 256, 0, ctx->stream>>>(
+          km.cand.as<uint64_t>(), ncand, km.T.as<int32_t>(), km.rcnt.as<int32_t>(), km.roffc.as<int64_t>(),
+          km.rlistc.as<uint32_t>(), threshold, km.kshift, km.counter.as<unsigned long long>(),
+          km.keys.as<uint64_t>(), km.dists.as<float>(), km.cap);
+      CRISPY_LAUNCHED(ctx);
+      timer_end(ctx, "kmer_cand", ev, (double)ncand);
+      return CRISPY_OK;
     }));
   } else {
     km.cand_cap = std::max(km.cand_cap, guess);

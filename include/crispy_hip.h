@@ -112,6 +112,12 @@ int crispy_kmer_run(crispy_ctx *ctx, int K, float threshold, int64_t tile_begin,
                     int64_t tile_end, int64_t *count);
 /* Number of tile slots for n reads (the shard index space). */
 int64_t crispy_kmer_num_tiles(int32_t num_reads);
+/* Where K <= 6 pairs that the tile's bound cannot reject get their exact
+ * count: 0 = in the tile kernel itself; 1 = the tile writes them out as
+ * candidates and a second kernel counts them; -1 (the default) = the second
+ * form from 10^9 read pairs up, the first below.  Results do not depend on
+ * it.  No counterpart in the reference: a tuning and test switch. */
+int crispy_kmer_set_exact_pass(crispy_ctx *ctx, int mode);
 /* Copies result entries [first, first+count) to host: pairs as int32 (i, j),
  * dists as float32 (either pointer may be NULL). */
 int crispy_kmer_fetch(crispy_ctx *ctx, int64_t first, int64_t count, int32_t *pairs,

@@ -154,6 +154,30 @@ def test_kmer_random_vs_oracle(ctx, K):
         assert np.array_equal(bits(d), bits(od))
 
 
+@pytest.mark.parametrize("K", range(1, 7))
+def test_kmer_exact_pass_forms(ctx, K):
+    """K <= 6: the exact counts in the tile kernel and in a pass of their own
+    over the tile's candidates give the oracle's pairs and distances, at a
+    threshold where most candidates pass and at one where most fail."""
+    say("kmer exact pass forms K", K)
+    rng = np.random.default_rng(3000 + K)
+    seqs = random_reads(rng, 300, max(K, 8), 300, repeats=True)
+    r = ca.Reads.from_list(seqs)
+    ro = O.Reads.from_list(seqs)
+    ctx.upload_reads(ca.KMER, r)
+    try:
+        for thr in (0.25, 0.6):
+            op, od = O.kmer_all(ro, K, thr, nthreads=1)
+            for mode in (0, 1):
+                ctx.kmer_exact_pass(mode)
+                ctx.kmer_run(K, thr)
+                p, d = ctx.kmer_fetch()
+                assert np.array_equal(p, op), (thr, mode)
+                assert np.array_equal(bits(d), bits(od)), (thr, mode)
+    finally:
+        ctx.kmer_exact_pass(-1)
+
+
 def test_kmer_tile_ranges_partition(ctx):
     say("kmer tile ranges")
     seqs, _, _ = synth.make("reads1k")
