@@ -139,31 +139,6 @@ constexpr int kCandPopShift = 44;
 constexpr int64_t kExactPassPairs = 1000000000;
 constexpr uint64_t kCandKeyMask = (1ull << 40) - 1;
 
-// repeat_min_sum over global memory with the next entry of each list loaded a
-// step ahead, so an advance does not wait for its load.
-__device__ __forceinline__ uint32_t repeat_min_sum_prefetch(const uint32_t *__restrict__ A, int na,
-                                                            const uint32_t *__restrict__ B, int nb) {
-  uint32_t sum = 0;
-  int x = 0, y = 0;
-  uint32_t ea = A[0], eb = B[0];
-  uint32_t ea1 = na > 1 ? A[1] : 0u, eb1 = nb > 1 ? B[1] : 0u;
-  while (x < na && y < nb) {
-    const uint32_t va = ea >> 16, vb = eb >> 16;
-    if (va == vb) sum += min(ea & 0xFFFFu, eb & 0xFFFFu);
-    if (va <= vb) {
-      ++x;
-      ea = ea1;
-      ea1 = x + 1 < na ? A[x + 1] : 0u;
-    }
-    if (va >= vb) {
-      ++y;
-      eb = eb1;
-      eb1 = y + 1 < nb ? B[y + 1] : 0u;
-    }
-  }
-  return sum;
-}
-
 // kmerMain.cpp:180 / kmerCPU.cpp:45
 __device__ __forceinline__ bool passes(float dist, float thr) {
   const float diff = dist - thr;
@@ -694,7 +669,7 @@ __global__ __launch_bounds__(256) void kmer_exact_kernel(
       const int i = (int)(key >> kshift), j = (int)(key & ((1ull << kshift) - 1));
       uint32_t m = (uint32_t)(e >> kCandPopShift);
       const int na = rcnt[i], nb = rcnt[j];
-      if (na > 0 && nb > 0) m += repeat_min_sum_prefetch(rlistc + roffc[i], na, rlistc + roffc[j], nb);
+      if (na > 0 && nb > 0) m += repeat_min_sum(rlistc + roffc[i], na, rlistc + roffc[j], nb);
       dist = 1.0f - (float)m / (float)min(Tn[i], Tn[j]);
       pass = passes(dist, thr);
     }
