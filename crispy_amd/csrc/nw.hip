@@ -433,7 +433,10 @@ static int nw_core(crispy_ctx *ctx, const int2 *d_pairs, int64_t np, int band) {
   CRISPY_TRY(ensure(ctx, nw.hist, sizeof(unsigned int) * kNumBuckets));
   CRISPY_TRY(ensure(ctx, nw.cells, sizeof(unsigned long long) * kNumBuckets));
   CRISPY_TRY(ensure(ctx, nw.plan, sizeof(NwPlan)));
-  if (ctx->timing) CRISPY_TRY(ensure(ctx, nw.stat, sizeof(unsigned long long) * (kNumFamilies + 1)));
+  if (ctx->timing && !nw.stat.p) {  // the cell counters start at zero (timing_reset clears them later)
+    CRISPY_TRY(ensure(ctx, nw.stat, sizeof(unsigned long long) * (kNumFamilies + 1)));
+    CRISPY_HIP(ctx, hipMemsetAsync(nw.stat.p, 0, nw.stat.bytes, ctx->stream));
+  }
   if (np == 0) return CRISPY_OK;
   uint32_t *order = nw.order.as<uint32_t>();
   float *out = nw.dist.as<float>();
