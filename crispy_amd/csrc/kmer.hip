@@ -135,6 +135,10 @@ __device__ __forceinline__ uint32_t repeat_min_sum(P A, int na, P B, int nb) {
 // Candidate entries of the tile pass: the pair key i << kshift | j (at most
 // 40 bits) and, above it, the pair's presence popcount (at most 4096).
 constexpr int kCandPopShift = 44;
+// Codes per list in a lane's merge window of the K = 7, 8 candidate kernel,
+// and the codes past the last K-mer list that a window may read.
+constexpr int kCandWin = 16;
+constexpr int kListSlack = 32;
 // Pairs from which the K <= 6 exact counts run as a pass of their own (auto mode).
 constexpr int64_t kExactPassPairs = 1000000000;
 constexpr uint64_t kCandKeyMask = (1ull << 40) - 1;
@@ -592,39 +596,62 @@ __global__ __launch_bounds__(256) void kmer_cand_kernel(
     const int64_t *__restrict__ loff, int K, const int32_t *__restrict__ len, float thr, int kshift,
     unsigned long long *__restrict__ counter, uint64_t *__restrict__ out_keys,
     float *__restrict__ out_dists, int64_t cap) {
-  const int lane = threadIdx.x & 63;
+  // Each lane merges its candidate's two lists through a window of kCandWin
+  // codes of each, staged in LDS; the lanes of a wave refill their windows in
+  // the same round, so the wave waits for memory once per round (every ~13+
+  // codes) instead of whenever one of its lanes crosses a word.
+  __shared__ __align__(16) uint16_t win[256][2][kCandWin];
+  const int t = threadIdx.x, lane = t & 63;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   // every lane of a wave runs the same number of outer iterations (shuffles)
-  const int64_t base0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63);
+  const int64_t base0 = (int64_t)blockIdx.x * blockDim.x + (t & ~63);
   for (int64_t b = base0; b < ncand; b += stride) {
     const int64_t c = b + lane;
     bool pass = false;
     float dist = 0.f;
     uint64_t key = 0;
+    const uint16_t *La = lists, *Lb = lists;
+    unsigned short l1 = 0, l2 = 0;
     if (c < ncand) {
       key = cand[c] & kCandKeyMask;
       const int i = (int)(key >> kshift), j = (int)(key & ((1ull << kshift) - 1));
-      // the lists start at multiples of 4 codes: read them 4 at a time, the
-      // next word already in flight (the buffer has 8 codes of slack at the end)
-      const uint64_t *A = reinterpret_cast<const uint64_t *>(lists + loff[i]);
-      const uint64_t *Bl = reinterpret_cast<const uint64_t *>(lists + loff[j]);
-      const unsigned short l1 = (unsigned short)(len[i] - K + 1), l2 = (unsigned short)(len[j] - K + 1);
-      uint64_t a0 = A[0], a1 = A[1], b0 = Bl[0], b1 = Bl[1];
-      unsigned short matches = 0;
-      int k = 0, l = 0;
-      while (k < l1 && l < l2) {
-        const uint32_t x = (uint32_t)(a0 >> (16 * (k & 3))) & 0xFFFFu;
-        const uint32_t y = (uint32_t)(b0 >> (16 * (l & 3))) & 0xFFFFu;
-        matches = (unsigned short)(matches + (x == y));
-        if (x <= y && (++k & 3) == 0) {
-          a0 = a1;
-          a1 = A[(k >> 2) + 1];
+      La = lists + loff[i];
+      Lb = lists + loff[j];
+      l1 = (unsigned short)(len[i] - K + 1);
+      l2 = (unsigned short)(len[j] - K + 1);
+    }
+    unsigned short matches = 0;
+    int k = 0, l = 0;
+    bool active = c < ncand && l1 > 0 && l2 > 0;
+    while (__ballot(active)) {
+      if (active) {
+        // the lists start at multiples of 4 codes: windows start on a word
+        // (4 codes) and may read up to kListSlack codes past the last list
+        const int ka = k & ~3, lb = l & ~3;
+        const uint64_t *A = reinterpret_cast<const uint64_t *>(La + ka);
+        const uint64_t *B = reinterpret_cast<const uint64_t *>(Lb + lb);
+        uint64_t wa[kCandWin / 4], wb[kCandWin / 4];
+#pragma unroll
+        for (int q = 0; q < kCandWin / 4; ++q) {
+          wa[q] = A[q];
+          wb[q] = B[q];
         }
-        if (x >= y && (++l & 3) == 0) {
-          b0 = b1;
-          b1 = Bl[(l >> 2) + 1];
+#pragma unroll
+        for (int q = 0; q < kCandWin / 4; ++q) {
+          reinterpret_cast<uint64_t *>(win[t][0])[q] = wa[q];
+          reinterpret_cast<uint64_t *>(win[t][1])[q] = wb[q];
         }
+        const int ea = min((int)l1, ka + kCandWin), eb = min((int)l2, lb + kCandWin);
+        while (k < ea && l < eb) {
+          const uint32_t x = win[t][0][k - ka], y = win[t][1][l - lb];
+          matches = (unsigned short)(matches + (x == y));
+          k += x <= y;
+          l += x >= y;
+        }
+        active = k < l1 && l < l2;
       }
+    }
+    if (c < ncand) {
       const unsigned short mn = l1 < l2 ? l1 : l2;
       dist = 1.0f - (float)matches / (float)mn;
       pass = passes(dist, thr);
@@ -823,7 +850,7 @@ static int build_profiles(crispy_ctx *ctx, int KP, int K) {
     loff[0] = 0;
     for (int32_t r = 0; r < n; ++r) loff[r + 1] = loff[r] + ((rs.h_len[r] - K + 1 + 3) & ~3);  // 8-byte aligned
     CRISPY_TRY(ensure(ctx, km.loff, offs));
-    CRISPY_TRY(ensure(ctx, km.lists, sizeof(uint16_t) * (loff[n] + 8)));
+    CRISPY_TRY(ensure(ctx, km.lists, sizeof(uint16_t) * (loff[n] + kListSlack)));
     CRISPY_HIP(ctx, hipMemcpyAsync(km.loff.p, loff, offs, hipMemcpyHostToDevice, ctx->stream));
   }
   size_t tbytes = 0;
