@@ -371,7 +371,7 @@ __global__ __launch_bounds__(1024) void nw_plan_kernel(const unsigned int *__res
 // m * 2^e with m < 16 (similar pair counts launch alike and share a captured
 // graph; at most 1/8 of the blocks are spare).  0 if no kB of the range
 // belongs to the family.
-static unsigned family_grid(int f, int64_t np, int kb_lo, int kb_hi, int forced) {
+static int64_t family_grid(int f, int64_t np, int kb_lo, int kb_hi, int forced) {
   int64_t per = 0, buckets = 0;
   for (int kb = std::max(kb_lo, 0); kb <= std::min(kb_hi, kKbSliceMax); ++kb) {
     if (nw_family_of(kb, forced) != f) continue;
@@ -383,8 +383,7 @@ static unsigned family_grid(int f, int64_t np, int kb_lo, int kb_hi, int forced)
   int64_t g = (np + per - 1) / per + buckets;
   int e = 0;
   while ((g >> e) >= 16) ++e;
-  g = (((g - 1) >> e) + 1) << e;
-  return (unsigned)std::min<int64_t>(g, 0x7FFFFFFF);
+  return (((g - 1) >> e) + 1) << e;
 }
 
 // The DP phase's launches after the bucket sort: the plan, then every family
@@ -466,7 +465,13 @@ static int nw_core(crispy_ctx *ctx, const int2 *d_pairs, int64_t np, int band) {
   const NwFusedArgs args{rs.pack.as<uint32_t>(), rs.poff.as<int64_t>(), rs.len.as<int32_t>(),
                          d_pairs, order, plan, out};
   unsigned grid[kNumFamilies];
-  for (int f = 0; f < kNumFamilies; ++f) grid[f] = family_grid(f, np, kb_lo, kb_hi, nw.slice_width);
+  for (int f = 0; f < kNumFamilies; ++f) {
+    const int64_t g = family_grid(f, np, kb_lo, kb_hi, nw.slice_width);
+    if (g > 0x7FFFFFFF)
+      return set_err(ctx, CRISPY_ERR_ARG, "%lld pairs need %lld wave tasks in one launch; split the call",
+                     (long long)np, (long long)g);
+    grid[f] = (unsigned)g;
+  }
   std::vector<uintptr_t> key = {
       (uintptr_t)args.pack, (uintptr_t)args.poff, (uintptr_t)args.len, (uintptr_t)args.pairs,
       (uintptr_t)args.order, (uintptr_t)args.plan, (uintptr_t)args.out, (uintptr_t)nw.hist.p,
