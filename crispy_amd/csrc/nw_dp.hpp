@@ -406,20 +406,6 @@ __device__ __forceinline__ void nw_sliced_wave(const uint32_t *__restrict__ pack
           bJLA = capLA;
         }
       }
-      if (i == L2) {
-#pragma unroll
-        for (int k = 0; k < C; ++k) {
-          const int j = i - KB + base_d + k;
-          if (j >= 1 && j <= L1 && base_d + k < W) {
-            const int Mr = Mt[k] - 4 * i;
-            if (Mr >= bIM) {
-              bIM = Mr;
-              bIj = j;
-              bILA = LA[k];
-            }
-          }
-        }
-      }
     }
     // this lane's last cell of row i is lane g+1's "up" at the next step
     // (a DPP wave shift: lane l reads lane l-1; the first lane of a group ignores it)
@@ -444,6 +430,22 @@ __device__ __forceinline__ void nw_sliced_wave(const uint32_t *__restrict__ pack
   for (; t <= top_end; ++t) step(t, std::integral_constant<int, kRowTop>{});
   for (; t <= mid_end; ++t) step(t, std::integral_constant<int, kRowFull>{});
   for (; t <= tmax; ++t) step(t, std::integral_constant<int, kRowCapture>{});
+  // Last row: a lane computes rows 1..R only, so when R = L2 its registers
+  // still hold row L2 (when R < L2, row L2 has no in-band cell).
+  if (has_pair && R == L2) {
+#pragma unroll
+    for (int k = 0; k < C; ++k) {
+      const int j = L2 - KB + base_d + k;
+      if (j >= 1 && j <= L1 && base_d + k < W) {
+        const int Mr = Mt[k] - 4 * L2;
+        if (Mr >= bIM) {
+          bIM = Mr;
+          bIj = j;
+          bILA = LA[k];
+        }
+      }
+    }
+  }
 
   red[lane][0] = bIM;
   red[lane][1] = bIj;
