@@ -172,26 +172,14 @@ __device__ __forceinline__ void nw_reg_wave(const uint32_t *__restrict__ pack,
     const int dl = L1 - i + KB;  // position of column L1 in this row
     int capM = kNeg;
     uint32_t capLA = 0;
-    nw_row<KB, MODE>(i, Mt, Mv, LA, m, dl, capM, capLA);
+    // rows past the pair's last in-band row R leave its registers alone, so
+    // they still hold row R when the loop ends
+    if (i <= R) nw_row<KB, MODE>(i, Mt, Mv, LA, m, dl, capM, capLA);
     if (MODE != kRowFull && dl >= 0 && dl < W && i <= L2) {
       const int Mr = capM - 4 * i;
       if (Mr >= bestJ) {
         bestJ = Mr;
         bestJLA = capLA;
-      }
-    }
-    if (i >= L2min && i == L2) {
-#pragma unroll
-      for (int d = 0; d < W; ++d) {
-        const int j = i - KB + d;
-        if (j >= 1 && j <= L1) {
-          const int Mr = Mt[d] - 4 * i;
-          if (Mr >= bestI) {
-            bestI = Mr;
-            bestILA = LA[d];
-            bestIj = j;
-          }
-        }
       }
     }
     // slide the window by one base
@@ -214,6 +202,22 @@ __device__ __forceinline__ void nw_reg_wave(const uint32_t *__restrict__ pack,
   for (; i <= top_end; ++i) step(i, std::integral_constant<int, kRowTop>{});
   for (; i <= mid_end; ++i) step(i, std::integral_constant<int, kRowFull>{});
   for (; i <= Rmax; ++i) step(i, std::integral_constant<int, kRowCapture>{});
+  // Last row: when R = L2 the registers hold row L2 (when R < L2, row L2 has
+  // no in-band cell).
+  if (R == L2) {
+#pragma unroll
+    for (int d = 0; d < W; ++d) {
+      const int j = L2 - KB + d;
+      if (j >= 1 && j <= L1) {
+        const int Mr = Mt[d] - 4 * L2;
+        if (Mr >= bestI) {
+          bestI = Mr;
+          bestILA = LA[d];
+          bestIj = j;
+        }
+      }
+    }
+  }
   uint32_t la;
   int j;
   if (bestI > bestJ) {
