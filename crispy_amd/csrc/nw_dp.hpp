@@ -121,8 +121,8 @@ __device__ __forceinline__ void nw_reg_wave(const uint32_t *__restrict__ pack,
   // Rows past L1 + kB hold no in-band cell: the last column is out of reach
   // and the last row, if that low, is out of band.  So a pair's DP ends at
   // row R = min(L2, L1 + kB).  Wave-uniform bounds (pairs are sorted by
-  // (kB, R, L1), so these are close); lanes past their own R compute rows
-  // that no capture reads.
+  // (kB, R), so these are close); lanes past their own R sit out the rows
+  // that the others still compute.
   const int R = min(L2, L1 + KB);
   int l1m = L1, l2lo = L2, l2hi = R;
 #pragma unroll
