@@ -124,15 +124,13 @@ __device__ __forceinline__ void nw_reg_wave(const uint32_t *__restrict__ pack,
   // (kB, R), so these are close); lanes past their own R sit out the rows
   // that the others still compute.
   const int R = min(L2, L1 + KB);
-  int l1m = L1, l2lo = L2, l2hi = R;
+  int l1m = L1, l2hi = R;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     l1m = min(l1m, __shfl_xor(l1m, off, 64));
-    l2lo = min(l2lo, __shfl_xor(l2lo, off, 64));
     l2hi = max(l2hi, __shfl_xor(l2hi, off, 64));
   }
   const int L1min = __builtin_amdgcn_readfirstlane(l1m);
-  const int L2min = __builtin_amdgcn_readfirstlane(l2lo);
   const int Rmax = __builtin_amdgcn_readfirstlane(l2hi);
 
   int Mt[W], Mv[W];
