@@ -20,7 +20,12 @@ namespace crispy {
 //   Mv = Mt + 4 + pen_v(U)    (value seen as "left" by row i+1)
 //   LA = (N << 16) | F        N = AL - ML, F = AL - j: a diagonal step adds
 //                              match << 16, up adds nothing, left adds 1.
-enum { kRowFull = 0, kRowCapture = 1, kRowTop = 2 };
+// Row variants: full (no end-cell work), capture (column L1 may be in band:
+// record its cell), top (rows 1..kB: the boundary cell (i, 0) is in band),
+// top with capture (both; only for reads shorter than about 2 kB).
+enum { kRowFull = 0, kRowCapture = 1, kRowTop = 2, kRowTopCap = 3 };
+constexpr bool row_top(int mode) { return mode == kRowTop || mode == kRowTopCap; }
+constexpr bool row_captures(int mode) { return mode == kRowCapture || mode == kRowTopCap; }
 
 template <int KB, int MODE>
 __device__ __forceinline__ void nw_row(const int i, int (&Mt)[2 * KB + 1], int (&Mv)[2 * KB + 1],
@@ -32,7 +37,7 @@ __device__ __forceinline__ void nw_row(const int i, int (&Mt)[2 * KB + 1], int (
   uint32_t LAu = 0u;  // its LA
 #pragma unroll
   for (int d = 0; d < W; ++d) {
-    if (MODE == kRowTop) {
+    if (row_top(MODE)) {
       if (d < KB - i) continue;  // column j = i-kB+d < 0: not part of the matrix
       if (d == KB - i) {         // boundary cell (i, 0): M 0, ML 0, AL i
         Mt[d] = 4 * i;
@@ -68,7 +73,7 @@ __device__ __forceinline__ void nw_row(const int i, int (&Mt)[2 * KB + 1], int (
     // (SGPR pairs, spilled to VGPR lanes).
     asm volatile("" : "+v"(Mv[d]));
     LA[d] = la;
-    if (MODE != kRowFull) {
+    if (row_captures(MODE)) {
       const bool c = d == dl;
       capM = c ? M : capM;
       capLA = c ? la : capLA;
@@ -173,7 +178,7 @@ __device__ __forceinline__ void nw_reg_wave(const uint32_t *__restrict__ pack,
     // rows past the pair's last in-band row R leave its registers alone, so
     // they still hold row R when the loop ends
     if (i <= R) nw_row<KB, MODE>(i, Mt, Mv, LA, m, dl, capM, capLA);
-    if (MODE != kRowFull && dl >= 0 && dl < W && i <= L2) {
+    if (row_captures(MODE) && dl >= 0 && dl < W && i <= L2) {
       const int Mr = capM - 4 * i;
       if (Mr >= bestJ) {
         bestJ = Mr;
@@ -191,13 +196,15 @@ __device__ __forceinline__ void nw_reg_wave(const uint32_t *__restrict__ pack,
       fcnt = 16;
     }
   };
-  // Rows 1..kB hold the boundary column; rows below L1min-kB cannot reach
-  // column L1 of any lane.  Three loops, one row variant each, so the
-  // state keeps one register assignment per loop.
+  // Rows 1..kB hold the boundary column; rows above L1min-kB cannot reach
+  // column L1 of any lane.  One loop per row variant, so the state keeps one
+  // register assignment per loop.
   const int top_end = min(KB, Rmax);
+  const int top_plain_end = min(top_end, L1min - KB - 1);
   const int mid_end = min(Rmax, max(top_end, L1min - KB - 1));
   int i = 1;
-  for (; i <= top_end; ++i) step(i, std::integral_constant<int, kRowTop>{});
+  for (; i <= top_plain_end; ++i) step(i, std::integral_constant<int, kRowTop>{});
+  for (; i <= top_end; ++i) step(i, std::integral_constant<int, kRowTopCap>{});
   for (; i <= mid_end; ++i) step(i, std::integral_constant<int, kRowFull>{});
   for (; i <= Rmax; ++i) step(i, std::integral_constant<int, kRowCapture>{});
   // Last row: when R = L2 the registers hold row L2 (when R < L2, row L2 has
@@ -262,7 +269,7 @@ __device__ __forceinline__ void nw_slice_cell(const int k, const int i, const ui
   const uint32_t pk = cd ? kPenDiag : (cu ? kPenUp : kPenLeft);
   int mh = M + (int)(int16_t)(pk & 0xFFFFu);
   int mv = M + ((int)pk >> 16);
-  if (MODE == kRowTop) {  // boundary cell (i, 0): M 0, ML 0, AL i
+  if (row_top(MODE)) {  // boundary cell (i, 0): M 0, ML 0, AL i
     const bool b = k == k0;
     M = b ? 4 * i : M;
     mv = b ? 4 * i + 4 : mv;
@@ -275,7 +282,7 @@ __device__ __forceinline__ void nw_slice_cell(const int k, const int i, const ui
   LA_k = la;
   Mh = mh;
   LAu = la;
-  if (MODE != kRowFull) {
+  if (row_captures(MODE)) {
     const bool c = k == kc;
     capM = c ? M : capM;
     capLA = c ? la : capLA;
@@ -400,7 +407,7 @@ __device__ __forceinline__ void nw_sliced_wave(const uint32_t *__restrict__ pack
         nw_slice_cell<KEND, MODE>(k, i, s, Mt[k], Mv[k], LA[k], lM, lLA, Mh, LAu, k0, kc, capM,
                                   capLA);
       }
-      if (MODE != kRowFull && kc >= 0 && kc < C && i >= L1 - KB) {
+      if (row_captures(MODE) && kc >= 0 && kc < C && i >= L1 - KB) {
         const int Mr = capM - 4 * i;
         if (Mr >= bJM) {
           bJM = Mr;
@@ -427,9 +434,11 @@ __device__ __forceinline__ void nw_sliced_wave(const uint32_t *__restrict__ pack
     }
   };
   const int top_end = min(KB + G - 1, tmax);
+  const int top_plain_end = min(top_end, L1min - KB - 1);
   const int mid_end = max(top_end, min(tmax, L1min - KB - 1));
   int t = 1;
-  for (; t <= top_end; ++t) step(t, std::integral_constant<int, kRowTop>{});
+  for (; t <= top_plain_end; ++t) step(t, std::integral_constant<int, kRowTop>{});
+  for (; t <= top_end; ++t) step(t, std::integral_constant<int, kRowTopCap>{});
   for (; t <= mid_end; ++t) step(t, std::integral_constant<int, kRowFull>{});
   for (; t <= tmax; ++t) step(t, std::integral_constant<int, kRowCapture>{});
   // Last row: a lane computes rows 1..R only, so when R = L2 its registers
