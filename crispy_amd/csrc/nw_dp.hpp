@@ -433,9 +433,13 @@ __device__ __forceinline__ void nw_sliced_wave(const uint32_t *__restrict__ pack
       fcnt = 16;
     }
   };
-  const int top_end = min(KB + G - 1, tmax);
-  const int top_plain_end = min(top_end, L1min - KB - 1);
-  const int mid_end = max(top_end, min(tmax, L1min - KB - 1));
+  // Lane g holds the boundary cell (i, 0) at k0 = kB - t - g (C - 1), so only
+  // steps t <= kB have one; its first in-band cell of column L1 (i >= L1 - kB
+  // inside its slice) comes at step L1 - kB + G - 1 at the earliest (lane G-1).
+  const int top_end = min(KB, tmax);
+  const int cap_from = L1min - KB + G - 1;
+  const int top_plain_end = min(top_end, cap_from - 1);
+  const int mid_end = max(top_end, min(tmax, cap_from - 1));
   int t = 1;
   for (; t <= top_plain_end; ++t) step(t, std::integral_constant<int, kRowTop>{});
   for (; t <= top_end; ++t) step(t, std::integral_constant<int, kRowTopCap>{});
