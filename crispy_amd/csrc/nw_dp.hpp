@@ -266,16 +266,16 @@ __device__ __forceinline__ void nw_slice_cell(const int k, const int i, const ui
   constexpr uint32_t kPenDiag = ((uint32_t)(4 + kGO) << 16) | (uint16_t)kGO;
   constexpr uint32_t kPenUp = ((uint32_t)4 << 16) | (uint16_t)kGE;
   constexpr uint32_t kPenLeft = ((uint32_t)(4 + kGE) << 16) | 0u;
-  const uint32_t pk = cd ? kPenDiag : (cu ? kPenUp : kPenLeft);
-  int mh = M + (int)(int16_t)(pk & 0xFFFFu);
-  int mv = M + ((int)pk >> 16);
-  if (row_top(MODE)) {  // boundary cell (i, 0): M 0, ML 0, AL i
+  uint32_t pk = cd ? kPenDiag : (cu ? kPenUp : kPenLeft);
+  if (row_top(MODE)) {  // boundary cell (i, 0): M 0, ML 0, AL i; pen_h = pen_v = 0
+    constexpr uint32_t kPenBoundary = (uint32_t)4 << 16;
     const bool b = k == k0;
     M = b ? 4 * i : M;
-    mv = b ? 4 * i + 4 : mv;
     la = b ? (((uint32_t)i << 16) | (uint32_t)i) : la;
-    mh = b ? 4 * i : mh;
+    pk = b ? kPenBoundary : pk;
   }
+  const int mh = M + (int)(int16_t)(pk & 0xFFFFu);
+  const int mv = M + ((int)pk >> 16);
   Mt_k = M;
   Mv_k = mv;
   asm volatile("" : "+v"(Mv_k));
