@@ -543,21 +543,102 @@ __global__ void unpack_keys_kernel(const uint64_t *__restrict__ keys, int64_t n,
 // K-mer lists sorted as signed short (kmerMain.cpp:156-159) -- its K = 8
 // ordering included -- for the exact count.
 
-// One block per read: the T = L-K+1 codes in signed-short order.
+// Ascending bitonic sort of the 64 R keys of one wave, R per lane: key
+// e = R lane + q sits in v[q] of lane `lane`.  Stages with partner distance
+// j >= R pair register q of lanes lane and lane ^ (j / R); shorter ones pair
+// registers q and q + j of one lane.
+template <int R>
+__device__ __forceinline__ void bitonic_wave(uint32_t (&v)[R], const int lane) {
+  constexpr int kLogP = (R == 8 ? 3 : 4) + 6;
+#pragma unroll
+  for (int lk = 1; lk <= kLogP; ++lk) {
+    const int k = 1 << lk;
+#pragma unroll
+    for (int lj = lk - 1; lj >= 0; --lj) {
+      const int j = 1 << lj;
+      if (j >= R) {
+        const int lx = j / R;
+        const bool lower = (lane & lx) == 0;
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+          const uint32_t a = v[q];
+          const uint32_t b = (uint32_t)__shfl_xor((int)a, lx, 64);
+          const bool up = ((R * lane + q) & k) == 0;
+          const bool keep_min = lower == up;
+          v[q] = keep_min ? (a < b ? a : b) : (a < b ? b : a);
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+          if (q & j) continue;
+          const uint32_t a = v[q], b = v[q + j];
+          const bool up = ((R * lane + q) & k) == 0;
+          const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+          v[q] = up ? lo : hi;
+          v[q + j] = up ? hi : lo;
+        }
+      }
+    }
+  }
+}
+
+// The T <= 64 R codes of one read sorted in one wave's registers: lane `lane`
+// builds codes R lane .. R lane + R - 1 from two packed words (the R + K - 1
+// bases they span start at base 8 lane or 16 lane, inside a 64-bit window).
+template <int K, int R>
+__device__ __forceinline__ void list_wave(const uint32_t *__restrict__ s, const int nw, const int T,
+                                          const int lane, uint16_t *__restrict__ dst) {
+  static_assert(R + K - 1 <= 24, "the lane's bases fit the 64-bit window past its offset");
+  const int p0 = R * lane;
+  const int w = p0 >> 4;
+  const uint32_t lo = w < nw ? s[w] : 0u;
+  const uint32_t hi = w + 1 < nw ? s[w + 1] : 0u;
+  const uint64_t win = ((uint64_t)hi << 32 | lo) >> (2 * (p0 & 15));
+  constexpr uint32_t kMask = (1u << (2 * K)) - 1u;
+  uint32_t c = 0;
+#pragma unroll
+  for (int m = 0; m < K; ++m) c = (c << 2) | (uint32_t)((win >> (2 * m)) & 3u);
+  uint32_t v[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    if (q > 0) c = ((c << 2) | (uint32_t)((win >> (2 * (q + K - 1))) & 3u)) & kMask;
+    // signed-short order of the code = unsigned order of code ^ 0x8000;
+    // the padding sorts after every code
+    v[q] = p0 + q < T ? (c ^ 0x8000u) : 0xFFFFFFFFu;
+  }
+  bitonic_wave<R>(v, lane);
+#pragma unroll
+  for (int q = 0; q < R; ++q)
+    if (p0 + q < T) dst[p0 + q] = (uint16_t)(v[q] ^ 0x8000u);
+}
+
+// One wave per read: the T = L-K+1 codes in signed-short order.  Up to 1,024
+// codes sort in registers; longer reads sort in LDS (dynamic shared memory of
+// CRISPY_MAX_READ_LEN keys, given only when some read needs it).
 template <int K>
-__global__ __launch_bounds__(256) void kmer_list_kernel(const uint32_t *__restrict__ pack,
-                                                        const int64_t *__restrict__ poff,
-                                                        const int32_t *__restrict__ len,
-                                                        const int64_t *__restrict__ loff,
-                                                        uint16_t *__restrict__ lists) {
-  __shared__ uint32_t keys[CRISPY_MAX_READ_LEN];
+__global__ __launch_bounds__(64) void kmer_list_kernel(const uint32_t *__restrict__ pack,
+                                                       const int64_t *__restrict__ poff,
+                                                       const int32_t *__restrict__ len,
+                                                       const int64_t *__restrict__ loff,
+                                                       uint16_t *__restrict__ lists) {
+  extern __shared__ uint32_t keys[];
   const int r = blockIdx.x;
   const int tid = threadIdx.x;
-  const int T = len[r] - K + 1;
+  const int L = len[r];
+  const int T = L - K + 1;
   const uint32_t *s = pack + poff[r];
+  uint16_t *dst = lists + loff[r];
+  if (T <= 512) {
+    list_wave<K, 8>(s, (L + 15) >> 4, T, tid, dst);
+    return;
+  }
+  if (T <= 1024) {
+    list_wave<K, 16>(s, (L + 15) >> 4, T, tid, dst);
+    return;
+  }
   int P = 1;
   while (P < T) P <<= 1;
-  for (int t = tid; t < P; t += 256) {
+  for (int t = tid; t < P; t += 64) {
     uint32_t c = 0xFFFFFFFFu;
     if (t < T) {
       c = 0;
@@ -566,14 +647,14 @@ __global__ __launch_bounds__(256) void kmer_list_kernel(const uint32_t *__restri
         const int p = t + q;
         c = (c << 2) | ((s[p >> 4] >> (2 * (p & 15))) & 3u);
       }
-      c ^= 0x8000u;  // signed-short order of the 16-bit code = unsigned order of code ^ 0x8000
+      c ^= 0x8000u;
     }
     keys[t] = c;
   }
   __syncthreads();
   for (int k = 2; k <= P; k <<= 1) {
     for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int t = tid; t < P; t += 256) {
+      for (int t = tid; t < P; t += 64) {
         const int o = t ^ j;
         if (o > t) {
           const uint32_t a = keys[t], b = keys[o];
@@ -587,8 +668,7 @@ __global__ __launch_bounds__(256) void kmer_list_kernel(const uint32_t *__restri
       __syncthreads();
     }
   }
-  uint16_t *dst = lists + loff[r];
-  for (int t = tid; t < T; t += 256) dst[t] = (uint16_t)(keys[t] ^ 0x8000u);
+  for (int t = tid; t < T; t += 64) dst[t] = (uint16_t)(keys[t] ^ 0x8000u);
 }
 
 __global__ __launch_bounds__(256) void kmer_cand_kernel(
@@ -859,6 +939,8 @@ static int build_profiles(crispy_ctx *ctx, int KP, int K) {
   CRISPY_TRY(ensure(ctx, ctx->temp, tbytes));
   // The profile pass (a captured graph with graphs on): profiles, the
   // repeat lists compacted in read order, and for K > 6 the K-mer lists.
+  // the K-mer list kernel sorts reads of more than 1,024 K-mers in LDS
+  const size_t list_lds = rs.maxlen - K + 1 > 1024 ? sizeof(uint32_t) * CRISPY_MAX_READ_LEN : 0;
   hipEvent_t ev;
   timer_begin(ctx, "kmer_profile", &ev);
   CRISPY_TRY(run_segment(
@@ -866,7 +948,8 @@ static int build_profiles(crispy_ctx *ctx, int KP, int K) {
       {(uintptr_t)KP, (uintptr_t)K, (uintptr_t)n, (uintptr_t)rs.pack.p, (uintptr_t)rs.poff.p, (uintptr_t)rs.len.p,
        (uintptr_t)km.bitmap.p, (uintptr_t)km.T.p, (uintptr_t)km.rcnt.p, (uintptr_t)km.roff.p, (uintptr_t)km.rlist.p,
        (uintptr_t)km.excess.p, (uintptr_t)km.roffc.p, (uintptr_t)km.rlistc.p, (uintptr_t)ctx->temp.p,
-       (uintptr_t)tbytes, (uintptr_t)(K > 6 ? km.loff.p : nullptr), (uintptr_t)(K > 6 ? km.lists.p : nullptr)},
+       (uintptr_t)tbytes, (uintptr_t)(K > 6 ? km.loff.p : nullptr), (uintptr_t)(K > 6 ? km.lists.p : nullptr),
+       (uintptr_t)list_lds},
       "k-mer profile pass", [&](bool) -> int {
         CRISPY_TRY(kProfile[KP](ctx, rs));
         size_t tb = tbytes;
@@ -877,13 +960,13 @@ static int build_profiles(crispy_ctx *ctx, int KP, int K) {
             km.rlistc.as<uint32_t>(), n);
         CRISPY_LAUNCHED(ctx);
         if (K == 7)
-          kmer_list_kernel<7><<<n, 256, 0, ctx->stream>>>(rs.pack.as<uint32_t>(), rs.poff.as<int64_t>(),
-                                                         rs.len.as<int32_t>(), km.loff.as<int64_t>(),
-                                                         km.lists.as<uint16_t>());
+          kmer_list_kernel<7><<<n, 64, list_lds, ctx->stream>>>(rs.pack.as<uint32_t>(), rs.poff.as<int64_t>(),
+                                                               rs.len.as<int32_t>(), km.loff.as<int64_t>(),
+                                                               km.lists.as<uint16_t>());
         else if (K == 8)
-          kmer_list_kernel<8><<<n, 256, 0, ctx->stream>>>(rs.pack.as<uint32_t>(), rs.poff.as<int64_t>(),
-                                                         rs.len.as<int32_t>(), km.loff.as<int64_t>(),
-                                                         km.lists.as<uint16_t>());
+          kmer_list_kernel<8><<<n, 64, list_lds, ctx->stream>>>(rs.pack.as<uint32_t>(), rs.poff.as<int64_t>(),
+                                                               rs.len.as<int32_t>(), km.loff.as<int64_t>(),
+                                                               km.lists.as<uint16_t>());
         CRISPY_LAUNCHED(ctx);
         return CRISPY_OK;
       }));

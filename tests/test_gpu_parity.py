@@ -154,6 +154,40 @@ def test_kmer_random_vs_oracle(ctx, K):
         assert np.array_equal(bits(d), bits(od))
 
 
+@pytest.mark.parametrize("K", [7, 8])
+def test_kmer_long_reads_vs_oracle(ctx, K):
+    """K-mer lists of every sort form: T = L-K+1 codes up to 512 and up to
+    1,024 sort in registers (8 and 16 per lane), longer reads in LDS; lengths
+    straddle both limits and reach the 2,048 bp maximum, with near copies so
+    that pairs pass."""
+    say("kmer long reads K", K)
+    rng = np.random.default_rng(4000 + K)
+    lens = [T + K - 1 for T in (511, 512, 513, 1023, 1024, 1025)] + [2048, 2047, 1500, 700]
+    lens += [int(x) for x in rng.integers(400, 2049, 30)]
+    alph = np.frombuffer(b"ACGT", np.uint8)
+    base = alph[rng.integers(0, 4, 2048)]
+    seqs = []
+    for k, L in enumerate(lens):
+        if k % 3 == 0:
+            s = alph[rng.integers(0, 4, L)]
+        elif k % 3 == 1:
+            s = base[:L].copy()
+            for _ in range(L // 40):
+                s[int(rng.integers(0, L))] = alph[int(rng.integers(0, 4))]
+        else:
+            s = np.frombuffer(b"TC", np.uint8)[rng.integers(0, 2, L)]
+        seqs.append(s.tobytes())
+    r = ca.Reads.from_list(seqs)
+    ro = O.Reads.from_list(seqs)
+    ctx.upload_reads(ca.KMER, r)
+    for thr in (0.3, 1.0):
+        ctx.kmer_run(K, thr)
+        p, d = ctx.kmer_fetch()
+        op, od = O.kmer_all(ro, K, thr, nthreads=1)
+        assert np.array_equal(p, op)
+        assert np.array_equal(bits(d), bits(od))
+
+
 @pytest.mark.parametrize("K", range(1, 7))
 def test_kmer_exact_pass_forms(ctx, K):
     """K <= 6: the exact counts in the tile kernel and in a pass of their own
