@@ -31,12 +31,15 @@ template <int KB, int MODE>
 __device__ __forceinline__ void nw_row(const int i, int (&Mt)[2 * KB + 1], int (&Mv)[2 * KB + 1],
                                        uint32_t (&LA)[2 * KB + 1],
                                        const uint32_t (&m)[(2 * KB + 1 + 15) / 16], const int dl,
-                                       int &capM, uint32_t &capLA) {
+                                       const int dmax, int &capM, uint32_t &capLA) {
   constexpr int W = 2 * KB + 1;
   int Mh = kNeg;      // "up" value of cell d-1 of this row (with pen_h applied)
   uint32_t LAu = 0u;  // its LA
 #pragma unroll
   for (int d = 0; d < W; ++d) {
+    // past column L1 of every lane: never read by a cell at or before column
+    // L1 (the bound drops by one per row, like the cells' reach)
+    if (row_captures(MODE) && d > dmax) continue;
     if (row_top(MODE)) {
       if (d < KB - i) continue;  // column j = i-kB+d < 0: not part of the matrix
       if (d == KB - i) {         // boundary cell (i, 0): M 0, ML 0, AL i
@@ -129,13 +132,15 @@ __device__ __forceinline__ void nw_reg_wave(const uint32_t *__restrict__ pack,
   // (kB, R), so these are close); lanes past their own R sit out the rows
   // that the others still compute.
   const int R = min(L2, L1 + KB);
-  int l1m = L1, l2hi = R;
+  int l1m = L1, l1M = L1, l2hi = R;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     l1m = min(l1m, __shfl_xor(l1m, off, 64));
+    l1M = max(l1M, __shfl_xor(l1M, off, 64));
     l2hi = max(l2hi, __shfl_xor(l2hi, off, 64));
   }
   const int L1min = __builtin_amdgcn_readfirstlane(l1m);
+  const int L1max = __builtin_amdgcn_readfirstlane(l1M);
   const int Rmax = __builtin_amdgcn_readfirstlane(l2hi);
 
   int Mt[W], Mv[W];
@@ -177,7 +182,7 @@ __device__ __forceinline__ void nw_reg_wave(const uint32_t *__restrict__ pack,
     uint32_t capLA = 0;
     // rows past the pair's last in-band row R leave its registers alone, so
     // they still hold row R when the loop ends
-    if (i <= R) nw_row<KB, MODE>(i, Mt, Mv, LA, m, dl, capM, capLA);
+    if (i <= R) nw_row<KB, MODE>(i, Mt, Mv, LA, m, dl, L1max - i + KB, capM, capLA);
     if (row_captures(MODE) && dl >= 0 && dl < W && i <= L2) {
       const int Mr = capM - 4 * i;
       if (Mr >= bestJ) {
