@@ -227,4 +227,4 @@ def num_threads():
 
 def ref_available():
     return all(os.access(os.path.join(REF_DIR, b), os.X_OK)
-               for b in ("kmerDist", "genDist", "preprocess", "aveclust", "sparsecut"))
+               for b in ("kmerDist", "preprocess", "aveclust", "sparsecut"))

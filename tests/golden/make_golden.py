@@ -1,6 +1,12 @@
 """Writes the golden fixtures in tests/golden/ from the reference's own CPU
-path (kmerDist -c, genDist -c, preprocess), compiled from the reference
-sources into oracle/_ref by `make -C oracle ref`.
+path (kmerDist -c, preprocess), compiled with g++ from the reference sources
+into oracle/_ref by `make -C oracle ref`.
+
+The genDist fixtures (NW distances and .npair_f/.ndist_f files) come from
+`make -C oracle ref-gendist-rocthrust`: genCPU.cpp compiled against rocThrust,
+because the NVIDIA Thrust it includes is not in this image.  That is not the
+reference's own build, so those fixtures are regression data only; the NW
+oracle is pinned by hand-derived cases (tests/test_oracle_nw_rules.py).
 
 Run from the repository root in a container that has the reference sources:
     python tests/golden/make_golden.py
