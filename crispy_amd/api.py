@@ -101,7 +101,7 @@ def load_library():
                                     ctypes.c_int),
         "crispy_server_job_error": ([_vp, ctypes.c_int64], ctypes.c_char_p),
         "crispy_server_fetch": ([_vp, ctypes.c_int64, _i32p, _f32p], ctypes.c_int),
-        "crispy_server_fetch_csr": ([_vp, ctypes.c_int64, _i64p, _i32p, _f32p], ctypes.c_int),
+        "crispy_server_fetch_csr": ([_vp, ctypes.c_int64, ctypes.c_int32, _i64p, _i32p, _f32p], ctypes.c_int),
         "crispy_server_status": ([_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                   _i64p, _i64p], ctypes.c_int),
     }
@@ -492,8 +492,8 @@ class Server:
         rp = np.zeros(n_rows + 1, np.int64)
         cols = np.zeros(max(c, 1), np.int32)
         vals = np.zeros(max(c, 1), np.float32)
-        self._job_check(jid, self._L.crispy_server_fetch_csr(self._h, jid, _p(rp, _i64p), _p(cols, _i32p),
-                                                             _p(vals, _f32p)))
+        self._job_check(jid, self._L.crispy_server_fetch_csr(self._h, jid, int(n_rows), _p(rp, _i64p),
+                                                             _p(cols, _i32p), _p(vals, _f32p)))
         return rp, cols[:c], vals[:c]
 
     def status(self):

@@ -130,6 +130,10 @@ struct NwState {
   GraphSegment g_dp, g_filter;  // captured DP phase and filter
   int64_t count = 0;
   bool valid = false;
+  // order of the pairs the result came from: the CSR view needs them sorted by
+  // (i, j) with every i below its row count
+  bool ij_sorted = false;
+  int32_t max_row = -1;
 };
 
 // Input side (preprocess-equivalent ordering and merge).

@@ -992,10 +992,13 @@ static int run_counted(crispy_ctx *ctx, int64_t &cap, DevBuf &keys, DevBuf *dist
     CRISPY_HIP(ctx, hipMemcpyAsync(&c, km.counter.p, sizeof c, hipMemcpyDeviceToHost, ctx->stream));
     CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     *count = (int64_t)c;
-    if (*count <= cap) break;
-    cap = *count + *count / 8 + 1024;  // rerun with room for every entry
+    if (*count <= cap) return CRISPY_OK;
+    if (attempt == 0) cap = *count + *count / 8 + 1024;  // rerun with room for every entry
   }
-  return CRISPY_OK;
+  // the rerun produced more entries than its own first count: the buffers
+  // hold only `cap` of them, so nothing downstream may read `*count`
+  return set_err(ctx, CRISPY_ERR_STATE, "k-mer output overflowed twice (%lld entries, room for %lld)",
+                 (long long)*count, (long long)cap);
 }
 
 static int sort_keys(crispy_ctx *ctx, int64_t count) {

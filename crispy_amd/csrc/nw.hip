@@ -557,6 +557,16 @@ static int upload_pairs(crispy_ctx *ctx, const int32_t *pairs, int64_t np) {
     if (pairs[p] < 0 || pairs[p] >= n)
       return set_err(ctx, CRISPY_ERR_ARG, "pair %lld names read %d outside [0, %d)", (long long)(p / 2),
                      pairs[p], n);
+  bool sorted = true;
+  int32_t max_row = -1;
+  for (int64_t p = 0; p < np; ++p) {
+    max_row = std::max(max_row, pairs[2 * p]);
+    if (p > 0 && (pairs[2 * p] < pairs[2 * p - 2] ||
+                  (pairs[2 * p] == pairs[2 * p - 2] && pairs[2 * p + 1] < pairs[2 * p - 1])))
+      sorted = false;
+  }
+  ctx->nw.ij_sorted = sorted;
+  ctx->nw.max_row = max_row;
   CRISPY_TRY(ensure(ctx, ctx->nw.in_pairs, sizeof(int2) * (np + 1)));
   if (np > 0)
     CRISPY_HIP(ctx, hipMemcpyAsync(ctx->nw.in_pairs.p, pairs, sizeof(int2) * np, hipMemcpyHostToDevice,
@@ -633,6 +643,9 @@ int crispy_nw_run_on_kmer(crispy_ctx *ctx, int band, float threshold, int64_t *c
   ctx->nw.valid = false;
   const int64_t np = ctx->km.count;
   const int2 *d = kmer_result_pairs(ctx);
+  // the k-mer result is sorted by (i, j) and names reads of this slot
+  ctx->nw.ij_sorted = true;
+  ctx->nw.max_row = ctx->reads[CRISPY_READS_NW].n - 1;
   CRISPY_TRY(nw_core(ctx, d, np, band));
   int64_t c = 0;
   CRISPY_TRY(nw_filter(ctx, d, np, threshold, &c));
@@ -706,6 +719,10 @@ int crispy_nw_csr(crispy_ctx *ctx, int32_t n_rows, int64_t *row_ptr, int32_t *co
   if (!ctx || n_rows < 0 || !row_ptr) return CRISPY_ERR_ARG;
   NwState &nw = ctx->nw;
   if (!nw.valid) return set_err(ctx, CRISPY_ERR_STATE, "no NW result");
+  if (!nw.ij_sorted)
+    return set_err(ctx, CRISPY_ERR_STATE, "NW result is not in (i, j) order: its input pairs were not sorted");
+  if (nw.max_row >= n_rows)
+    return set_err(ctx, CRISPY_ERR_ARG, "NW result names row %d, the CSR has %d rows", nw.max_row, n_rows);
   CRISPY_HIP(ctx, hipSetDevice(ctx->device));
   const int64_t c = nw.count;
   CRISPY_TRY(ensure(ctx, nw.csr_rows, sizeof(int64_t) * ((size_t)n_rows + 1)));

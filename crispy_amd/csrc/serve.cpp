@@ -33,6 +33,7 @@ struct Job {
   int64_t est = 0;
   int64_t held = 0;  // device bytes the slot's context held after the job
   bool done = false;
+  bool consumed = false;  // fetched: results handed out, the job forgotten
   int status = CRISPY_OK;
   std::string error;
   int64_t kmer_pairs = 0;
@@ -58,8 +59,13 @@ struct crispy_server {
   Clock::time_point t0 = Clock::now();
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
-  std::deque<Job *> queue;
-  std::map<int64_t, std::unique_ptr<Job>> jobs;
+  std::deque<std::shared_ptr<Job>> queue;
+  // Jobs are shared with the threads waiting on them, so a fetch that forgets
+  // a job cannot free it under another waiter.
+  std::map<int64_t, std::shared_ptr<Job>> jobs;
+  // Error messages handed out by crispy_server_job_error stay valid until the
+  // server is destroyed.
+  std::map<int64_t, std::string> errors;
   std::vector<crispy_ctx *> ctxs;
   std::vector<std::thread> workers;
 };
@@ -92,7 +98,7 @@ void run_job(crispy_ctx *ctx, Job &job) {
 void worker_loop(crispy_server *srv, int slot) {
   crispy_ctx *ctx = srv->ctxs[slot];
   for (;;) {
-    Job *job = nullptr;
+    std::shared_ptr<Job> job;
     {
       std::unique_lock<std::mutex> lk(srv->mu);
       for (;;) {
@@ -233,7 +239,7 @@ int64_t crispy_server_submit(crispy_server *srv, const char *kmer_bases, const i
   if (!srv || n < 0 || n > CRISPY_MAX_NUM_READS || (n > 0 && (!kmer_bases || !kmer_offsets)) ||
       K < 1 || K > 8)
     return CRISPY_ERR_ARG;
-  auto job = std::make_unique<Job>();
+  auto job = std::make_shared<Job>();
   job->n = n;
   job->K = K;
   job->kthr = kmer_threshold;
@@ -258,11 +264,10 @@ int64_t crispy_server_submit(crispy_server *srv, const char *kmer_bases, const i
   if (job->est > srv->budget) return CRISPY_ERR_NOMEM;  // would never be admitted
   job->id = srv->next_id++;
   job->t_submit = Clock::now();
-  Job *raw = job.get();
-  srv->jobs[raw->id] = std::move(job);
-  srv->queue.push_back(raw);
+  srv->jobs[job->id] = job;
+  srv->queue.push_back(job);
   srv->cv_work.notify_all();
-  return raw->id;
+  return job->id;
 }
 
 int crispy_server_wait(crispy_server *srv, int64_t job_id, int64_t *kmer_pairs, int64_t *nw_pairs,
@@ -271,8 +276,9 @@ int crispy_server_wait(crispy_server *srv, int64_t job_id, int64_t *kmer_pairs, 
   std::unique_lock<std::mutex> lk(srv->mu);
   auto it = srv->jobs.find(job_id);
   if (it == srv->jobs.end()) return CRISPY_ERR_ARG;
-  Job *job = it->second.get();
+  const std::shared_ptr<Job> job = it->second;
   srv->cv_done.wait(lk, [&] { return job->done; });
+  if (job->consumed) return CRISPY_ERR_STATE;  // fetched meanwhile by another thread
   if (kmer_pairs) *kmer_pairs = job->kmer_pairs;
   if (nw_pairs) *nw_pairs = (int64_t)job->dists.size();
   using ms = std::chrono::duration<double, std::milli>;
@@ -304,7 +310,9 @@ const char *crispy_server_job_error(crispy_server *srv, int64_t job_id) {
   std::lock_guard<std::mutex> lk(srv->mu);
   auto it = srv->jobs.find(job_id);
   if (it == srv->jobs.end() || !it->second->done) return "";
-  return it->second->error.c_str();
+  std::string &kept = srv->errors[job_id];
+  kept = it->second->error;
+  return kept.c_str();
 }
 
 int crispy_server_fetch(crispy_server *srv, int64_t job_id, int32_t *pairs, float *dists) {
@@ -312,25 +320,32 @@ int crispy_server_fetch(crispy_server *srv, int64_t job_id, int32_t *pairs, floa
   std::unique_lock<std::mutex> lk(srv->mu);
   auto it = srv->jobs.find(job_id);
   if (it == srv->jobs.end()) return CRISPY_ERR_ARG;
-  Job *job = it->second.get();
+  const std::shared_ptr<Job> job = it->second;
   srv->cv_done.wait(lk, [&] { return job->done; });
+  if (job->consumed) return CRISPY_ERR_STATE;  // fetched meanwhile by another thread
   const int status = job->status;
   if (status == CRISPY_OK) {
     if (pairs && !job->pairs.empty()) std::memcpy(pairs, job->pairs.data(), job->pairs.size() * sizeof(int32_t));
     if (dists && !job->dists.empty()) std::memcpy(dists, job->dists.data(), job->dists.size() * sizeof(float));
   }
-  srv->jobs.erase(it);
+  if (!job->error.empty()) srv->errors[job_id] = job->error;
+  job->consumed = true;
+  srv->jobs.erase(job_id);
+  srv->cv_done.notify_all();
   return status;
 }
 
-int crispy_server_fetch_csr(crispy_server *srv, int64_t job_id, int64_t *row_ptr, int32_t *cols,
-                            float *vals) {
+int crispy_server_fetch_csr(crispy_server *srv, int64_t job_id, int32_t n_rows, int64_t *row_ptr,
+                            int32_t *cols, float *vals) {
   if (!srv || !row_ptr) return CRISPY_ERR_ARG;
   std::unique_lock<std::mutex> lk(srv->mu);
   auto it = srv->jobs.find(job_id);
   if (it == srv->jobs.end()) return CRISPY_ERR_ARG;
-  Job *job = it->second.get();
+  const std::shared_ptr<Job> job = it->second;
+  // row_ptr holds n_rows + 1 entries: it must be the job's read count
+  if (n_rows != job->n) return CRISPY_ERR_ARG;
   srv->cv_done.wait(lk, [&] { return job->done; });
+  if (job->consumed) return CRISPY_ERR_STATE;  // fetched meanwhile by another thread
   const int status = job->status;
   if (status == CRISPY_OK) {
     std::memcpy(row_ptr, job->row_ptr.data(), job->row_ptr.size() * sizeof(int64_t));
@@ -339,7 +354,10 @@ int crispy_server_fetch_csr(crispy_server *srv, int64_t job_id, int64_t *row_ptr
       for (size_t k = 0; k < c; ++k) cols[k] = job->pairs[2 * k + 1];
     if (vals && c) std::memcpy(vals, job->dists.data(), c * sizeof(float));
   }
-  srv->jobs.erase(it);
+  if (!job->error.empty()) srv->errors[job_id] = job->error;
+  job->consumed = true;
+  srv->jobs.erase(job_id);
+  srv->cv_done.notify_all();
   return status;
 }
 

@@ -142,7 +142,8 @@ int crispy_nw_fetch(crispy_ctx *ctx, int64_t first, int64_t count, int32_t *pair
  * on the device from the (i, j)-ordered pairs of crispy_nw_run_on_kmer (or of
  * crispy_nw_run on (i, j)-ordered input); row_ptr[n_rows + 1] on the host,
  * cols / vals (count entries, either may be NULL) the column ids and
- * distances. */
+ * distances.  CRISPY_ERR_STATE when the input pairs were not (i, j)-ordered,
+ * CRISPY_ERR_ARG when some pair's i is not below n_rows. */
 int crispy_nw_csr(crispy_ctx *ctx, int32_t n_rows, int64_t *row_ptr, int32_t *cols, float *vals);
 /* Band cells per lane of the lane-sliced NW kernel (kB > 32): 0 = chosen per
  * kB bucket by the library's cost model (the default), 24 or 32 = forced.
@@ -266,15 +267,20 @@ int crispy_server_wait(crispy_server *srv, int64_t job_id, int64_t *kmer_pairs, 
  * finished job, its admission estimate and the bytes its slot held after it. */
 int crispy_server_job_times(crispy_server *srv, int64_t job_id, double *submit_ms, double *start_ms,
                             double *end_ms, int64_t *est_bytes, int64_t *held_bytes);
-/* Error message of a finished job that failed ("" otherwise). */
+/* Error message of a finished job that failed ("" otherwise); the string
+ * stays valid until crispy_server_destroy. */
 const char *crispy_server_job_error(crispy_server *srv, int64_t job_id);
 /* Waits for the job, copies its NW-passing pairs (int32 x2, ascending (i, j))
- * and distances out and forgets the job. */
+ * and distances out and forgets the job.  A job is fetched once: a second
+ * fetch (or a wait that a fetch overtook) returns CRISPY_ERR_STATE, or
+ * CRISPY_ERR_ARG once the job is forgotten. */
 int crispy_server_fetch(crispy_server *srv, int64_t job_id, int32_t *pairs, float *dists);
-/* Like crispy_server_fetch, as CSR: row_ptr[n + 1] (built on the device),
- * cols and vals (nw_pairs entries, either may be NULL); forgets the job. */
-int crispy_server_fetch_csr(crispy_server *srv, int64_t job_id, int64_t *row_ptr, int32_t *cols,
-                            float *vals);
+/* Like crispy_server_fetch, as CSR: row_ptr[n_rows + 1] (built on the
+ * device), cols and vals (nw_pairs entries, either may be NULL); forgets the
+ * job.  n_rows must be the job's read count (CRISPY_ERR_ARG otherwise, the
+ * job is kept). */
+int crispy_server_fetch_csr(crispy_server *srv, int64_t job_id, int32_t n_rows, int64_t *row_ptr,
+                            int32_t *cols, float *vals);
 /* Jobs running and queued, device bytes charged to the slots now and at most
  * so far (idle slots are charged what their contexts hold and give it back
  * when the queue head would not fit otherwise). */

@@ -170,9 +170,22 @@ def test_csr_views(jobs, solo):
         c.kmer_run(job["K"], job["kthr"])
         c.nw_run_on_kmer(job["band"], job["nthr"])
         rp, cols, vals = c.nw_csr(n)
-    assert np.array_equal(rp, want_rp) and np.array_equal(cols, p[:, 1])
-    assert np.array_equal(vals.view(np.uint32), d.view(np.uint32))
+        assert np.array_equal(rp, want_rp) and np.array_equal(cols, p[:, 1])
+        assert np.array_equal(vals.view(np.uint32), d.view(np.uint32))
+        # a row count below the largest row, and pairs not in (i, j) order, are refused
+        with pytest.raises(ca.CrispyError):
+            c.nw_csr(int(p[:, 0].max()))
+        c.nw_run(p[::-1].copy(), job["band"], job["nthr"])
+        with pytest.raises(ca.CrispyError):
+            c.nw_csr(n)
     with ca.Server(0, 2, 32 << 30) as srv:
-        rp2, cols2, vals2 = srv.fetch_csr(submit(srv, job), n)
+        jid = submit(srv, job)
+        # row_ptr sized for another read count: refused, the job is kept
+        with pytest.raises(ca.CrispyError):
+            srv.fetch_csr(jid, n - 1)
+        rp2, cols2, vals2 = srv.fetch_csr(jid, n)
+        # a job is fetched once
+        with pytest.raises(ca.CrispyError):
+            srv.fetch(jid)
     assert np.array_equal(rp2, want_rp) and np.array_equal(cols2, p[:, 1])
     assert np.array_equal(vals2.view(np.uint32), d.view(np.uint32))
