@@ -72,6 +72,9 @@ def load_library():
         "crispy_kmer_set_exact_pass": ([_vp, ctypes.c_int], ctypes.c_int),
         "crispy_nw_autotune": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "crispy_sort_by_dist": ([_vp, _i32p, _f32p, ctypes.c_int64], ctypes.c_int),
+        "crispy_nw_file_split": ([_vp, ctypes.c_int64, ctypes.c_int64, _i64p, ctypes.c_int32,
+                                  ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
+        "crispy_nw_fetch_sorted": ([_vp, ctypes.c_int64, ctypes.c_int64, _i32p, _f32p], ctypes.c_int),
         "crispy_timing_enable": ([_vp, ctypes.c_int], ctypes.c_int),
         "crispy_timing_reset": ([_vp], ctypes.c_int),
         "crispy_timing_query": ([_vp, ctypes.c_char_p, _f64p, _i64p, _f64p], ctypes.c_int),
@@ -362,6 +365,25 @@ class Context:
         self._check(self._L.crispy_sort_by_dist(self._h, _p(pairs, _i32p), _p(dists, _f32p),
                                                 len(dists)))
         return pairs, dists
+
+    def nw_file_split(self, chunk_pairs=576 * 256 * 4, flush_pairs=576 * 256, max_files=4096):
+        """Result indices where the reference's output files end (its input
+        read in `chunk_pairs` chunks, a file closed at >= `flush_pairs`)."""
+        ends = np.zeros(max_files, np.int64)
+        nf = ctypes.c_int32()
+        self._check(self._L.crispy_nw_file_split(self._h, chunk_pairs, flush_pairs, _p(ends, _i64p),
+                                                 max_files, ctypes.byref(nf)))
+        return ends[:nf.value]
+
+    def nw_fetch_sorted(self, first=0, count=None):
+        """Result entries [first, first + count) ordered by (dist, i, j), sorted
+        on the device: one output file's contents."""
+        if count is None:
+            count = self.nw_count - first
+        p = np.zeros((count, 2), np.int32)
+        d = np.zeros(count, np.float32)
+        self._check(self._L.crispy_nw_fetch_sorted(self._h, first, count, _p(p, _i32p), _p(d, _f32p)))
+        return p, d
 
     # -- input side
     def preprocess_primer_filter(self, rec, primer: bytes, max_mismatches=2):

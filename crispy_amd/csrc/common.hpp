@@ -126,9 +126,11 @@ struct NwState {
   DevBuf plan;  // NwPlan of the DP phase (bucket table, family task lists, counters)
   DevBuf stat;  // cells per family (generic last) summed over timed runs
   DevBuf csr_rows, csr_cols;  // CSR view of the (i, j)-ordered result
+  DevBuf sort_k0, sort_k1, sort_b0, sort_b1, sort_pairs, sort_dists;  // (dist, i, j) sorts
   int slice_width = 0;  // sliced-kernel cells per lane: 0 = cost model, 24 or 32 = forced
   GraphSegment g_dp, g_filter;  // captured DP phase and filter
   int64_t count = 0;
+  int64_t in_count = 0;  // input pairs of the last run (the filter's scan covers them)
   bool valid = false;
   // order of the pairs the result came from: the CSR view needs them sorted by
   // (i, j) with every i below its row count

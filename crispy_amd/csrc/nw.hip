@@ -599,6 +599,45 @@ int nw_timer_reset(crispy_ctx *ctx) {
   return CRISPY_OK;
 }
 
+// Sorts n (pair, dist) entries by (dist, i, j) into out_pairs / out_dists
+// (device buffers that may alias the inputs).  Distances are >= 0, so their
+// bit patterns order like the values.  Input already in (i, j) order needs
+// one stable sort by distance; other input is first sorted by (i, j).
+static int sort_dist_ij(crispy_ctx *ctx, const int2 *pairs, const float *dists, int64_t n, bool ij_sorted,
+                        int2 *out_pairs, float *out_dists) {
+  NwState &nw = ctx->nw;
+  CRISPY_TRY(ensure(ctx, nw.sort_k0, sizeof(uint64_t) * n));
+  CRISPY_TRY(ensure(ctx, nw.sort_k1, sizeof(uint64_t) * n));
+  CRISPY_TRY(ensure(ctx, nw.sort_b0, sizeof(uint32_t) * n));
+  CRISPY_TRY(ensure(ctx, nw.sort_b1, sizeof(uint32_t) * n));
+  uint64_t *k0 = nw.sort_k0.as<uint64_t>(), *k1 = nw.sort_k1.as<uint64_t>();
+  uint32_t *b0 = nw.sort_b0.as<uint32_t>(), *b1 = nw.sort_b1.as<uint32_t>();
+  pack_pair_keys_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(pairs, n, k0);
+  dist_bits_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(dists, n, b0);
+  CRISPY_LAUNCHED(ctx);
+  size_t t1 = 0, t2 = 0;
+  if (!ij_sorted)
+    CRISPY_HIP(ctx, rocprim::radix_sort_pairs<RadixOnesweep>(nullptr, t1, k0, k1, b0, b1, (size_t)n, 0, 40,
+                                                             ctx->stream));
+  CRISPY_HIP(ctx, rocprim::radix_sort_pairs<RadixOnesweep>(nullptr, t2, b0, b1, k0, k1, (size_t)n, 0, 32,
+                                                           ctx->stream));
+  CRISPY_TRY(ensure(ctx, ctx->temp, std::max(t1, t2)));
+  if (!ij_sorted) {
+    // 1) by (i, j) into k1/b1, 2) stable by dist back into k0/b0
+    CRISPY_HIP(ctx, rocprim::radix_sort_pairs<RadixOnesweep>(ctx->temp.p, t1, k0, k1, b0, b1, (size_t)n, 0, 40,
+                                                             ctx->stream));
+    CRISPY_HIP(ctx, rocprim::radix_sort_pairs<RadixOnesweep>(ctx->temp.p, t2, b1, b0, k1, k0, (size_t)n, 0, 32,
+                                                             ctx->stream));
+    unpack_pair_keys_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(k0, b0, n, out_pairs, out_dists);
+  } else {
+    CRISPY_HIP(ctx, rocprim::radix_sort_pairs<RadixOnesweep>(ctx->temp.p, t2, b0, b1, k0, k1, (size_t)n, 0, 32,
+                                                             ctx->stream));
+    unpack_pair_keys_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(k1, b1, n, out_pairs, out_dists);
+  }
+  CRISPY_LAUNCHED(ctx);
+  return CRISPY_OK;
+}
+
 void release_graphs(crispy_ctx *ctx) {
   drop_segment(ctx->nw.g_dp);
   drop_segment(ctx->nw.g_filter);
@@ -629,6 +668,7 @@ int crispy_nw_run(crispy_ctx *ctx, const int32_t *pairs, int64_t npairs, int ban
   int64_t c = 0;
   CRISPY_TRY(nw_filter(ctx, d, npairs, threshold, &c));
   ctx->nw.count = c;
+  ctx->nw.in_count = npairs;
   ctx->nw.valid = true;
   if (count) *count = c;
   return CRISPY_OK;
@@ -650,6 +690,7 @@ int crispy_nw_run_on_kmer(crispy_ctx *ctx, int band, float threshold, int64_t *c
   int64_t c = 0;
   CRISPY_TRY(nw_filter(ctx, d, np, threshold, &c));
   ctx->nw.count = c;
+  ctx->nw.in_count = np;
   ctx->nw.valid = true;
   if (count) *count = c;
   return CRISPY_OK;
@@ -669,6 +710,60 @@ int crispy_nw_fetch(crispy_ctx *ctx, int64_t first, int64_t count, int32_t *pair
   if (dists)
     CRISPY_HIP(ctx, hipMemcpyAsync(dists, nw.out_dists.as<float>() + first, sizeof(float) * count,
                                    hipMemcpyDeviceToHost, ctx->stream));
+  CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return CRISPY_OK;
+}
+
+int crispy_nw_file_split(crispy_ctx *ctx, int64_t chunk_pairs, int64_t flush_pairs, int64_t *file_ends,
+                         int32_t max_files, int32_t *nfiles) {
+  if (!ctx || chunk_pairs < 1 || flush_pairs < 1 || max_files < 0 || !nfiles || (max_files > 0 && !file_ends))
+    return CRISPY_ERR_ARG;
+  NwState &nw = ctx->nw;
+  if (!nw.valid) return set_err(ctx, CRISPY_ERR_STATE, "no NW result");
+  const int64_t np = nw.in_count;
+  const int64_t chunks = (np + chunk_pairs - 1) / chunk_pairs;
+  // the filter's exclusive scan: sel[p] = passing pairs among inputs [0, p)
+  std::vector<uint32_t> ends((size_t)chunks, 0);
+  if (chunks > 0) {
+    CRISPY_HIP(ctx, hipSetDevice(ctx->device));
+    for (int64_t c = 0; c + 1 < chunks; ++c)
+      CRISPY_HIP(ctx, hipMemcpyAsync(&ends[(size_t)c], nw.sel.as<uint32_t>() + (c + 1) * chunk_pairs,
+                                     sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ends[(size_t)chunks - 1] = (uint32_t)nw.count;
+  }
+  int32_t f = 0;
+  int64_t begin = 0;
+  for (int64_t c = 0; c < chunks; ++c) {
+    if ((int64_t)ends[(size_t)c] - begin >= flush_pairs) {
+      if (f < max_files) file_ends[f] = ends[(size_t)c];
+      ++f;
+      begin = ends[(size_t)c];
+    }
+  }
+  if (nw.count > begin) {
+    if (f < max_files) file_ends[f] = nw.count;
+    ++f;
+  }
+  *nfiles = f;
+  return f <= max_files ? CRISPY_OK : set_err(ctx, CRISPY_ERR_ARG, "%d files, room for %d", f, max_files);
+}
+
+int crispy_nw_fetch_sorted(crispy_ctx *ctx, int64_t first, int64_t count, int32_t *pairs, float *dists) {
+  if (!ctx || (count > 0 && (!pairs || !dists))) return CRISPY_ERR_ARG;
+  NwState &nw = ctx->nw;
+  if (!nw.valid) return set_err(ctx, CRISPY_ERR_STATE, "no NW result");
+  if (first < 0 || count < 0 || first + count > nw.count)
+    return set_err(ctx, CRISPY_ERR_ARG, "fetch range outside the result");
+  if (count == 0) return CRISPY_OK;
+  CRISPY_HIP(ctx, hipSetDevice(ctx->device));
+  CRISPY_TRY(ensure(ctx, nw.sort_pairs, sizeof(int2) * count));
+  CRISPY_TRY(ensure(ctx, nw.sort_dists, sizeof(float) * count));
+  CRISPY_TRY(sort_dist_ij(ctx, nw.out_pairs.as<int2>() + first, nw.out_dists.as<float>() + first, count,
+                          nw.ij_sorted, nw.sort_pairs.as<int2>(), nw.sort_dists.as<float>()));
+  CRISPY_HIP(ctx, hipMemcpyAsync(pairs, nw.sort_pairs.p, sizeof(int2) * count, hipMemcpyDeviceToHost, ctx->stream));
+  CRISPY_HIP(ctx, hipMemcpyAsync(dists, nw.sort_dists.p, sizeof(float) * count, hipMemcpyDeviceToHost,
+                                 ctx->stream));
   CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return CRISPY_OK;
 }
@@ -769,33 +864,14 @@ int crispy_sort_by_dist(crispy_ctx *ctx, int32_t *pairs, float *dists, int64_t n
       return set_err(ctx, CRISPY_ERR_ARG, "distance %g is not >= 0", (double)dists[k]);
   CRISPY_HIP(ctx, hipSetDevice(ctx->device));
   NwState &nw = ctx->nw;
-  nw.valid = false;
-  // (i, j) keys in the out buffers, dist bits in keys/keys_alt
-  CRISPY_TRY(ensure(ctx, nw.in_pairs, sizeof(int2) * n));
-  CRISPY_TRY(ensure(ctx, nw.out_pairs, sizeof(uint64_t) * n));
-  CRISPY_TRY(ensure(ctx, nw.scratch, sizeof(uint64_t) * n));
-  CRISPY_TRY(ensure(ctx, nw.dist, sizeof(float) * n));
-  CRISPY_TRY(ensure(ctx, nw.keys, sizeof(uint32_t) * n));
-  CRISPY_TRY(ensure(ctx, nw.keys_alt, sizeof(uint32_t) * n));
-  uint64_t *k0 = nw.out_pairs.as<uint64_t>(), *k1 = nw.scratch.as<uint64_t>();
-  uint32_t *b0 = nw.keys.as<uint32_t>(), *b1 = nw.keys_alt.as<uint32_t>();
-  CRISPY_HIP(ctx, hipMemcpyAsync(nw.in_pairs.p, pairs, sizeof(int2) * n, hipMemcpyHostToDevice, ctx->stream));
-  CRISPY_HIP(ctx, hipMemcpyAsync(nw.dist.p, dists, sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
-  pack_pair_keys_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(nw.in_pairs.as<int2>(), n, k0);
-  dist_bits_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(nw.dist.as<float>(), n, b0);
-  CRISPY_LAUNCHED(ctx);
-  // 1) by (i, j)   2) stable by dist  ->  (dist, i, j)
-  size_t t1 = 0, t2 = 0;
-  CRISPY_HIP(ctx, rocprim::radix_sort_pairs<RadixOnesweep>(nullptr, t1, k0, k1, b0, b1, (size_t)n, 0, 40, ctx->stream));
-  CRISPY_HIP(ctx, rocprim::radix_sort_pairs<RadixOnesweep>(nullptr, t2, b1, b0, k1, k0, (size_t)n, 0, 32, ctx->stream));
-  CRISPY_TRY(ensure(ctx, ctx->temp, std::max(t1, t2)));
-  CRISPY_HIP(ctx, rocprim::radix_sort_pairs<RadixOnesweep>(ctx->temp.p, t1, k0, k1, b0, b1, (size_t)n, 0, 40, ctx->stream));
-  CRISPY_HIP(ctx, rocprim::radix_sort_pairs<RadixOnesweep>(ctx->temp.p, t2, b1, b0, k1, k0, (size_t)n, 0, 32, ctx->stream));
-  unpack_pair_keys_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(k0, b0, n, nw.in_pairs.as<int2>(),
-                                                                 nw.dist.as<float>());
-  CRISPY_LAUNCHED(ctx);
-  CRISPY_HIP(ctx, hipMemcpyAsync(pairs, nw.in_pairs.p, sizeof(int2) * n, hipMemcpyDeviceToHost, ctx->stream));
-  CRISPY_HIP(ctx, hipMemcpyAsync(dists, nw.dist.p, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream));
+  CRISPY_TRY(ensure(ctx, nw.sort_pairs, sizeof(int2) * n));
+  CRISPY_TRY(ensure(ctx, nw.sort_dists, sizeof(float) * n));
+  CRISPY_HIP(ctx, hipMemcpyAsync(nw.sort_pairs.p, pairs, sizeof(int2) * n, hipMemcpyHostToDevice, ctx->stream));
+  CRISPY_HIP(ctx, hipMemcpyAsync(nw.sort_dists.p, dists, sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
+  CRISPY_TRY(sort_dist_ij(ctx, nw.sort_pairs.as<int2>(), nw.sort_dists.as<float>(), n, false,
+                          nw.sort_pairs.as<int2>(), nw.sort_dists.as<float>()));
+  CRISPY_HIP(ctx, hipMemcpyAsync(pairs, nw.sort_pairs.p, sizeof(int2) * n, hipMemcpyDeviceToHost, ctx->stream));
+  CRISPY_HIP(ctx, hipMemcpyAsync(dists, nw.sort_dists.p, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream));
   CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return CRISPY_OK;
 }

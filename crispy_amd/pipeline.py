@@ -79,6 +79,20 @@ def compute_gen_dist(ctx: Context, reads: Reads, in_pair_path: str, pair_prefix:
     return nfiles, total
 
 
+def fused_output_files(ctx: Context):
+    """The output files of the current NW result (crispy_nw_run_on_kmer): the
+    reference's file split of its input (the k-mer pairs read in
+    CHUNK_PAIRS chunks, a file closed at >= FLUSH_PAIRS, genCPU.cpp:482-505)
+    and each file's pairs in (dist, i, j) order, sorted on the device.
+    Returns a list of (pairs (n, 2) int32, dists float32)."""
+    out = []
+    first = 0
+    for end in ctx.nw_file_split(CHUNK_PAIRS, FLUSH_PAIRS):
+        out.append(ctx.nw_fetch_sorted(first, int(end) - first))
+        first = int(end)
+    return out
+
+
 def preprocess(ctx: Context, in_path: str, out_path: str = None, fastq: bool = False,
                sd_allowed: float = None, primer: bytes = None, max_mismatches: int = 2):
     """The reference preprocess main (preprocess.cpp:686-726): read, optional

@@ -172,6 +172,21 @@ int crispy_nw_dists(crispy_ctx *ctx, const int32_t *pairs, int64_t npairs, int b
 /* Sorts host arrays (pairs int32 x2, dists) ascending by (dist, i, j) on the
  * device; the per-file order of writeVectorToFile_GPU (genCUDA.cu:39-45). */
 int crispy_sort_by_dist(crispy_ctx *ctx, int32_t *pairs, float *dists, int64_t n);
+/* The reference's output-file split of the current NW result: its input pairs
+ * read in chunks of `chunk_pairs` (loadPairs, genMain.cpp:129-139; 589,824 in
+ * the reference), a file closed after a chunk once it holds >= `flush_pairs`
+ * passing pairs and a last file for the rest (genCPU.cpp:482-505: 147,456 on
+ * the CPU path; genCUDA.cu:228-242: 589,824 on the GPU path).  Writes the
+ * result index where each file ends into file_ends[0..*nfiles); read from the
+ * filter's device scan, no pair is copied.  CRISPY_ERR_ARG when more than
+ * max_files files would be needed (*nfiles says how many). */
+int crispy_nw_file_split(crispy_ctx *ctx, int64_t chunk_pairs, int64_t flush_pairs, int64_t *file_ends,
+                         int32_t max_files, int32_t *nfiles);
+/* Result entries [first, first + count) sorted by (dist, i, j) on the device
+ * and copied to the host: one output file's contents in the order
+ * writeVectorToFile sorts them (thrust::sort_by_key, genCUDA.cu:39-45,
+ * genCPU.cpp:36-40; ties by (i, j)).  The result itself is left as it was. */
+int crispy_nw_fetch_sorted(crispy_ctx *ctx, int64_t first, int64_t count, int32_t *pairs, float *dists);
 
 /* --------------------------------------------------------------- timing */
 /* When enabled, every kernel launch is bracketed by HIP events on the

@@ -17,13 +17,25 @@
  *   oracle_kmer_pair       -- multiset intersection (K<=7) by brute force,
  *                             the K=8 hi/lo closed form, and the reference
  *                             kmerDist -c output (golden).
- *   oracle_nw_pair         -- reference genDist -c output (golden) for
- *                             band 2/5/10 and full mode, plus closed-form
- *                             special cases (identical reads, one
- *                             substitution, kB = 0).
- *   oracle_gendist_files   -- reference genDist -c .npair_f/.ndist_f files.
+ *   oracle_nw_pair         -- hand-derived cell-by-cell cases of both forms
+ *                             (tests/test_oracle_nw_rules.py: zero-cost gap
+ *                             switch, the -1000000 band edge, the diag > up >
+ *                             left tie chain, the >= last maxima, the final
+ *                             tie), each shown to catch a planted mistake in
+ *                             a copy of this file; closed forms (identical
+ *                             reads, one substitution, kB = 0).  Fixtures from
+ *                             genCPU.cpp built against rocThrust (not the
+ *                             reference's own build) are regression data.
+ *   oracle_gendist_files   -- its file split and order: the reference's loop
+ *                             (genCPU.cpp:466-505) by reading, and the same
+ *                             rocThrust-built regression fixtures.
+ *   oracle_nw_pairs_as_shipped -- the same distances as oracle_nw_pair (test)
+ *                             under genDist's shipped thread settings; timing
+ *                             only.
  */
+#define _GNU_SOURCE
 #include <math.h>
+#include <sched.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -424,6 +436,42 @@ int oracle_nw_pairs(const char *bases, const int64_t *offs, int32_t n,
                                       band, &ml, &al, NULL);
         if (out_ml) out_ml[p] = ml;
         if (out_al) out_al[p] = al;
+    }
+    return bad ? -1 : 0;
+}
+
+/* The same distances under genDist's shipped CPU settings, for timing: the  */
+/* pair loop as genCPU.cpp:346-480 runs it (schedule(static,1), one critical */
+/* section per pair) with `nthreads` threads confined to `ncpus` CPUs, as    */
+/* the shipped build pins the process to CPUs 0-7 (genMain.cpp:49-73 with    */
+/* -DSET_AFFINITY, src/genDist/Makefile:6) after taking omp_get_num_procs()  */
+/* threads (genMain.cpp:336).  The first `ncpus` CPUs this process may use   */
+/* stand in for 0-7; each thread restores its own CPU set afterwards.        */
+int oracle_nw_pairs_as_shipped(const char *bases, const int64_t *offs, int32_t n,
+                               const int32_t *pairs, int64_t npairs, int band, int nthreads,
+                               int ncpus, float *out_dists) {
+    cpu_set_t allowed, pin;
+    if (sched_getaffinity(0, sizeof allowed, &allowed)) return -1;
+    CPU_ZERO(&pin);
+    for (int c = 0, got = 0; c < CPU_SETSIZE && got < ncpus; ++c)
+        if (CPU_ISSET(c, &allowed)) { CPU_SET(c, &pin); ++got; }
+    if (nthreads < 1) nthreads = 1;
+    int bad = 0;
+    #pragma omp parallel num_threads(nthreads)
+    {
+        cpu_set_t mine;
+        int saved = sched_getaffinity(0, sizeof mine, &mine) == 0;
+        sched_setaffinity(0, sizeof pin, &pin);
+        #pragma omp for schedule(static, 1)
+        for (int64_t p = 0; p < npairs; ++p) {
+            int32_t a = pairs[2 * p], b = pairs[2 * p + 1];
+            if (a < 0 || b < 0 || a >= n || b >= n) { bad = 1; continue; }
+            float d = oracle_nw_pair(bases + offs[a], (int)(offs[a + 1] - offs[a]),
+                                     bases + offs[b], (int)(offs[b + 1] - offs[b]), band, NULL, NULL, NULL);
+            #pragma omp critical
+            out_dists[p] = d;
+        }
+        if (saved) sched_setaffinity(0, sizeof mine, &mine);
     }
     return bad ? -1 : 0;
 }

@@ -57,6 +57,9 @@ def lib():
         L.oracle_nw_pairs.argtypes = [ctypes.c_char_p, _i64p, ctypes.c_int32, _i32p,
                                       ctypes.c_int64, ctypes.c_int, ctypes.c_int, _f32p,
                                       _i32p, _i32p]
+        L.oracle_nw_pairs_as_shipped.argtypes = [ctypes.c_char_p, _i64p, ctypes.c_int32, _i32p,
+                                                 ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+                                                 ctypes.c_int, _f32p]
         L.oracle_gendist_files.argtypes = [ctypes.c_char_p, _i64p, ctypes.c_int32, _i32p,
                                            ctypes.c_int64, ctypes.c_int, ctypes.c_float,
                                            ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p,
@@ -186,6 +189,19 @@ def nw_pairs(reads: Reads, pairs: np.ndarray, band: int, nthreads=None):
     if rc:
         raise ValueError("pair index out of range")
     return d, ml, al
+
+
+def nw_pairs_as_shipped(reads: Reads, pairs: np.ndarray, band: int, nthreads: int, ncpus: int):
+    """NW distances under genDist's shipped CPU settings (nthreads threads
+    confined to ncpus CPUs, static,1 schedule, a critical section per pair)."""
+    pairs = np.ascontiguousarray(pairs, dtype=np.int32)
+    d = np.zeros(len(pairs), np.float32)
+    rc = lib().oracle_nw_pairs_as_shipped(reads.bases, _p(reads.offsets, _i64p), reads.n,
+                                          _p(pairs, _i32p), len(pairs), band, nthreads, ncpus,
+                                          _p(d, _f32p))
+    if rc:
+        raise ValueError("pair index out of range or CPU set unavailable")
+    return d
 
 
 def gendist_files(reads: Reads, pairs: np.ndarray, band: int, thr: float, prefix: str,

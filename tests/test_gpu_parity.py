@@ -505,6 +505,59 @@ def test_cli_file_split(tmp_path):
     assert not os.path.exists(f"{fa}.npair_{int(g['nfiles'])}")
 
 
+def test_device_file_split_and_sorted_fetch(ctx):
+    """The device output side: the file split of a 719,400-pair run read from
+    the filter's scan, and each file sorted by (dist, i, j) on the device,
+    equal the reference files (two files) entry for entry; the same on a
+    fused reads1k run against the oracle's genDist files."""
+    say("device file split")
+    g = np.load(os.path.join(GOLD, "synth_multifile.npz"))
+    seqs, _, _ = synth.make("reads1k", n=1200)
+    r = ca.Reads.from_list(seqs)
+    ctx.upload_reads(ca.NW, r)
+    ctx.nw_run(all_pairs(int(g["n"])), 10, float(g["thr"]))
+    files = ca.pipeline.fused_output_files(ctx)
+    assert len(files) == int(g["nfiles"])
+    for f, (p, d) in enumerate(files):
+        assert len(d) == int(g[f"f{f}_count"])
+        assert np.array_equal(canon(p, d)[:, 0], bits(d).astype(np.int64))  # already (dist, i, j)
+        assert np.array_equal(canon(p, d)[:, 1:], p.astype(np.int64))
+        assert digest(p, d) == str(g[f"f{f}_sha256"])
+    # small chunks and flush sizes: files end after the chunks that bring a
+    # file to the flush size
+    ends = ctx.nw_file_split(50000, 30000)
+    sel = np.array([O.passes(float(x), float(g["thr"])) for x in
+                    ctx.nw_dists(all_pairs(int(g["n"])), 10)], bool)
+    want, acc, c0 = [], 0, 0
+    for c in range(0, len(sel), 50000):
+        acc += int(sel[c:c + 50000].sum())
+        if acc - c0 >= 30000:
+            want.append(acc)
+            c0 = acc
+    if acc > c0:
+        want.append(acc)
+    assert len(want) > 2 and np.array_equal(ends, np.array(want, np.int64))
+    # fused run against the oracle's files
+    seqs, _, _ = synth.make("reads1k")
+    r = ca.Reads.from_list(seqs)
+    ctx.upload_reads(ca.KMER, r)
+    ctx.upload_reads(ca.NW, r)
+    ctx.kmer_run(6, ca.kmer_default_threshold(r.n))
+    kp, _ = ctx.kmer_fetch(dists=False)
+    nthr = ca.nw_default_threshold(r.n)
+    ctx.nw_run_on_kmer(10, nthr)
+    files = ca.pipeline.fused_output_files(ctx)
+    ro = O.Reads.from_list(seqs)
+    import tempfile
+    with tempfile.TemporaryDirectory() as td:
+        nf, tot = O.gendist_files(ro, kp, 10, nthr, os.path.join(td, "o"))
+        assert nf == len(files) and tot == sum(len(d) for _, d in files)
+        for f, (p, d) in enumerate(files):
+            op = np.fromfile(os.path.join(td, f"o.npair_{f}"), np.uint32).reshape(-1, 2)
+            od = np.fromfile(os.path.join(td, f"o.ndist_{f}"), np.float32)
+            assert np.array_equal(p.astype(np.uint32), op) and np.array_equal(bits(d), bits(od))
+
+
 def test_python_mirror_matches_cli(ctx, tmp_path):
     say("python mirror")
     fa = tmp_path / "c.fa"

@@ -404,3 +404,15 @@ def test_oracle_preprocess_merge_is_stable_last_wins(tmp_path):
     assert O.preprocess_file(str(fa), o) == 3
     assert open(o, "rb").read() == b"> d\nAC\n> f\nAAAA\n> e\nCCCC\n"
     assert open(o + ".frq", "rb").read() == b"d 1\nf 2\ne 3\n"
+
+
+def test_nw_as_shipped_settings_same_distances():
+    """The timing-only form under genDist's shipped thread settings (threads
+    confined to a few CPUs, static,1 schedule, a critical section per pair)
+    gives the same distances as the plain oracle."""
+    seqs, _, _ = synth.make("reads1k", n=300)
+    r = O.Reads.from_list(seqs)
+    p = np.array([[i, j] for i in range(0, 40) for j in range(i + 1, 80)], np.int32)
+    a, _, _ = O.nw_pairs(r, p, 10)
+    b = O.nw_pairs_as_shipped(r, p, 10, 4, 2)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
