@@ -72,6 +72,7 @@ def load_library():
         "crispy_kmer_set_exact_pass": ([_vp, ctypes.c_int], ctypes.c_int),
         "crispy_nw_autotune": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "crispy_sort_by_dist": ([_vp, _i32p, _f32p, ctypes.c_int64], ctypes.c_int),
+        "crispy_nw_dists_noalloc": ([_vp, _vp, ctypes.c_int64, ctypes.c_int, _vp], ctypes.c_int),
         "crispy_nw_file_split": ([_vp, ctypes.c_int64, ctypes.c_int64, _i64p, ctypes.c_int32,
                                   ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
         "crispy_nw_fetch_sorted": ([_vp, ctypes.c_int64, ctypes.c_int64, _i32p, _f32p], ctypes.c_int),
@@ -358,6 +359,13 @@ class Context:
         self._check(self._L.crispy_nw_dists(self._h, _p(pairs, _i32p), len(pairs), band,
                                             _p(d, _f32p)))
         return d
+
+    def nw_dists_noalloc(self, d_pairs, npairs, band, d_dists):
+        """crispy_nw_dists_noalloc: distances of `npairs` device pairs (int32
+        x2 at address d_pairs) into the float32 device array at d_dists, with
+        no device allocation (the library's fixed per-device scratch)."""
+        self._check(self._L.crispy_nw_dists_noalloc(self._h, ctypes.c_void_p(d_pairs), int(npairs), band,
+                                                    ctypes.c_void_p(d_dists)))
 
     def sort_by_dist(self, pairs, dists):
         pairs = np.array(pairs, dtype=np.int32).reshape(-1, 2)

@@ -34,6 +34,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <mutex>
 
 #include <cstring>
 
@@ -638,6 +639,59 @@ static int sort_dist_ij(crispy_ctx *ctx, const int2 *pairs, const float *dists, 
   return CRISPY_OK;
 }
 
+// ------------------------------------------------------ no-allocation form
+// crispy_nw_dists_noalloc keeps everything beyond its inputs and outputs in
+// `scratch`, one array per device that lives as long as the process.  In host
+// code the array's name is not a device address: its address on a device
+// comes from hipGetSymbolAddress with that device current (noalloc_scratch).
+// Layout, each part 256-byte aligned: bucket keys, sorted keys, pair indices,
+// sorted indices (4 bytes per pair each), the bucket histogram and cell
+// counts, the DP plan, a bad-pair flag, then the radix sort's temporary
+// storage.
+constexpr int64_t kNoAllocMaxPairs = int64_t(1) << 20;
+constexpr size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+constexpr size_t kNaKeys = 0;
+constexpr size_t kNaKeysAlt = kNaKeys + 4 * kNoAllocMaxPairs;
+constexpr size_t kNaIdx = kNaKeysAlt + 4 * kNoAllocMaxPairs;
+constexpr size_t kNaOrder = kNaIdx + 4 * kNoAllocMaxPairs;
+constexpr size_t kNaHist = kNaOrder + 4 * kNoAllocMaxPairs;
+constexpr size_t kNaCells = kNaHist + align256(sizeof(unsigned int) * kNumBuckets);
+constexpr size_t kNaPlan = kNaCells + align256(sizeof(unsigned long long) * kNumBuckets);
+constexpr size_t kNaFlag = kNaPlan + align256(sizeof(NwPlan));
+constexpr size_t kNaTemp = kNaFlag + 256;
+constexpr size_t kNaTempBytes = size_t(24) << 20;
+constexpr size_t kNoAllocScratchBytes = kNaTemp + kNaTempBytes;
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+// Sets *bad when a pair names a read outside [0, n).
+__global__ void nw_check_pairs_kernel(const int2 *__restrict__ pairs, int64_t np, int32_t n,
+                                      unsigned int *__restrict__ bad) {
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    const int2 q = pairs[p];
+    if (q.x < 0 || q.x >= n || q.y < 0 || q.y >= n) *bad = 1u;
+  }
+}
+
+// Device address of `scratch` on `device`, looked up once per device with the
+// device current (the caller has made it current).
+static int noalloc_scratch(crispy_ctx *ctx, unsigned char **out) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static unsigned char *addr[kMaxDevices] = {};
+  if (ctx->device < 0 || ctx->device >= kMaxDevices)
+    return set_err(ctx, CRISPY_ERR_ARG, "device %d outside [0, %d)", ctx->device, kMaxDevices);
+  std::lock_guard<std::mutex> lk(mu);
+  if (!addr[ctx->device]) {
+    void *ptr = nullptr;
+    CRISPY_HIP(ctx, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    addr[ctx->device] = static_cast<unsigned char *>(ptr);
+  }
+  *out = addr[ctx->device];
+  return CRISPY_OK;
+}
+
 void release_graphs(crispy_ctx *ctx) {
   drop_segment(ctx->nw.g_dp);
   drop_segment(ctx->nw.g_filter);
@@ -836,6 +890,72 @@ int crispy_nw_csr(crispy_ctx *ctx, int32_t n_rows, int64_t *row_ptr, int32_t *co
     CRISPY_HIP(ctx, hipMemcpyAsync(cols, nw.csr_cols.p, sizeof(int32_t) * c, hipMemcpyDeviceToHost, ctx->stream));
   if (c > 0 && vals)
     CRISPY_HIP(ctx, hipMemcpyAsync(vals, nw.out_dists.p, sizeof(float) * c, hipMemcpyDeviceToHost, ctx->stream));
+  CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return CRISPY_OK;
+}
+
+int crispy_nw_dists_noalloc(crispy_ctx *ctx, const int32_t *d_pairs, int64_t npairs, int band,
+                            float *d_dists) {
+  if (!ctx || npairs < 0 || (npairs > 0 && (!d_pairs || !d_dists))) return CRISPY_ERR_ARG;
+  if (npairs > kNoAllocMaxPairs)
+    return set_err(ctx, CRISPY_ERR_ARG, "%lld pairs: the no-allocation form takes at most %lld per call",
+                   (long long)npairs, (long long)kNoAllocMaxPairs);
+  ReadStore &rs = ctx->reads[CRISPY_READS_NW];
+  if (rs.n <= 0) return set_err(ctx, CRISPY_ERR_STATE, "no reads in the NW slot");
+  // only the register and lane-sliced kernels: no per-lane DP scratch
+  if (band < 2 || !rs.all_clean || rs.maxlen / band > kKbSliceMax)
+    return set_err(ctx, CRISPY_ERR_ARG,
+                   "the no-allocation form needs band >= 2, ACGT-only reads and kB <= %d (band %d, "
+                   "longest read %d)", kKbSliceMax, band, rs.maxlen);
+  if (npairs == 0) return CRISPY_OK;
+  CRISPY_HIP(ctx, hipSetDevice(ctx->device));
+  unsigned char *base = nullptr;
+  CRISPY_TRY(noalloc_scratch(ctx, &base));
+  uint32_t *keys = reinterpret_cast<uint32_t *>(base + kNaKeys);
+  uint32_t *keys_alt = reinterpret_cast<uint32_t *>(base + kNaKeysAlt);
+  uint32_t *idx = reinterpret_cast<uint32_t *>(base + kNaIdx);
+  uint32_t *order = reinterpret_cast<uint32_t *>(base + kNaOrder);
+  unsigned int *hist = reinterpret_cast<unsigned int *>(base + kNaHist);
+  unsigned long long *cells = reinterpret_cast<unsigned long long *>(base + kNaCells);
+  NwPlan *plan = reinterpret_cast<NwPlan *>(base + kNaPlan);
+  unsigned int *bad = reinterpret_cast<unsigned int *>(base + kNaFlag);
+  void *temp = base + kNaTemp;
+  const int2 *pairs = reinterpret_cast<const int2 *>(d_pairs);
+  const int kb_lo = rs.minlen / band, kb_hi = rs.maxlen / band;
+  const BucketKey bkey = bucket_key(kb_lo, std::min(kb_hi, kKbSliceMax), rs.maxlen);
+  size_t tbytes = 0;
+  CRISPY_HIP(ctx, rocprim::radix_sort_pairs<RadixOnesweep>(nullptr, tbytes, keys, keys_alt, idx, order,
+                                                           (size_t)npairs, 0, bkey.bits, ctx->stream));
+  if (tbytes > kNaTempBytes)
+    return set_err(ctx, CRISPY_ERR_STATE, "radix sort needs %zu bytes of scratch, %zu reserved", tbytes,
+                   kNaTempBytes);
+  // the previous call's contents stay in the array: clear what starts at zero
+  CRISPY_HIP(ctx, hipMemsetAsync(bad, 0, sizeof(unsigned int), ctx->stream));
+  CRISPY_HIP(ctx, hipMemsetAsync(hist, 0, sizeof(unsigned int) * kNumBuckets, ctx->stream));
+  CRISPY_HIP(ctx, hipMemsetAsync(cells, 0, sizeof(unsigned long long) * kNumBuckets, ctx->stream));
+  nw_check_pairs_kernel<<<grid_for(npairs), 256, 0, ctx->stream>>>(pairs, npairs, rs.n, bad);
+  CRISPY_LAUNCHED(ctx);
+  unsigned int *h_bad = reinterpret_cast<unsigned int *>(ctx->h_hist);
+  CRISPY_HIP(ctx, hipMemcpyAsync(h_bad, bad, sizeof(unsigned int), hipMemcpyDeviceToHost, ctx->stream));
+  CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (*h_bad) return set_err(ctx, CRISPY_ERR_ARG, "a pair names a read outside [0, %d)", rs.n);
+  nw_bucket_kernel<<<(int)std::min<int64_t>((npairs + 255) / 256, 1024), 256, 0, ctx->stream>>>(
+      pairs, npairs, rs.len.as<int32_t>(), rs.clean.as<uint8_t>(), band, bkey, keys, idx, hist, cells);
+  CRISPY_LAUNCHED(ctx);
+  CRISPY_HIP(ctx, rocprim::radix_sort_pairs<RadixOnesweep>(temp, tbytes, keys, keys_alt, idx, order,
+                                                           (size_t)npairs, 0, bkey.bits, ctx->stream));
+  const NwFusedArgs args{rs.pack.as<uint32_t>(), rs.poff.as<int64_t>(), rs.len.as<int32_t>(),
+                         pairs, order, plan, d_dists};
+  nw_plan_kernel<<<1, 1024, 0, ctx->stream>>>(hist, cells, ctx->nw.slice_width, plan, nullptr);
+  CRISPY_LAUNCHED(ctx);
+  CRISPY_TRY(fork_workers(ctx));
+  for (int f = 0; f < kNumFamilies; ++f) {
+    const int64_t g = family_grid(f, npairs, kb_lo, kb_hi, ctx->nw.slice_width);
+    if (!g) continue;
+    kFamilyLaunch[f](ctx->workers[f % kNumWorkers], (unsigned)g, args);
+    CRISPY_LAUNCHED(ctx);
+  }
+  CRISPY_TRY(join_workers(ctx));
   CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return CRISPY_OK;
 }

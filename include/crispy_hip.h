@@ -169,6 +169,19 @@ int crispy_nw_autotune(crispy_ctx *ctx, int band, int repeats, int *chosen);
 /* Unfiltered distances for host pairs (every pair gets a distance). */
 int crispy_nw_dists(crispy_ctx *ctx, const int32_t *pairs, int64_t npairs, int band,
                     float *dists);
+/* crispy_nw_dists without any device allocation, for callers that cannot
+ * allocate (tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  It synchronizes its stream (it reads back a
+ * pair-range check), so it is not meant for graph capture.  d_pairs (int32 x2)
+ * and d_dists are device arrays; the reads are those already uploaded to the
+ * context's NW slot.  Bucketing, sort and DP plan use one fixed scratch array
+ * per device (about 40 MB, held until the process ends), so: at most 2^20
+ * pairs per call, band >= 2, ACGT-only reads and kB <= 767 (CRISPY_ERR_ARG
+ * otherwise; such pairs need the generic kernel's per-lane scratch).  Every
+ * call on a device shares that array: calls that overlap on one device race.
+ * Blocks until the distances are written. */
+int crispy_nw_dists_noalloc(crispy_ctx *ctx, const int32_t *d_pairs, int64_t npairs, int band,
+                            float *d_dists);
 /* Sorts host arrays (pairs int32 x2, dists) ascending by (dist, i, j) on the
  * device; the per-file order of writeVectorToFile_GPU (genCUDA.cu:39-45). */
 int crispy_sort_by_dist(crispy_ctx *ctx, int32_t *pairs, float *dists, int64_t n);
