@@ -33,7 +33,7 @@ METRIC = "k-mer pairs/s + banded-NW GCUPS, 1 GPU; served jobs/s with p50/p99 lat
 VALU_PEAK_TOPS = 39.3    # 256 CU x 64 lanes x 2.4 GHz int32 lane-ops/s
 MFMA_FP4_PEAK_TOPS = 10100.0  # MI355X dense FP4 matrix ops/s (10.1 PFLOP/s)
 NW_OPS_PER_CELL = 13     # nominal int ops per in-band DP cell
-KMER_KERNELS = ("kmer_profile", "kmer_tile", "kmer_cand_sort", "kmer_cand", "kmer_sort")
+KMER_KERNELS = ("kmer_profile", "kmer_tile", "kmer_cand_pop", "kmer_cand_sort", "kmer_cand", "kmer_sort")
 NW_KERNELS = ("nw_bucket", "nw_band_reg", "nw_band_sliced", "nw_generic", "nw_filter")
 # the NW launches of one step run concurrently on side streams; this event
 # pair on the main stream brackets all of them (fork to join)
@@ -323,6 +323,7 @@ def run_workload(ca, workload, K_override, steps, warmup, graphs, output_side=Tr
     ctx.sync()
     dt = time.perf_counter() - t0
     res = dict(ctx=ctx, seqs=seqs, cfg=cfg, n=n, K=K, band=band, kthr=kthr, nthr=nthr, kc=kc, nc=nc,
+               tile_bits=ctx.kmer_tile_bits(),
                ms_step=dt / steps * 1e3, nfiles=len(files) if files is not None else None)
     if output_side:
         # the same steps without the output side (file split, sort, copy out)
@@ -412,8 +413,10 @@ def main():
             f"avg {v['avg_ms']:.4f} ms")
 
     kt = kern["kmer_tile"]
-    # the tile is an FP4 matrix product over the k<=6 presence vectors: 2 ops x kdim per pair
-    kdim = 32 * max(16, 4 ** min(K, 6) // 32)
+    # the tile is an FP4 matrix product over the presence vectors (the k <= 6
+    # codes, or their 2,048 hash buckets for the k = 6 candidate filter): 2 ops
+    # x kdim per pair
+    kdim = r["tile_bits"]
     kmer_tile_tops = kt["units"] * 2 * kdim / (kt["total_ms"] * 1e-3) / 1e12 if kt["total_ms"] else 0
     dp_launches = sum(kern[k]["launches"] for k in ("nw_band_reg", "nw_band_sliced", "nw_generic"))
     if phase_ms >= kt["total_ms"] / steps:
@@ -435,7 +438,8 @@ def main():
         roofline = {"kernel": "kmer_tile", "bound": "mfma", "achieved": kmer_tile_tops,
                     "peak": MFMA_FP4_PEAK_TOPS, "unit": "TFLOP/s (FP4 MFMA)",
                     "frac": kmer_tile_tops / MFMA_FP4_PEAK_TOPS,
-                    "work": (f"2 x {kdim} FP4 ops (presence-vector dot product) per read pair, "
+                    "work": (f"2 x {kdim} FP4 ops (presence-vector dot product"
+                             f"{', 4,096 6-mer codes hashed into 2,048 buckets' if kdim == 2048 else ''}) per read pair, "
                              f"{int(kt['units'] / max(1, kt['launches']))} pairs per launch"),
                     "avg_launch_ms": kt["avg_ms"], "traffic": traffic, "traffic_source": src}
 

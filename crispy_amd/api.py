@@ -70,6 +70,7 @@ def load_library():
         "crispy_nw_set_slice_width": ([_vp, ctypes.c_int], ctypes.c_int),
         "crispy_set_graphs": ([_vp, ctypes.c_int], ctypes.c_int),
         "crispy_kmer_set_exact_pass": ([_vp, ctypes.c_int], ctypes.c_int),
+        "crispy_kmer_tile_bits": ([_vp], ctypes.c_int),
         "crispy_nw_autotune": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "crispy_sort_by_dist": ([_vp, _i32p, _f32p, ctypes.c_int64], ctypes.c_int),
         "crispy_nw_dists_noalloc": ([_vp, _vp, ctypes.c_int64, ctypes.c_int, _vp], ctypes.c_int),
@@ -323,6 +324,11 @@ class Context:
     def nw_slice_width(self, cells):
         """Band cells per lane of the lane-sliced NW kernel: 0 = cost model (default), 24 or 32."""
         self._check(self._L.crispy_nw_set_slice_width(self._h, cells))
+
+    def kmer_tile_bits(self):
+        """Presence bits per read the last k-mer tile pass multiplied (4^min(K,6)
+        padded to >= 512, or 2,048 for the hashed k = 6 candidate filter)."""
+        return int(self._L.crispy_kmer_tile_bits(self._h))
 
     def kmer_exact_pass(self, mode):
         """K <= 6: exact counts of the pairs the tile's bound keeps in the tile

@@ -36,6 +36,8 @@ constexpr int kGenericBucket = kKbSliceMax + 1;
 // k-mer tile geometry.
 constexpr int kTile = 128;        // reads per tile side
 constexpr int kChunkWords = 16;   // bitmap words staged in LDS per step
+constexpr int kHashBits = 2048;   // buckets of the hashed k = 6 candidate filter
+constexpr int kHashWords = kHashBits / 32;
 
 // Words of the k-mer presence bitmap of one read (4^K bits), padded to a
 // multiple of kChunkWords.
@@ -104,6 +106,10 @@ struct KmerState {
   int K = 0;
   int profile_gen = -1;
   DevBuf bitmap, T, rcnt, roff, rlist, excess;
+  // k = 6 candidate filter: presence of the 4,096 codes hashed into 2,048
+  // buckets, and T - (distinct buckets); `hashed` when the profile holds them
+  DevBuf hbitmap, hexcess;
+  bool hashed = false;
   // the repeat lists back to back in read order (rlistc + roffc[r])
   DevBuf roffc, rlistc;
   // K = 7, 8: signed-sorted K-mer lists and the k = 6 candidate pairs

@@ -41,30 +41,43 @@
 namespace crispy {
 
 // --------------------------------------------------------------- profiles
+// Bucket of a k = 6 code in the hashed candidate filter (multiplicative hash,
+// top 11 bits).  Merging codes into buckets can only raise
+// sum_b min(count_a(b), count_b(b)), so the filter's bound stays an upper
+// bound of the k-mer matches.
+__host__ __device__ inline uint32_t kmer_bucket(uint32_t code) { return (code * 2654435761u) >> 21; }
+static_assert(kHashBits == 2048, "kmer_bucket yields 11 bits");
+
 // One block per read: the codes of its T = L-K+1 k-mers (first base most
 // significant, as READ::formTuples kmerMain.cpp:117-154) counted in an LDS
 // histogram over all 4^K codes; the bins then give the presence bitmap, the
-// repeat entries in code order and excess = T - (distinct codes).
-template <int K>
+// repeat entries in code order and excess = T - (distinct codes).  HASH
+// (K = 6) also writes the presence of the codes' buckets (kmer_bucket) and
+// hexcess = T - (distinct buckets).
+template <int K, bool HASH = false>
 __global__ __launch_bounds__(256) void kmer_profile_kernel(
     const uint32_t *__restrict__ pack, const int64_t *__restrict__ poff,
     const int32_t *__restrict__ len, uint32_t *__restrict__ bitmap, int32_t *__restrict__ Tout,
     int32_t *__restrict__ rcnt, const int64_t *__restrict__ roff, uint32_t *__restrict__ rlist,
-    int32_t *__restrict__ excess) {
+    int32_t *__restrict__ excess, uint32_t *__restrict__ hbitmap, int32_t *__restrict__ hexcess) {
   constexpr int WP = kmer_words(K);
   constexpr int NBINS = 1 << (2 * K);
   constexpr int BPT = NBINS >= 256 ? NBINS / 256 : 1;  // bins per thread (within one bitmap word)
+  static_assert(!HASH || K == 6, "the hashed filter covers k = 6");
   __shared__ uint32_t hist[NBINS];
   __shared__ uint32_t bm[WP];
+  __shared__ uint32_t hbm[HASH ? kHashWords : 1];
   __shared__ int scan[256];
-  __shared__ int s_distinct;
+  __shared__ int s_distinct, s_hdistinct;
   const int r = blockIdx.x;
   const int tid = threadIdx.x;
   const int T = len[r] - K + 1;  // >= 1, checked by the host
   const uint32_t *s = pack + poff[r];
   for (int v = tid; v < NBINS; v += 256) hist[v] = 0;
   for (int w = tid; w < WP; w += 256) bm[w] = 0;
-  if (tid == 0) s_distinct = 0;
+  if (HASH)
+    for (int w = tid; w < kHashWords; w += 256) hbm[w] = 0;
+  if (tid == 0) s_distinct = s_hdistinct = 0;
   __syncthreads();
   for (int t = tid; t < T; t += 256) {
     uint32_t c = 0;
@@ -87,6 +100,10 @@ __global__ __launch_bounds__(256) void kmer_profile_kernel(
       bits |= (c > 0 ? 1u : 0u) << k;
       ndist += c > 0;
       nrep += c >= 2;
+      if (HASH && c > 0) {
+        const uint32_t h = kmer_bucket((uint32_t)(b0 + k));
+        atomicOr(&hbm[h >> 5], 1u << (h & 31));
+      }
     }
     if (bits) atomicOr(&bm[b0 >> 5], bits << (b0 & 31));
   }
@@ -114,6 +131,15 @@ __global__ __launch_bounds__(256) void kmer_profile_kernel(
   }
   uint32_t *dst = bitmap + (int64_t)r * WP;
   for (int w = tid; w < WP; w += 256) dst[w] = bm[w];
+  if (HASH) {  // the scan's barriers ordered the bucket bits before these reads
+    if (tid < kHashWords) {
+      const uint32_t w = hbm[tid];
+      hbitmap[(int64_t)r * kHashWords + tid] = w;
+      atomicAdd(&s_hdistinct, __popc(w));
+    }
+    __syncthreads();
+    if (tid == 0) hexcess[r] = T - s_hdistinct;
+  }
 }
 
 // sum_v min(e_a[v], e_b[v]) over two repeat lists sorted by code (entries
@@ -258,7 +284,7 @@ __device__ __forceinline__ int wave_incl_sum(int v, int lane) {
   return v;
 }
 
-template <int K, bool CAND>
+template <int WP, bool CAND>
 __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
     const uint32_t *__restrict__ bitmap, const int32_t *__restrict__ Tn,
     const int32_t *__restrict__ rcnt, const int64_t *__restrict__ roff,
@@ -266,8 +292,7 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
     int64_t tile_begin, int64_t ntiles, int64_t per_xcd, float thr, float cut, int tshift, int kshift,
     unsigned long long *__restrict__ counter, uint64_t *__restrict__ out_keys,
     float *__restrict__ out_dists, int64_t cap) {
-  static_assert(K >= 1 && K <= 6, "bitmap tiles hold k <= 6");
-  constexpr int WP = kmer_words(K);
+  static_assert(WP % kChunkWords == 0 && WP <= 128, "bitmap tiles hold at most 4,096 presence bits");
   constexpr int NCH = WP / kChunkWords;
   constexpr int kRow = kChunkWords + 4;  // padded row stride of a staged chunk (words)
   constexpr int kStageWords = 2 * kTile * kRow;
@@ -753,6 +778,29 @@ __global__ __launch_bounds__(256) void kmer_cand_kernel(
   }
 }
 
+// Exact presence popcount of k = 6 candidates that the hashed filter let
+// through (their keys carry the bucket popcount): half a wave per candidate,
+// lane l of the half ANDs words [4l, 4l + 4) of the two reads' 4,096-bit
+// presence bitmaps; the sum replaces the key's popcount field.
+__global__ __launch_bounds__(256) void kmer_cand_popcount_kernel(uint64_t *__restrict__ cand, int64_t ncand,
+                                                                 const uint32_t *__restrict__ bitmap,
+                                                                 int kshift) {
+  constexpr int WP = kmer_words(6);
+  static_assert(WP == 4 * 32, "32 lanes of 4 words cover a k = 6 bitmap");
+  const int l = threadIdx.x & 31;
+  const int64_t halves = (int64_t)gridDim.x * (blockDim.x >> 5);
+  for (int64_t c = (int64_t)blockIdx.x * (blockDim.x >> 5) + (threadIdx.x >> 5); c < ncand; c += halves) {
+    const uint64_t key = cand[c] & kCandKeyMask;
+    const uint32_t i = (uint32_t)(key >> kshift), j = (uint32_t)(key & ((1ull << kshift) - 1));
+    const uint4 a = *reinterpret_cast<const uint4 *>(bitmap + (size_t)i * WP + 4 * l);
+    const uint4 b = *reinterpret_cast<const uint4 *>(bitmap + (size_t)j * WP + 4 * l);
+    int p = __popc(a.x & b.x) + __popc(a.y & b.y) + __popc(a.z & b.z) + __popc(a.w & b.w);
+#pragma unroll
+    for (int off = 16; off > 0; off >>= 1) p += __shfl_xor(p, off, 32);
+    if (l == 0) cand[c] = key | ((uint64_t)p << kCandPopShift);
+  }
+}
+
 // Exact count of the K <= 6 candidates that the tile pass wrote out (pair key
 // plus its presence popcount in the high bits): matches = popcount + the
 // repeat-list merge (the same sum as in the tile's own survivor pass), the
@@ -818,10 +866,20 @@ __global__ __launch_bounds__(256) void compact_repeats_kernel(const int32_t *__r
 template <int K>
 static int launch_profile(crispy_ctx *ctx, ReadStore &rs) {
   KmerState &km = ctx->km;
+  if constexpr (K == 6) {
+    if (km.hashed) {
+      kmer_profile_kernel<6, true><<<rs.n, 256, 0, ctx->stream>>>(
+          rs.pack.as<uint32_t>(), rs.poff.as<int64_t>(), rs.len.as<int32_t>(), km.bitmap.as<uint32_t>(),
+          km.T.as<int32_t>(), km.rcnt.as<int32_t>(), km.roff.as<int64_t>(), km.rlist.as<uint32_t>(),
+          km.excess.as<int32_t>(), km.hbitmap.as<uint32_t>(), km.hexcess.as<int32_t>());
+      CRISPY_LAUNCHED(ctx);
+      return CRISPY_OK;
+    }
+  }
   kmer_profile_kernel<K><<<rs.n, 256, 0, ctx->stream>>>(
       rs.pack.as<uint32_t>(), rs.poff.as<int64_t>(), rs.len.as<int32_t>(), km.bitmap.as<uint32_t>(),
       km.T.as<int32_t>(), km.rcnt.as<int32_t>(), km.roff.as<int64_t>(),
-      km.rlist.as<uint32_t>(), km.excess.as<int32_t>());
+      km.rlist.as<uint32_t>(), km.excess.as<int32_t>(), nullptr, nullptr);
   CRISPY_LAUNCHED(ctx);
   return CRISPY_OK;
 }
@@ -845,11 +903,17 @@ static double tile_units(int32_t n, int nb, int64_t tb, int64_t nt) {
 }
 
 // The tile pass: zero the result counter, then the tile kernel over slots
-// [tb, tb + nt) (a captured graph with graphs on).
-template <int K, bool CAND>
+// [tb, tb + nt) (a captured graph with graphs on).  HASHED: the candidate
+// form on the k = 6 bucket bitmaps (kHashWords words per read) and their
+// excess; otherwise the presence bitmaps of K.
+template <int K, bool CAND, bool HASHED = false>
 static int launch_tiles(crispy_ctx *ctx, int32_t n, int nb, int64_t tb, int64_t nt, float thr,
                         int tshift, uint64_t *out_keys, float *out_dists, int64_t cap) {
+  static_assert(!HASHED || (K == 6 && CAND), "the hashed filter is the k = 6 candidate form");
+  constexpr int WP = HASHED ? kHashWords : kmer_words(K);
   KmerState &km = ctx->km;
+  const DevBuf &bits = HASHED ? km.hbitmap : km.bitmap;
+  const DevBuf &excess = HASHED ? km.hexcess : km.excess;
   const int64_t per_xcd = (nt + 7) / 8;
   const int64_t grid = per_xcd * 8;
   const float cut = 1.0f - thr - 2e-5f;
@@ -859,16 +923,16 @@ static int launch_tiles(crispy_ctx *ctx, int32_t n, int nb, int64_t tb, int64_t 
   timer_begin(ctx, "kmer_tile", &ev);
   CRISPY_TRY(run_segment(
       ctx, km.g_tile,
-      {(uintptr_t)K, (uintptr_t)CAND, (uintptr_t)km.bitmap.p, (uintptr_t)km.T.p, (uintptr_t)km.rcnt.p,
-       (uintptr_t)km.roffc.p, (uintptr_t)km.rlistc.p, (uintptr_t)km.excess.p, (uintptr_t)n, (uintptr_t)nb,
+      {(uintptr_t)WP, (uintptr_t)CAND, (uintptr_t)bits.p, (uintptr_t)km.T.p, (uintptr_t)km.rcnt.p,
+       (uintptr_t)km.roffc.p, (uintptr_t)km.rlistc.p, (uintptr_t)excess.p, (uintptr_t)n, (uintptr_t)nb,
        (uintptr_t)tb, (uintptr_t)nt, thr_bits, (uintptr_t)tshift, (uintptr_t)km.kshift, (uintptr_t)km.counter.p,
        (uintptr_t)out_keys, (uintptr_t)out_dists, (uintptr_t)cap},
       "k-mer tile pass", [&](bool) -> int {
         zero_counter_kernel<<<1, 1, 0, ctx->stream>>>(km.counter.as<unsigned long long>());
         CRISPY_LAUNCHED(ctx);
-        kmer_tile_mfma_kernel<K, CAND><<<(unsigned)grid, 128, 0, ctx->stream>>>(
-            km.bitmap.as<uint32_t>(), km.T.as<int32_t>(), km.rcnt.as<int32_t>(), km.roffc.as<int64_t>(),
-            km.rlistc.as<uint32_t>(), km.excess.as<int32_t>(), n, nb, tb, nt, per_xcd, thr, cut, tshift,
+        kmer_tile_mfma_kernel<WP, CAND><<<(unsigned)grid, 128, 0, ctx->stream>>>(
+            bits.as<uint32_t>(), km.T.as<int32_t>(), km.rcnt.as<int32_t>(), km.roffc.as<int64_t>(),
+            km.rlistc.as<uint32_t>(), excess.as<int32_t>(), n, nb, tb, nt, per_xcd, thr, cut, tshift,
             km.kshift, km.counter.as<unsigned long long>(), out_keys, out_dists, cap);
         CRISPY_LAUNCHED(ctx);
         return CRISPY_OK;
@@ -896,13 +960,14 @@ static const TileFn kTilesCand[7] = {nullptr,
                                      launch_tiles<3, true>,
                                      launch_tiles<4, true>,
                                      launch_tiles<5, true>,
-                                     launch_tiles<6, true>};
+                                     launch_tiles<6, true, true>};
 
 // Profiles of the bitmap word size KP (KP = K for K <= 6, 6 above); every
 // read must hold at least one K-mer.
-static int build_profiles(crispy_ctx *ctx, int KP, int K) {
+static int build_profiles(crispy_ctx *ctx, int KP, int K, bool hash) {
   ReadStore &rs = ctx->reads[CRISPY_READS_KMER];
   KmerState &km = ctx->km;
+  km.hashed = hash && KP == 6;
   // Rebuilt on every run: the per-read profiles are part of the stage's work.
   const int32_t n = rs.n;
   for (int32_t r = 0; r < n; ++r)
@@ -920,6 +985,10 @@ static int build_profiles(crispy_ctx *ctx, int KP, int K) {
   CRISPY_TRY(ensure(ctx, km.T, sizeof(int32_t) * (n + 1)));
   CRISPY_TRY(ensure(ctx, km.rcnt, sizeof(int32_t) * (n + 1)));
   CRISPY_TRY(ensure(ctx, km.excess, sizeof(int32_t) * (n + 1)));
+  if (km.hashed) {
+    CRISPY_TRY(ensure(ctx, km.hbitmap, sizeof(uint32_t) * (size_t)n * kHashWords));
+    CRISPY_TRY(ensure(ctx, km.hexcess, sizeof(int32_t) * (n + 1)));
+  }
   CRISPY_TRY(ensure(ctx, km.roff, sizeof(int64_t) * (n + 1)));
   CRISPY_TRY(ensure(ctx, km.rlist, sizeof(uint32_t) * (roff[n] + 1)));
   CRISPY_HIP(ctx, hipMemcpyAsync(km.roff.p, roff, offs, hipMemcpyHostToDevice, ctx->stream));
@@ -949,7 +1018,8 @@ static int build_profiles(crispy_ctx *ctx, int KP, int K) {
        (uintptr_t)km.bitmap.p, (uintptr_t)km.T.p, (uintptr_t)km.rcnt.p, (uintptr_t)km.roff.p, (uintptr_t)km.rlist.p,
        (uintptr_t)km.excess.p, (uintptr_t)km.roffc.p, (uintptr_t)km.rlistc.p, (uintptr_t)ctx->temp.p,
        (uintptr_t)tbytes, (uintptr_t)(K > 6 ? km.loff.p : nullptr), (uintptr_t)(K > 6 ? km.lists.p : nullptr),
-       (uintptr_t)list_lds},
+       (uintptr_t)list_lds, (uintptr_t)km.hashed, (uintptr_t)(km.hashed ? km.hbitmap.p : nullptr),
+       (uintptr_t)(km.hashed ? km.hexcess.p : nullptr)},
       "k-mer profile pass", [&](bool) -> int {
         CRISPY_TRY(kProfile[KP](ctx, rs));
         size_t tb = tbytes;
@@ -1034,6 +1104,11 @@ extern "C" {
 
 int64_t crispy_kmer_num_tiles(int32_t n) { return num_slots((n + kTile - 1) / kTile); }
 
+int crispy_kmer_tile_bits(crispy_ctx *ctx) {
+  if (!ctx || ctx->km.K == 0) return 0;
+  return ctx->km.hashed ? kHashBits : 32 * kmer_words(ctx->km.K <= 6 ? ctx->km.K : 6);
+}
+
 int crispy_kmer_set_exact_pass(crispy_ctx *ctx, int mode) {
   if (!ctx) return CRISPY_ERR_ARG;
   if (mode < -1 || mode > 1) return set_err(ctx, CRISPY_ERR_ARG, "exact pass mode %d: -1, 0 or 1", mode);
@@ -1071,17 +1146,18 @@ int crispy_kmer_run(crispy_ctx *ctx, int K, float threshold, int64_t tile_begin,
   km.kshift = 1;
   while ((1 << km.kshift) < n) ++km.kshift;
   const int KP = K <= 6 ? K : 6;
-  CRISPY_TRY(build_profiles(ctx, KP, K));
-  // first guess at the result size: the pairs in the range, at most 4 Mi
-  const int64_t pairs_in_range = std::min<int64_t>(nt * kTile * kTile, (int64_t)n * (n - 1) / 2);
-  const int64_t guess = std::min<int64_t>((int64_t)1 << 22, std::max<int64_t>(4096, pairs_in_range));
-  km.cap = std::max(km.cap, guess);
   // K <= 6: exact counts in the tile, or in a pass of their own over the
   // tile's candidates.  Measured on MI355X: the lean candidate tile plus the
   // pass wins from ~10^9 pairs up (reads100k 9.8 -> 9.1 ms, reads250k 50.2
   // -> 44.0 ms per k-mer run); below, the pass's latency-bound merges cost more
-  // than the tile saves (reads20k 0.78 -> 0.97 ms).
+  // than the tile saves (reads20k 0.78 -> 0.97 ms).  The candidate tiles of
+  // k = 6 run on the hashed bitmaps (half the words).
   const bool separate = km.exact_pass == 1 || (km.exact_pass == -1 && (int64_t)n * (n - 1) / 2 >= kExactPassPairs);
+  CRISPY_TRY(build_profiles(ctx, KP, K, K > 6 || separate));
+  // first guess at the result size: the pairs in the range, at most 4 Mi
+  const int64_t pairs_in_range = std::min<int64_t>(nt * kTile * kTile, (int64_t)n * (n - 1) / 2);
+  const int64_t guess = std::min<int64_t>((int64_t)1 << 22, std::max<int64_t>(4096, pairs_in_range));
+  km.cap = std::max(km.cap, guess);
   if (K <= 6 && !separate) {
     CRISPY_TRY(run_counted(ctx, km.cap, km.keys, &km.dists, &km.count, [&]() {
       return kTiles[K](ctx, n, nb, tile_begin, nt, threshold, 0, km.keys.as<uint64_t>(),
@@ -1095,6 +1171,14 @@ int crispy_kmer_run(crispy_ctx *ctx, int K, float threshold, int64_t tile_begin,
                            km.cand_cap);
     }));
     km.candidates = ncand;
+    if (km.hashed && ncand > 0) {  // the exact presence popcount of each candidate
+      hipEvent_t pev;
+      timer_begin(ctx, "kmer_cand_pop", &pev);
+      kmer_cand_popcount_kernel<<<(int)std::min<int64_t>((ncand + 7) / 8, 16384), 256, 0, ctx->stream>>>(
+          km.cand.as<uint64_t>(), ncand, km.bitmap.as<uint32_t>(), km.kshift);
+      CRISPY_LAUNCHED(ctx);
+      timer_end(ctx, "kmer_cand_pop", pev, (double)ncand);
+    }
     CRISPY_TRY(run_counted(ctx, km.cap, km.keys, &km.dists, &km.count, [&]() -> int {
       zero_counter_kernel<<<1, 1, 0, ctx->stream>>>(km.counter.as<unsigned long long>());
       CRISPY_LAUNCHED(ctx);
@@ -1114,7 +1198,7 @@ int crispy_kmer_run(crispy_ctx *ctx, int K, float threshold, int64_t tile_begin,
     km.cand_cap = std::max(km.cand_cap, guess);
     int64_t ncand = 0;
     CRISPY_TRY(run_counted(ctx, km.cand_cap, km.cand, nullptr, &ncand, [&]() {
-      return launch_tiles<6, true>(ctx, n, nb, tile_begin, nt, threshold, K - 6,
+      return launch_tiles<6, true, true>(ctx, n, nb, tile_begin, nt, threshold, K - 6,
                                    km.cand.as<uint64_t>(), nullptr, km.cand_cap);
     }));
     km.candidates = ncand;

@@ -118,6 +118,12 @@ int64_t crispy_kmer_num_tiles(int32_t num_reads);
  * form from 10^9 read pairs up, the first below.  Results do not depend on
  * it.  No counterpart in the reference: a tuning and test switch. */
 int crispy_kmer_set_exact_pass(crispy_ctx *ctx, int mode);
+/* Presence bits per read that the last crispy_kmer_run's tile pass multiplied
+ * (FP4 k-slots per pair): 4^min(K, 6) padded to at least 512, or 2,048 when
+ * the k = 6 candidate filter ran on bucket-hashed bitmaps (K > 6, or K = 6 with
+ * the exact counts in a pass of their own); 0 before any run.  No counterpart
+ * in the reference (measurement). */
+int crispy_kmer_tile_bits(crispy_ctx *ctx);
 /* Copies result entries [first, first+count) to host: pairs as int32 (i, j),
  * dists as float32 (either pointer may be NULL). */
 int crispy_kmer_fetch(crispy_ctx *ctx, int64_t first, int64_t count, int32_t *pairs,

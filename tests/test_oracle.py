@@ -416,3 +416,53 @@ def test_nw_as_shipped_settings_same_distances():
     a, _, _ = O.nw_pairs(r, p, 10)
     b = O.nw_pairs_as_shipped(r, p, 10, 4, 2)
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def _bucket(codes):
+    """The k = 6 candidate filter's bucket of a code: the top 11 bits of
+    code * 2654435761 mod 2^32 (kmer_bucket in crispy_amd/csrc/kmer.hip)."""
+    return ((codes.astype(np.uint64) * 2654435761) & 0xFFFFFFFF) >> 21
+
+
+@pytest.mark.parametrize("K", [6, 7, 8])
+def test_hashed_filter_bound_is_sound(K):
+    """The hashed k = 6 candidate filter may keep extra pairs but must never
+    drop one: for every pair, popcount(bucket presence AND) + min(T - distinct
+    buckets) >= the oracle's exact K-mer matches (kmerCPU.cpp:29-43), since
+    merging codes into buckets only raises sum_b min(count_i, count_j) and
+    K-mer matches never exceed the 6-mer multiset intersection.  Brute force
+    over random, low-complexity, T/C-rich and near-copy reads."""
+    rng = np.random.default_rng(600 + K)
+    seqs = []
+    alph = np.frombuffer(b"ACGT", np.uint8)
+    base = alph[rng.integers(0, 4, 900)]
+    for k in range(90):
+        L = int(rng.integers(K + 2, 900))
+        kind = k % 4
+        if kind == 0:
+            s = np.resize(alph[rng.integers(0, 4, int(rng.integers(1, 6)))], L)
+        elif kind == 1:
+            s = np.frombuffer(b"TC", np.uint8)[rng.integers(0, 2, L)]
+        elif kind == 2:
+            s = base[:L].copy()
+            for _ in range(L // 50):
+                s[int(rng.integers(0, L))] = alph[int(rng.integers(0, 4))]
+        else:
+            s = alph[rng.integers(0, 4, L)]
+        seqs.append(s.tobytes())
+    reads = O.Reads.from_list(seqs)
+    n = reads.n
+    presence, hexcess = [], []
+    for s in seqs:
+        c = O.kmer_tuples(s, 6).astype(np.int64)
+        b = np.unique(_bucket(c))
+        presence.append(set(b.tolist()))
+        hexcess.append(len(c) - len(b))
+    pairs = np.array([[i, j] for i in range(n) for j in range(i + 1, n)], np.int32)
+    _, matches = O.kmer_pairs(reads, K, pairs)
+    tight = 0
+    for (i, j), m in zip(pairs, matches):
+        bound = len(presence[i] & presence[j]) + min(hexcess[i], hexcess[j])
+        assert bound >= m, (i, j, bound, m)
+        tight += bound == m
+    assert tight > 0  # the bound is attained (identical or near-identical reads)
