@@ -113,7 +113,7 @@ struct KmerState {
   // the repeat lists back to back in read order (rlistc + roffc[r])
   DevBuf roffc, rlistc;
   // K = 7, 8: signed-sorted K-mer lists and the k = 6 candidate pairs
-  DevBuf loff, lists, cand, cand_alt;
+  DevBuf loff, lists, cand, cand_alt, cand_n;
   int kshift = 20;  // result keys are i << kshift | j
   int exact_pass = -1;  // K <= 6 exact counts: -1 auto (by pair count), 0 in the tile, 1 a pass of their own
   int64_t cand_cap = 0;

@@ -319,9 +319,11 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
     const int r = q < kTile ? row0 + q : col0 + (q - kTile);
     const bool ok = r < n;
     const int T = ok ? Tn[r] : 1;
-    sT[q] = T;
-    sRc[q] = ok ? rcnt[r] : 0;
-    sRo[q] = ok ? roff[r] : 0;
+    if (!CAND) {  // the candidate form needs only the bound
+      sT[q] = T;
+      sRc[q] = ok ? rcnt[r] : 0;
+      sRo[q] = ok ? roff[r] : 0;
+    }
     // G = F - E (exact in fp32 when positive: E is an integer below F), >= 0
     const float G = fmaf((float)(T - tshift), cut, -1.0f) - (float)(ok ? excess[r] : 0);
     sG[q] = ok && G > 0.0f ? G : 0.0f;
@@ -778,27 +780,63 @@ __global__ __launch_bounds__(256) void kmer_cand_kernel(
   }
 }
 
-// Exact presence popcount of k = 6 candidates that the hashed filter let
-// through (their keys carry the bucket popcount): half a wave per candidate,
-// lane l of the half ANDs words [4l, 4l + 4) of the two reads' 4,096-bit
-// presence bitmaps; the sum replaces the key's popcount field.
-__global__ __launch_bounds__(256) void kmer_cand_popcount_kernel(uint64_t *__restrict__ cand, int64_t ncand,
-                                                                 const uint32_t *__restrict__ bitmap,
-                                                                 int kshift) {
+// Refines the k = 6 candidates that the hashed filter let through (their keys
+// carry the bucket popcount): half a wave per candidate, lane l of the half
+// ANDs words [4l, 4l + 4) of the two reads' 4,096-bit presence bitmaps for
+// the exact presence popcount p, and the tile's exact-bitmap bound
+// p + min(excess_i, excess_j) >= (T_min * cut - 1) keeps the candidates that
+// might pass; those are queued per wave in LDS and appended to `out` with p
+// in the popcount field (one atomic on *nout per up to 256 of them: a
+// same-address atomic per wave step serialises across the chip).
+constexpr int kRefineQueue = 256;
+__global__ __launch_bounds__(256) void kmer_cand_refine_kernel(
+    const uint64_t *__restrict__ cand, int64_t ncand, const uint32_t *__restrict__ bitmap,
+    const int32_t *__restrict__ Tn, const int32_t *__restrict__ excess, float cut, int kshift,
+    unsigned long long *__restrict__ nout, uint64_t *__restrict__ out) {
   constexpr int WP = kmer_words(6);
   static_assert(WP == 4 * 32, "32 lanes of 4 words cover a k = 6 bitmap");
-  const int l = threadIdx.x & 31;
-  const int64_t halves = (int64_t)gridDim.x * (blockDim.x >> 5);
-  for (int64_t c = (int64_t)blockIdx.x * (blockDim.x >> 5) + (threadIdx.x >> 5); c < ncand; c += halves) {
-    const uint64_t key = cand[c] & kCandKeyMask;
-    const uint32_t i = (uint32_t)(key >> kshift), j = (uint32_t)(key & ((1ull << kshift) - 1));
-    const uint4 a = *reinterpret_cast<const uint4 *>(bitmap + (size_t)i * WP + 4 * l);
-    const uint4 b = *reinterpret_cast<const uint4 *>(bitmap + (size_t)j * WP + 4 * l);
-    int p = __popc(a.x & b.x) + __popc(a.y & b.y) + __popc(a.z & b.z) + __popc(a.w & b.w);
+  __shared__ uint64_t sq[4][kRefineQueue];
+  const int lane = threadIdx.x & 63, h = lane >> 5, l = lane & 31, wv = threadIdx.x >> 6;
+  uint64_t *q = sq[wv];
+  int qn = 0;  // wave-uniform
+  auto flush = [&]() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(nout, (unsigned long long)qn);
+    base = __shfl(base, 0, 64);
+    for (int e = lane; e < qn; e += 64) out[base + e] = q[e];
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    qn = 0;
+  };
+  const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+  const int64_t w0 = (int64_t)blockIdx.x * (blockDim.x >> 6) + wv;
+  for (int64_t cb = 2 * w0; cb < ncand; cb += 2 * waves) {  // wave-uniform
+    const int64_t c = cb + h;
+    const bool valid = c < ncand;
+    uint64_t key = 0;
+    uint32_t i = 0, j = 0;
+    int p = 0;
+    if (valid) {
+      key = cand[c] & kCandKeyMask;
+      i = (uint32_t)(key >> kshift);
+      j = (uint32_t)(key & ((1ull << kshift) - 1));
+      const uint4 a = *reinterpret_cast<const uint4 *>(bitmap + (size_t)i * WP + 4 * l);
+      const uint4 b = *reinterpret_cast<const uint4 *>(bitmap + (size_t)j * WP + 4 * l);
+      p = __popc(a.x & b.x) + __popc(a.y & b.y) + __popc(a.z & b.z) + __popc(a.w & b.w);
+    }
 #pragma unroll
     for (int off = 16; off > 0; off >>= 1) p += __shfl_xor(p, off, 32);
-    if (l == 0) cand[c] = key | ((uint64_t)p << kCandPopShift);
+    bool keep = false;
+    if (valid && l == 0) {
+      const float F = fmaf((float)min(Tn[i], Tn[j]), cut, -1.0f);
+      keep = (float)(p + min(excess[i], excess[j])) >= F;
+    }
+    const uint64_t ball = __ballot(keep);
+    if (keep) q[qn + __popcll(ball & ((1ull << lane) - 1))] = key | ((uint64_t)p << kCandPopShift);
+    qn += __popcll(ball);
+    if (qn > kRefineQueue - 2) flush();
   }
+  if (qn > 0) flush();
 }
 
 // Exact count of the K <= 6 candidates that the tile pass wrote out (pair key
@@ -806,10 +844,13 @@ __global__ __launch_bounds__(256) void kmer_cand_popcount_kernel(uint64_t *__res
 // repeat-list merge (the same sum as in the tile's own survivor pass), the
 // reference distance and predicate, the passing pairs appended.
 __global__ __launch_bounds__(256) void kmer_exact_kernel(
-    const uint64_t *__restrict__ cand, int64_t ncand, const int32_t *__restrict__ Tn,
+    const uint64_t *__restrict__ cand, int64_t ncand_max, const unsigned long long *__restrict__ ncand_dev,
+    const int32_t *__restrict__ Tn,
     const int32_t *__restrict__ rcnt, const int64_t *__restrict__ roffc, const uint32_t *__restrict__ rlistc,
     float thr, int kshift, unsigned long long *__restrict__ counter, uint64_t *__restrict__ out_keys,
     float *__restrict__ out_dists, int64_t cap) {
+  // the candidate count: ncand_max, or the refine pass's count on the device
+  const int64_t ncand = ncand_dev ? (int64_t)min((unsigned long long)ncand_max, *ncand_dev) : ncand_max;
   const int lane = threadIdx.x & 63;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t base0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63);
@@ -1171,13 +1212,25 @@ int crispy_kmer_run(crispy_ctx *ctx, int K, float threshold, int64_t tile_begin,
                            km.cand_cap);
     }));
     km.candidates = ncand;
-    if (km.hashed && ncand > 0) {  // the exact presence popcount of each candidate
+    // hashed filter: the exact presence popcount and the exact-bitmap bound
+    // first, so that only the candidates the tile would have kept with the
+    // exact bitmaps go on to the repeat-list merge (their count stays on the
+    // device)
+    const uint64_t *exact_in = km.cand.as<uint64_t>();
+    const unsigned long long *exact_n = nullptr;
+    if (km.hashed && ncand > 0) {
+      CRISPY_TRY(ensure(ctx, km.cand_alt, sizeof(uint64_t) * ncand));
+      CRISPY_TRY(ensure(ctx, km.cand_n, sizeof(unsigned long long)));
       hipEvent_t pev;
       timer_begin(ctx, "kmer_cand_pop", &pev);
-      kmer_cand_popcount_kernel<<<(int)std::min<int64_t>((ncand + 7) / 8, 16384), 256, 0, ctx->stream>>>(
-          km.cand.as<uint64_t>(), ncand, km.bitmap.as<uint32_t>(), km.kshift);
+      zero_counter_kernel<<<1, 1, 0, ctx->stream>>>(km.cand_n.as<unsigned long long>());
+      kmer_cand_refine_kernel<<<(int)std::min<int64_t>((ncand + 7) / 8, 2048), 256, 0, ctx->stream>>>(
+          km.cand.as<uint64_t>(), ncand, km.bitmap.as<uint32_t>(), km.T.as<int32_t>(), km.excess.as<int32_t>(),
+          1.0f - threshold - 2e-5f, km.kshift, km.cand_n.as<unsigned long long>(), km.cand_alt.as<uint64_t>());
       CRISPY_LAUNCHED(ctx);
       timer_end(ctx, "kmer_cand_pop", pev, (double)ncand);
+      exact_in = km.cand_alt.as<uint64_t>();
+      exact_n = km.cand_n.as<unsigned long long>();
     }
     CRISPY_TRY(run_counted(ctx, km.cap, km.keys, &km.dists, &km.count, [&]() -> int {
       zero_counter_kernel<<<1, 1, 0, ctx->stream>>>(km.counter.as<unsigned long long>());
@@ -1187,7 +1240,7 @@ int crispy_kmer_run(crispy_ctx *ctx, int K, float threshold, int64_t tile_begin,
       timer_begin(ctx, "kmer_cand", &ev);
       const int blocks = (int)std::min<int64_t>((ncand + 255) / 256, 8192);
       kmer_exact_kernel<<<blocks, 256, 0, ctx->stream>>>(
-          km.cand.as<uint64_t>(), ncand, km.T.as<int32_t>(), km.rcnt.as<int32_t>(), km.roffc.as<int64_t>(),
+          exact_in, ncand, exact_n, km.T.as<int32_t>(), km.rcnt.as<int32_t>(), km.roffc.as<int64_t>(),
           km.rlistc.as<uint32_t>(), threshold, km.kshift, km.counter.as<unsigned long long>(),
           km.keys.as<uint64_t>(), km.dists.as<float>(), km.cap);
       CRISPY_LAUNCHED(ctx);

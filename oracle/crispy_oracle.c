@@ -329,6 +329,42 @@ int64_t oracle_kmer_all(const char *bases, const int64_t *offs, int32_t n,
     return count;
 }
 
+/* oracle_kmer_all over an explicit, ascending list of rows i (each against
+ * every j > i); for sampling large inputs in the tests. */
+int64_t oracle_kmer_rowlist(const char *bases, const int64_t *offs, int32_t n, int K, float thr,
+                            int nthreads, const int32_t *rows, int32_t nrows, int32_t *out_pairs,
+                            float *out_dists, int64_t cap) {
+    int64_t *toff;
+    uint16_t *tup = or_build_tuples(bases, offs, n, K, &toff);
+    if (!tup) return -1;
+    int64_t count = 0;
+    if (nthreads < 1) nthreads = 1;
+    for (int32_t r = 0; r < nrows; ++r) {
+        const int32_t i = rows[r];
+        if (i < 0 || i >= n) continue;
+        #pragma omp parallel num_threads(nthreads)
+        {
+            #pragma omp for nowait schedule(static, 1)
+            for (int32_t j = i + 1; j < n; ++j) {
+                float dist = oracle_kmer_pair(tup + toff[i], (int)(toff[i + 1] - toff[i]),
+                                              tup + toff[j], (int)(toff[j + 1] - toff[j]), NULL);
+                #pragma omp critical
+                if (oracle_pass(dist, thr)) {
+                    if (count < cap) {
+                        out_pairs[2 * count] = i;
+                        out_pairs[2 * count + 1] = j;
+                        if (out_dists) out_dists[count] = dist;
+                    }
+                    ++count;
+                }
+            }
+        }
+    }
+    free(tup);
+    free(toff);
+    return count;
+}
+
 /* Distances (no filter) for an explicit pair list; used by the tests. */
 int oracle_kmer_pairs(const char *bases, const int64_t *offs, int32_t n, int K,
                       const int32_t *pairs, int64_t npairs, float *out_dists,

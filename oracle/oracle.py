@@ -49,6 +49,10 @@ def lib():
                                       ctypes.c_float, ctypes.c_int, ctypes.c_int32,
                                       ctypes.c_int32, _i32p, _f32p, ctypes.c_int64]
         L.oracle_kmer_all.restype = ctypes.c_int64
+        L.oracle_kmer_rowlist.argtypes = [ctypes.c_char_p, _i64p, ctypes.c_int32, ctypes.c_int,
+                                          ctypes.c_float, ctypes.c_int, _i32p, ctypes.c_int32,
+                                          _i32p, _f32p, ctypes.c_int64]
+        L.oracle_kmer_rowlist.restype = ctypes.c_int64
         L.oracle_kmer_pairs.argtypes = [ctypes.c_char_p, _i64p, ctypes.c_int32, ctypes.c_int,
                                         _i32p, ctypes.c_int64, _f32p, _i32p]
         L.oracle_nw_pair.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p,
@@ -156,6 +160,25 @@ def kmer_all(reads: Reads, K: int, thr: float, nthreads=1, rows=None, cap=None):
         raise ValueError("read shorter than K")
     cnt = min(cnt, cap)
     return pairs[:cnt], dists[:cnt]
+
+
+def kmer_rows(reads: Reads, K: int, thr: float, rows, nthreads=None, cap=None):
+    """kmer_all restricted to the listed rows i (each against every j > i)."""
+    rows = np.ascontiguousarray(np.unique(np.asarray(rows, np.int32)))
+    if nthreads is None:
+        nthreads = lib().oracle_num_threads()
+    if cap is None:
+        cap = min(int(sum(reads.n - 1 - int(i) for i in rows)), 1 << 22)
+    while True:
+        out = np.zeros((max(cap, 1), 2), np.int32)
+        d = np.zeros(max(cap, 1), np.float32)
+        c = lib().oracle_kmer_rowlist(reads.bases, _p(reads.offsets, _i64p), reads.n, K, thr, nthreads,
+                                      _p(rows, _i32p), len(rows), _p(out, _i32p), _p(d, _f32p), cap)
+        if c < 0:
+            raise ValueError("read shorter than K")
+        if c <= cap:
+            return out[:c], d[:c]
+        cap = int(c)
 
 
 def kmer_pairs(reads: Reads, K: int, pairs: np.ndarray):

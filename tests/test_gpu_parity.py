@@ -575,12 +575,13 @@ def test_python_mirror_matches_cli(ctx, tmp_path):
         assert open(tmp_path / f"py.{ext}", "rb").read() == open(f"{fa}.{ext}", "rb").read()
 
 
-@pytest.mark.parametrize("workload,K,rows", [("reads20k", 6, 40), ("reads100k", 6, 24), ("reads100k", 8, 16),
-                                           ("reads250k", 6, 12)])
+@pytest.mark.parametrize("workload,K,rows", [("reads20k", 6, 256), ("reads100k", 6, 256), ("reads100k", 8, 256),
+                                           ("reads250k", 6, 256)])
 def test_full_size(ctx, workload, K, rows):
     """The benchmark workloads at full size (20k, 100k and 250k reads): exact
-    against the oracle on the first and the last `rows` rows of the k-mer
-    matrix and on a sample of NW pairs, invariants of the whole output (i < j,
+    against the oracle on `rows` seeded random rows of the k-mer matrix spread
+    over [0, n) plus its first and last four rows (edge tiles), and on a
+    4,000-pair sample of NW pairs; invariants of the whole output (i < j,
     (i, j) order, thresholds, NW pairs a subset of the k-mer pairs), and the
     same output from a second run."""
     say(f"{workload} full size, k={K}")
@@ -600,11 +601,13 @@ def test_full_size(ctx, workload, K, rows):
     key = p[:, 0].astype(np.int64) * r.n + p[:, 1]
     assert np.all(np.diff(key) > 0)
     assert all(O.passes(float(x), kthr) for x in d[:: max(1, len(d) // 5000)])
-    for r0, r1 in ((0, rows), (r.n - rows, r.n)):
-        op, od = O.kmer_all(ro, K, kthr, nthreads=O.num_threads(), rows=(r0, r1))
-        sel = (p[:, 0] >= r0) & (p[:, 0] < r1)
-        say(f"rows [{r0}, {r1}): {int(sel.sum())} pairs")
-        assert np.array_equal(canon(p[sel], d[sel]), canon(op, od))
+    rng = np.random.default_rng(1000 + K + r.n)
+    sample_rows = np.unique(np.concatenate([rng.choice(r.n - 1, size=rows, replace=False),
+                                            [0, 1, 2, 3, r.n - 5, r.n - 4, r.n - 3, r.n - 2]]))
+    op, od = O.kmer_rows(ro, K, kthr, sample_rows)
+    sel = np.isin(p[:, 0], sample_rows)
+    say(f"{len(sample_rows)} sampled rows: {int(sel.sum())} pairs")
+    assert np.array_equal(canon(p[sel], d[sel]), canon(op, od))
     nc = ctx.nw_run_on_kmer(band, nthr)
     np_, nd = ctx.nw_fetch()
     say("nw pairs", nc)
@@ -614,7 +617,7 @@ def test_full_size(ctx, workload, K, rows):
     assert all(O.passes(float(x), nthr) for x in nd[:: max(1, len(nd) // 5000)])
     # a sample agrees with the oracle, kept exactly when it passes
     rng = np.random.default_rng(9)
-    samp = rng.choice(len(p), size=min(2000, len(p)), replace=False)
+    samp = rng.choice(len(p), size=min(4000, len(p)), replace=False)
     od, _, _ = O.nw_pairs(ro, p[samp], band)
     full = ctx.nw_dists(p[samp], band)
     assert np.array_equal(bits(full), bits(od))
@@ -631,3 +634,80 @@ def test_full_size(ctx, workload, K, rows):
     np2, nd2 = ctx.nw_fetch()
     assert np.array_equal(p2, p) and np.array_equal(bits(d2), bits(d))
     assert np.array_equal(np2, np_) and np.array_equal(bits(nd2), bits(nd))
+
+
+REF_BIN = os.path.join(ROOT, "oracle", "_ref")
+
+
+def _partitions(path):
+    """sparsecut's .Cluster as {cutoff: sorted clusters} (its cluster order
+    follows an unordered_set of tree nodes, so only the partition is defined)."""
+    out = {}
+    for line in open(path):
+        f = line.strip().split("|")
+        if f[0]:
+            out[f[0].strip()] = sorted(tuple(sorted(int(x) for x in c.split())) for c in f[1:] if c.strip())
+    return out
+
+
+def _consumers(cwd, name, n, nfiles):
+    """crispy.sh:22-29 after the two distance stages: aveclust, then sparsecut
+    at fixed cutoffs (the R changepoint step that picks them is not run)."""
+    run([os.path.join(REF_BIN, "aveclust"), "-i", name, "-n", str(n), "-f", str(nfiles)], cwd)
+    with open(os.path.join(cwd, "cut.txt"), "w") as f:
+        f.write("0.03\n0.05\n")
+    run([os.path.join(REF_BIN, "sparsecut"), "-i", name, "-n", str(n), "-c", "cut.txt"], cwd)
+    return open(os.path.join(cwd, name + "_Merge"), "rb").read(), _partitions(os.path.join(cwd, name + ".Cluster"))
+
+
+@pytest.mark.skipif(not all(os.access(os.path.join(REF_BIN, b), os.X_OK) for b in ("aveclust", "sparsecut")),
+                    reason="the reference aveclust / sparsecut (built from its sources by make -C oracle ref) "
+                           "are not in oracle/_ref")
+def test_dropin_consumers(tmp_path):
+    """The drop-in loop of crispy.sh:18-29 on the shipped sample: bin/preprocess,
+    bin/kmerDist -k 6 and bin/genDist -b 10, then the reference's own aveclust
+    (k-way merge of the per-file-sorted .npair_f/.ndist_f, aveclust.cpp:747-802)
+    and sparsecut.  The consumers produce the same _Merge (byte for byte) and
+    the same clusters at each cutoff as on the oracle's genDist files.  Equal
+    distances inside a file may come in any order from the reference's unstable
+    sort: with them shuffled, the clusters at the cutoffs are still the same
+    (the dendrogram's merge order, and so _Merge, can change)."""
+    say("drop-in consumers")
+    work = tmp_path / "gpu"
+    work.mkdir()
+    raw = work / "sample"
+    raw.write_bytes(open(os.path.join(GOLD, "V2Mice_sample1000"), "rb").read())
+    r = subprocess.run([os.path.join(BIN, "preprocess"), "-i", str(raw)], cwd=work, capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 1, r.stdout[-2000:]  # the reference preprocess returns 1 on success
+    name = "sample_Clean"
+    run([os.path.join(BIN, "kmerDist"), "-i", name, "-k", "6"], work)
+    run([os.path.join(BIN, "genDist"), "-i", name, "-b", "10"], work)
+    nfiles = len([f for f in os.listdir(work) if f.startswith(name + ".ndist_")])
+    n = sum(1 for line in open(work / name, "rb") if line.startswith(b">"))
+    merge, parts = _consumers(str(work), name, n, nfiles)
+    # the same consumers on the oracle's genDist files of the same reads
+    owork = tmp_path / "oracle"
+    owork.mkdir()
+    (owork / name).write_bytes((work / name).read_bytes())
+    ro, go = O.read_fasta(str(owork / name), "kmer"), O.read_fasta(str(owork / name), "gen")
+    kp, _ = O.kmer_all(ro, 6, O.kmer_threshold(ro.n), nthreads=O.num_threads())
+    kp = kp[np.lexsort((kp[:, 1], kp[:, 0]))]
+    onf, _ = O.gendist_files(go, kp, 10, O.nw_threshold(go.n), str(owork / name))
+    assert onf == nfiles
+    omerge, oparts = _consumers(str(owork), name, n, nfiles)
+    assert merge == omerge
+    assert parts == oparts and len(parts) == 2
+    # another tie order inside each file: the same clusters
+    swork = tmp_path / "shuffled"
+    swork.mkdir()
+    (swork / name).write_bytes((work / name).read_bytes())
+    rng = np.random.default_rng(3)
+    for f in range(nfiles):
+        p = np.fromfile(work / f"{name}.npair_{f}", np.uint32).reshape(-1, 2)
+        d = np.fromfile(work / f"{name}.ndist_{f}", np.float32)
+        key = np.lexsort((rng.random(len(d)), d.view(np.uint32)))
+        p[key].tofile(swork / f"{name}.npair_{f}")
+        d[key].tofile(swork / f"{name}.ndist_{f}")
+    _, sparts = _consumers(str(swork), name, n, nfiles)
+    assert sparts == parts

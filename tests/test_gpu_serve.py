@@ -58,6 +58,34 @@ def solo(jobs):
     return [alone(j) for j in jobs]
 
 
+def test_large_served_job_matches_oracle(jobs):
+    """A 20,000-read job of the served workload's length law (the top of its
+    500-20,000 read range), served while 5 other jobs run, against the oracle
+    on 128 seeded random rows of its k-mer matrix plus its first and last rows:
+    its NW pairs in those rows, and their distances, are the oracle's."""
+    big = make_job(20000, 1999)
+    r = big["reads"]
+    with ca.Server(0, 4, 64 << 30) as srv:
+        others = [submit(srv, j) for j in jobs[1:6]]
+        jid = submit(srv, big)
+        p, d = srv.fetch(jid)
+        for o in others:
+            srv.fetch(o)
+    ro = O.Reads(r.bases, r.offsets)
+    rng = np.random.default_rng(77)
+    rows = np.unique(np.concatenate([rng.choice(r.n - 1, 128, replace=False), [0, r.n - 2]]))
+    kp, _ = O.kmer_rows(ro, big["K"], big["kthr"], rows)
+    od, _, _ = O.nw_pairs(ro, kp, big["band"])
+    keep = np.array([O.passes(float(x), big["nthr"]) for x in od], bool)
+    want_p = np.asarray(kp, np.int32).reshape(-1, 2)[keep]
+    want_d = np.asarray(od, np.float32)[keep]
+    order = np.lexsort((want_p[:, 1], want_p[:, 0]))
+    sel = np.isin(p[:, 0], rows)
+    say(f"served 20k job: {len(d)} NW pairs, {int(sel.sum())} in {len(rows)} sampled rows")
+    assert int(sel.sum()) > 0
+    assert same((want_p[order], want_d[order]), (p[sel], d[sel]))
+
+
 def test_small_job_matches_oracle(jobs, solo):
     job, (p, d) = jobs[0], solo[0]
     r = job["reads"]
