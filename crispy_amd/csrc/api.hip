@@ -225,7 +225,9 @@ static void release_all(crispy_ctx *ctx) {
                     &ctx->pre.keys_alt, &ctx->pre.flags,   &ctx->pre.sel,      &ctx->pre.nsel,
                     &ctx->pre.csum,    &ctx->pre.out_order, &ctx->pre.out_freq, &ctx->pre.primer, &ctx->pre.scratch,
                     &ctx->km.cand_alt,  &ctx->nw.csr_rows,
-                    &ctx->nw.csr_cols};
+                    &ctx->nw.csr_cols,  &ctx->km.hbitmap,   &ctx->km.hexcess,   &ctx->km.cand_n,
+                    &ctx->km.seg,       &ctx->km.seg_stat,  &ctx->nw.sort_k0,   &ctx->nw.sort_k1,
+                    &ctx->nw.sort_b0,   &ctx->nw.sort_b1,   &ctx->nw.sort_pairs, &ctx->nw.sort_dists};
   for (DevBuf *b : bufs) release(ctx, *b);
   for (auto &r : ctx->reads) {
     DevBuf *rb[] = {&r.bases, &r.off, &r.len, &r.pack, &r.poff, &r.clean};

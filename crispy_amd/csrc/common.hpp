@@ -38,6 +38,12 @@ constexpr int kTile = 128;        // reads per tile side
 constexpr int kChunkWords = 16;   // bitmap words staged in LDS per step
 constexpr int kHashBits = 2048;   // buckets of the hashed k = 6 candidate filter
 constexpr int kHashWords = kHashBits / 32;
+// The candidate tiles append to kCandSegs output segments, each with its own
+// counter kSegStride u64 words (4 KB) from the next: one counter for the
+// whole chip serialises the per-wave reservations (~600k atomics per
+// 10^5-read run), separate counters on separate lines spread them.
+constexpr int kCandSegs = 256;
+constexpr int kSegStride = 512;
 
 // Words of the k-mer presence bitmap of one read (4^K bits), padded to a
 // multiple of kChunkWords.
@@ -114,9 +120,12 @@ struct KmerState {
   DevBuf roffc, rlistc;
   // K = 7, 8: signed-sorted K-mer lists and the k = 6 candidate pairs
   DevBuf loff, lists, cand, cand_alt, cand_n;
+  // candidate segments: counters (kCandSegs x kSegStride u64) and their
+  // total and largest count (2 u64); cand holds kCandSegs x cand_seg_cap keys
+  DevBuf seg, seg_stat;
   int kshift = 20;  // result keys are i << kshift | j
   int exact_pass = -1;  // K <= 6 exact counts: -1 auto (by pair count), 0 in the tile, 1 a pass of their own
-  int64_t cand_cap = 0;
+  int64_t cand_seg_cap = 0;
   int64_t candidates = 0;
   // result
   DevBuf keys, keys_alt, dists, dists_alt, pairs, counter;

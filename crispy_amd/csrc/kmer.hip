@@ -306,6 +306,11 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
   __shared__ uint32_t sQ[2][1024];  // per-wave survivor queue
 
   const int64_t bid = blockIdx.x;
+  if (CAND) {  // this block's output segment and its counter; cap is a segment's room
+    const int seg = (int)(bid & (kCandSegs - 1));
+    counter += (size_t)seg * kSegStride;
+    out_keys += (size_t)seg * cap;
+  }
   const int64_t local = (bid & 7) * per_xcd + (bid >> 3);
   if (local >= ntiles) return;
   int bi, bj;
@@ -780,19 +785,22 @@ __global__ __launch_bounds__(256) void kmer_cand_kernel(
   }
 }
 
-// Refines the k = 6 candidates that the hashed filter let through (their keys
-// carry the bucket popcount): half a wave per candidate, lane l of the half
-// ANDs words [4l, 4l + 4) of the two reads' 4,096-bit presence bitmaps for
-// the exact presence popcount p, and the tile's exact-bitmap bound
-// p + min(excess_i, excess_j) >= (T_min * cut - 1) keeps the candidates that
-// might pass; those are queued per wave in LDS and appended to `out` with p
-// in the popcount field (one atomic on *nout per up to 256 of them: a
-// same-address atomic per wave step serialises across the chip).
+// Refines the candidates that the hashed k = 6 filter let through (their
+// keys carry the bucket popcount): half a wave per candidate, lane l of the
+// half ANDs words [4l, 4l + 4) of the two reads' 4,096-bit presence bitmaps
+// for the exact presence popcount p, and the tile's exact-bitmap bound
+// p + min(excess_i, excess_j) >= ((T_min - tshift) * cut - 1) keeps the
+// candidates that might pass; those are queued per wave in LDS and appended
+// to `out` with p in the popcount field (one atomic on *nout per up to 256 of
+// them: a same-address atomic per wave step serialises across the chip).
+// Input: the tile pass's kCandSegs segments (seg_cap keys each, counts in
+// `seg`); block b reads segment b mod kCandSegs, so the grid is a multiple of
+// kCandSegs.
 constexpr int kRefineQueue = 256;
 __global__ __launch_bounds__(256) void kmer_cand_refine_kernel(
-    const uint64_t *__restrict__ cand, int64_t ncand, const uint32_t *__restrict__ bitmap,
-    const int32_t *__restrict__ Tn, const int32_t *__restrict__ excess, float cut, int kshift,
-    unsigned long long *__restrict__ nout, uint64_t *__restrict__ out) {
+    const uint64_t *__restrict__ cand, const unsigned long long *__restrict__ seg, int64_t seg_cap,
+    const uint32_t *__restrict__ bitmap, const int32_t *__restrict__ Tn, const int32_t *__restrict__ excess,
+    float cut, int tshift, int kshift, unsigned long long *__restrict__ nout, uint64_t *__restrict__ out) {
   constexpr int WP = kmer_words(6);
   static_assert(WP == 4 * 32, "32 lanes of 4 words cover a k = 6 bitmap");
   __shared__ uint64_t sq[4][kRefineQueue];
@@ -808,8 +816,11 @@ __global__ __launch_bounds__(256) void kmer_cand_refine_kernel(
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     qn = 0;
   };
-  const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
-  const int64_t w0 = (int64_t)blockIdx.x * (blockDim.x >> 6) + wv;
+  const int sg = (int)(blockIdx.x % kCandSegs);
+  const int64_t ncand = min((int64_t)seg[(size_t)sg * kSegStride], seg_cap);
+  cand += (size_t)sg * seg_cap;
+  const int64_t waves = (int64_t)(gridDim.x / kCandSegs) * (blockDim.x >> 6);
+  const int64_t w0 = (int64_t)(blockIdx.x / kCandSegs) * (blockDim.x >> 6) + wv;
   for (int64_t cb = 2 * w0; cb < ncand; cb += 2 * waves) {  // wave-uniform
     const int64_t c = cb + h;
     const bool valid = c < ncand;
@@ -828,7 +839,7 @@ __global__ __launch_bounds__(256) void kmer_cand_refine_kernel(
     for (int off = 16; off > 0; off >>= 1) p += __shfl_xor(p, off, 32);
     bool keep = false;
     if (valid && l == 0) {
-      const float F = fmaf((float)min(Tn[i], Tn[j]), cut, -1.0f);
+      const float F = fmaf((float)(min(Tn[i], Tn[j]) - tshift), cut, -1.0f);
       keep = (float)(p + min(excess[i], excess[j])) >= F;
     }
     const uint64_t ball = __ballot(keep);
@@ -842,18 +853,22 @@ __global__ __launch_bounds__(256) void kmer_cand_refine_kernel(
 // Exact count of the K <= 6 candidates that the tile pass wrote out (pair key
 // plus its presence popcount in the high bits): matches = popcount + the
 // repeat-list merge (the same sum as in the tile's own survivor pass), the
-// reference distance and predicate, the passing pairs appended.
+// reference distance and predicate, the passing pairs appended.  Input:
+// `nseg` segments of seg_cap keys whose counts are ncnt[s * kSegStride] (the
+// tile pass's segments, or one run with its count at ncnt[0]); block b reads
+// segment b mod nseg, so the grid is a multiple of nseg.
 __global__ __launch_bounds__(256) void kmer_exact_kernel(
-    const uint64_t *__restrict__ cand, int64_t ncand_max, const unsigned long long *__restrict__ ncand_dev,
+    const uint64_t *__restrict__ cand, const unsigned long long *__restrict__ ncnt, int nseg, int64_t seg_cap,
     const int32_t *__restrict__ Tn,
     const int32_t *__restrict__ rcnt, const int64_t *__restrict__ roffc, const uint32_t *__restrict__ rlistc,
     float thr, int kshift, unsigned long long *__restrict__ counter, uint64_t *__restrict__ out_keys,
     float *__restrict__ out_dists, int64_t cap) {
-  // the candidate count: ncand_max, or the refine pass's count on the device
-  const int64_t ncand = ncand_dev ? (int64_t)min((unsigned long long)ncand_max, *ncand_dev) : ncand_max;
+  const int sg = (int)(blockIdx.x % nseg);
+  const int64_t ncand = min((int64_t)ncnt[(size_t)sg * kSegStride], seg_cap);
+  cand += (size_t)sg * seg_cap;
   const int lane = threadIdx.x & 63;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  const int64_t base0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63);
+  const int64_t stride = (int64_t)(gridDim.x / nseg) * blockDim.x;
+  const int64_t base0 = (int64_t)(blockIdx.x / nseg) * blockDim.x + (threadIdx.x & ~63);
   for (int64_t b = base0; b < ncand; b += stride) {
     const int64_t c = b + lane;
     bool pass = false;
@@ -929,6 +944,40 @@ static int launch_profile(crispy_ctx *ctx, ReadStore &rs) {
 // tile pass).
 __global__ void zero_counter_kernel(unsigned long long *c) { *c = 0ull; }
 
+// Zeroes the candidate segments' counters (one block of kCandSegs threads).
+__global__ __launch_bounds__(kCandSegs) void zero_segs_kernel(unsigned long long *seg) {
+  seg[(size_t)threadIdx.x * kSegStride] = 0ull;
+}
+
+// stat[0] = the segments' total count, stat[1] = the largest (one block of
+// kCandSegs threads).
+__global__ __launch_bounds__(kCandSegs) void seg_stat_kernel(const unsigned long long *__restrict__ seg,
+                                                             unsigned long long *__restrict__ stat) {
+  static_assert(kCandSegs % 64 == 0 && kCandSegs <= 1024, "whole waves, one block");
+  __shared__ unsigned long long ssum[kCandSegs / 64], smax[kCandSegs / 64];
+  unsigned long long c = seg[(size_t)threadIdx.x * kSegStride], m = c;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    c += __shfl_xor(c, off, 64);
+    const unsigned long long o = __shfl_xor(m, off, 64);
+    m = o > m ? o : m;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    ssum[threadIdx.x >> 6] = c;
+    smax[threadIdx.x >> 6] = m;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0, x = 0;
+    for (int w = 0; w < kCandSegs / 64; ++w) {
+      t += ssum[w];
+      x = smax[w] > x ? smax[w] : x;
+    }
+    stat[0] = t;
+    stat[1] = x;
+  }
+}
+
 // Read pairs i < j < n inside tile slots [tb, tb + nt): the tile pass's units.
 static double tile_units(int32_t n, int nb, int64_t tb, int64_t nt) {
   if (tb == 0 && nt == num_slots(nb)) return (double)n * (n - 1) / 2;
@@ -946,7 +995,9 @@ static double tile_units(int32_t n, int nb, int64_t tb, int64_t nt) {
 // The tile pass: zero the result counter, then the tile kernel over slots
 // [tb, tb + nt) (a captured graph with graphs on).  HASHED: the candidate
 // form on the k = 6 bucket bitmaps (kHashWords words per read) and their
-// excess; otherwise the presence bitmaps of K.
+// excess; otherwise the presence bitmaps of K.  CAND: the output is
+// kCandSegs segments of `cap` keys (counters in km.seg, their total and
+// largest count in km.seg_stat after the pass).
 template <int K, bool CAND, bool HASHED = false>
 static int launch_tiles(crispy_ctx *ctx, int32_t n, int nb, int64_t tb, int64_t nt, float thr,
                         int tshift, uint64_t *out_keys, float *out_dists, int64_t cap) {
@@ -960,22 +1011,30 @@ static int launch_tiles(crispy_ctx *ctx, int32_t n, int nb, int64_t tb, int64_t 
   const float cut = 1.0f - thr - 2e-5f;
   uint32_t thr_bits;
   std::memcpy(&thr_bits, &thr, sizeof thr_bits);
+  unsigned long long *counter = CAND ? km.seg.as<unsigned long long>() : km.counter.as<unsigned long long>();
   hipEvent_t ev;
   timer_begin(ctx, "kmer_tile", &ev);
   CRISPY_TRY(run_segment(
       ctx, km.g_tile,
       {(uintptr_t)WP, (uintptr_t)CAND, (uintptr_t)bits.p, (uintptr_t)km.T.p, (uintptr_t)km.rcnt.p,
        (uintptr_t)km.roffc.p, (uintptr_t)km.rlistc.p, (uintptr_t)excess.p, (uintptr_t)n, (uintptr_t)nb,
-       (uintptr_t)tb, (uintptr_t)nt, thr_bits, (uintptr_t)tshift, (uintptr_t)km.kshift, (uintptr_t)km.counter.p,
-       (uintptr_t)out_keys, (uintptr_t)out_dists, (uintptr_t)cap},
+       (uintptr_t)tb, (uintptr_t)nt, thr_bits, (uintptr_t)tshift, (uintptr_t)km.kshift, (uintptr_t)counter,
+       (uintptr_t)km.seg_stat.p, (uintptr_t)out_keys, (uintptr_t)out_dists, (uintptr_t)cap},
       "k-mer tile pass", [&](bool) -> int {
-        zero_counter_kernel<<<1, 1, 0, ctx->stream>>>(km.counter.as<unsigned long long>());
+        if (CAND)
+          zero_segs_kernel<<<1, kCandSegs, 0, ctx->stream>>>(counter);
+        else
+          zero_counter_kernel<<<1, 1, 0, ctx->stream>>>(counter);
         CRISPY_LAUNCHED(ctx);
         kmer_tile_mfma_kernel<WP, CAND><<<(unsigned)grid, 128, 0, ctx->stream>>>(
             bits.as<uint32_t>(), km.T.as<int32_t>(), km.rcnt.as<int32_t>(), km.roffc.as<int64_t>(),
             km.rlistc.as<uint32_t>(), excess.as<int32_t>(), n, nb, tb, nt, per_xcd, thr, cut, tshift,
-            km.kshift, km.counter.as<unsigned long long>(), out_keys, out_dists, cap);
+            km.kshift, counter, out_keys, out_dists, cap);
         CRISPY_LAUNCHED(ctx);
+        if (CAND) {
+          seg_stat_kernel<<<1, kCandSegs, 0, ctx->stream>>>(counter, km.seg_stat.as<unsigned long long>());
+          CRISPY_LAUNCHED(ctx);
+        }
         return CRISPY_OK;
       }));
   timer_end(ctx, "kmer_tile", ev, ctx->timing ? tile_units(n, nb, tb, nt) : 0.0);
@@ -1112,6 +1171,50 @@ static int run_counted(crispy_ctx *ctx, int64_t &cap, DevBuf &keys, DevBuf *dist
                  (long long)*count, (long long)cap);
 }
 
+// The candidate tile pass (`launch` zeroes the segment counters, appends to
+// kCandSegs segments of km.cand_seg_cap keys and totals them in km.seg_stat):
+// reruns once with larger segments if one overflowed; returns the total.
+template <class F>
+static int run_segmented(crispy_ctx *ctx, int64_t *count, F &&launch) {
+  KmerState &km = ctx->km;
+  CRISPY_TRY(ensure(ctx, km.seg, sizeof(unsigned long long) * kCandSegs * kSegStride));
+  CRISPY_TRY(ensure(ctx, km.seg_stat, 2 * sizeof(unsigned long long)));
+  unsigned long long st[2] = {0, 0};
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    CRISPY_TRY(ensure(ctx, km.cand, sizeof(uint64_t) * kCandSegs * km.cand_seg_cap));
+    CRISPY_TRY(launch());
+    CRISPY_HIP(ctx, hipMemcpyAsync(st, km.seg_stat.p, sizeof st, hipMemcpyDeviceToHost, ctx->stream));
+    CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *count = (int64_t)st[0];
+    if ((int64_t)st[1] <= km.cand_seg_cap) return CRISPY_OK;
+    if (attempt == 0) km.cand_seg_cap = (int64_t)st[1] + (int64_t)st[1] / 8 + 256;  // room for every segment
+  }
+  return set_err(ctx, CRISPY_ERR_STATE, "k-mer candidates overflowed a segment twice (%llu, room for %lld)",
+                 st[1], (long long)km.cand_seg_cap);
+}
+
+// The hashed filter's candidates (the tile pass's segments) refined by the
+// exact k = 6 bitmaps into km.cand_alt, their count left in km.cand_n.
+static int refine_candidates(crispy_ctx *ctx, int64_t ncand, float threshold, int tshift) {
+  KmerState &km = ctx->km;
+  CRISPY_TRY(ensure(ctx, km.cand_alt, sizeof(uint64_t) * std::max<int64_t>(ncand, 1)));
+  CRISPY_TRY(ensure(ctx, km.cand_n, sizeof(unsigned long long)));
+  hipEvent_t ev;
+  timer_begin(ctx, "kmer_cand_pop", &ev);
+  zero_counter_kernel<<<1, 1, 0, ctx->stream>>>(km.cand_n.as<unsigned long long>());
+  CRISPY_LAUNCHED(ctx);
+  // 8 candidates per block step; at most 8 blocks per segment
+  const int64_t per_seg = (ncand + kCandSegs - 1) / kCandSegs;
+  const int bps = (int)std::max<int64_t>(1, std::min<int64_t>(8, (per_seg + 63) / 64));
+  kmer_cand_refine_kernel<<<kCandSegs * bps, 256, 0, ctx->stream>>>(
+      km.cand.as<uint64_t>(), km.seg.as<unsigned long long>(), km.cand_seg_cap, km.bitmap.as<uint32_t>(),
+      km.T.as<int32_t>(), km.excess.as<int32_t>(), 1.0f - threshold - 2e-5f, tshift, km.kshift,
+      km.cand_n.as<unsigned long long>(), km.cand_alt.as<uint64_t>());
+  CRISPY_LAUNCHED(ctx);
+  timer_end(ctx, "kmer_cand_pop", ev, (double)ncand);
+  return CRISPY_OK;
+}
+
 static int sort_keys(crispy_ctx *ctx, int64_t count) {
   KmerState &km = ctx->km;
   if (count <= 1) {
@@ -1205,32 +1308,27 @@ int crispy_kmer_run(crispy_ctx *ctx, int K, float threshold, int64_t tile_begin,
                        km.dists.as<float>(), km.cap);
     }));
   } else if (K <= 6) {  // the tile pass writes candidates, a second pass counts them exactly
-    km.cand_cap = std::max(km.cand_cap, guess);
+    km.cand_seg_cap = std::max(km.cand_seg_cap, (guess + kCandSegs - 1) / kCandSegs);
     int64_t ncand = 0;
-    CRISPY_TRY(run_counted(ctx, km.cand_cap, km.cand, nullptr, &ncand, [&]() {
+    CRISPY_TRY(run_segmented(ctx, &ncand, [&]() {
       return kTilesCand[K](ctx, n, nb, tile_begin, nt, threshold, 0, km.cand.as<uint64_t>(), nullptr,
-                           km.cand_cap);
+                           km.cand_seg_cap);
     }));
     km.candidates = ncand;
-    // hashed filter: the exact presence popcount and the exact-bitmap bound
-    // first, so that only the candidates the tile would have kept with the
-    // exact bitmaps go on to the repeat-list merge (their count stays on the
-    // device)
+    // hashed filter (K = 6): the exact presence popcount and the exact-bitmap
+    // bound first, so that only the candidates the tile would have kept with
+    // the exact bitmaps go on to the repeat-list merge (one run, its count on
+    // the device); otherwise the merge reads the tile's segments
     const uint64_t *exact_in = km.cand.as<uint64_t>();
-    const unsigned long long *exact_n = nullptr;
+    const unsigned long long *exact_n = km.seg.as<unsigned long long>();
+    int nseg = kCandSegs;
+    int64_t seg_cap = km.cand_seg_cap;
     if (km.hashed && ncand > 0) {
-      CRISPY_TRY(ensure(ctx, km.cand_alt, sizeof(uint64_t) * ncand));
-      CRISPY_TRY(ensure(ctx, km.cand_n, sizeof(unsigned long long)));
-      hipEvent_t pev;
-      timer_begin(ctx, "kmer_cand_pop", &pev);
-      zero_counter_kernel<<<1, 1, 0, ctx->stream>>>(km.cand_n.as<unsigned long long>());
-      kmer_cand_refine_kernel<<<(int)std::min<int64_t>((ncand + 7) / 8, 2048), 256, 0, ctx->stream>>>(
-          km.cand.as<uint64_t>(), ncand, km.bitmap.as<uint32_t>(), km.T.as<int32_t>(), km.excess.as<int32_t>(),
-          1.0f - threshold - 2e-5f, km.kshift, km.cand_n.as<unsigned long long>(), km.cand_alt.as<uint64_t>());
-      CRISPY_LAUNCHED(ctx);
-      timer_end(ctx, "kmer_cand_pop", pev, (double)ncand);
+      CRISPY_TRY(refine_candidates(ctx, ncand, threshold, 0));
       exact_in = km.cand_alt.as<uint64_t>();
       exact_n = km.cand_n.as<unsigned long long>();
+      nseg = 1;
+      seg_cap = ncand;
     }
     CRISPY_TRY(run_counted(ctx, km.cap, km.keys, &km.dists, &km.count, [&]() -> int {
       zero_counter_kernel<<<1, 1, 0, ctx->stream>>>(km.counter.as<unsigned long long>());
@@ -1238,9 +1336,10 @@ int crispy_kmer_run(crispy_ctx *ctx, int K, float threshold, int64_t tile_begin,
       if (ncand == 0) return CRISPY_OK;
       hipEvent_t ev;
       timer_begin(ctx, "kmer_cand", &ev);
-      const int blocks = (int)std::min<int64_t>((ncand + 255) / 256, 8192);
-      kmer_exact_kernel<<<blocks, 256, 0, ctx->stream>>>(
-          exact_in, ncand, exact_n, km.T.as<int32_t>(), km.rcnt.as<int32_t>(), km.roffc.as<int64_t>(),
+      const int64_t per_seg = (ncand + nseg - 1) / nseg;
+      const int bps = (int)std::max<int64_t>(1, std::min<int64_t>(8192 / nseg, (per_seg + 255) / 256));
+      kmer_exact_kernel<<<nseg * bps, 256, 0, ctx->stream>>>(
+          exact_in, exact_n, nseg, seg_cap, km.T.as<int32_t>(), km.rcnt.as<int32_t>(), km.roffc.as<int64_t>(),
           km.rlistc.as<uint32_t>(), threshold, km.kshift, km.counter.as<unsigned long long>(),
           km.keys.as<uint64_t>(), km.dists.as<float>(), km.cap);
       CRISPY_LAUNCHED(ctx);
@@ -1248,43 +1347,51 @@ int crispy_kmer_run(crispy_ctx *ctx, int K, float threshold, int64_t tile_begin,
       return CRISPY_OK;
     }));
   } else {
-    km.cand_cap = std::max(km.cand_cap, guess);
+    km.cand_seg_cap = std::max(km.cand_seg_cap, (guess + kCandSegs - 1) / kCandSegs);
     int64_t ncand = 0;
-    CRISPY_TRY(run_counted(ctx, km.cand_cap, km.cand, nullptr, &ncand, [&]() {
+    CRISPY_TRY(run_segmented(ctx, &ncand, [&]() {
       return launch_tiles<6, true, true>(ctx, n, nb, tile_begin, nt, threshold, K - 6,
-                                   km.cand.as<uint64_t>(), nullptr, km.cand_cap);
+                                         km.cand.as<uint64_t>(), nullptr, km.cand_seg_cap);
     }));
     km.candidates = ncand;
+    // the exact k = 6 bitmap bound against T_K = T_6 - (K - 6), into one run
+    int64_t nref = 0;
+    if (ncand > 0) {
+      CRISPY_TRY(refine_candidates(ctx, ncand, threshold, K - 6));
+      unsigned long long c = 0;
+      CRISPY_HIP(ctx, hipMemcpyAsync(&c, km.cand_n.p, sizeof c, hipMemcpyDeviceToHost, ctx->stream));
+      CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      nref = (int64_t)c;
+    }
     // (i, j) order, so the lanes of a wave merge against the same few row lists
-    const uint64_t *cand = km.cand.as<uint64_t>();
-    if (ncand > 1) {
-      CRISPY_TRY(ensure(ctx, km.cand_alt, sizeof(uint64_t) * ncand));
+    const uint64_t *cand = km.cand_alt.as<uint64_t>();
+    if (nref > 1) {
       size_t tbytes = 0;
-      CRISPY_HIP(ctx, rocprim::radix_sort_keys<RadixOnesweep>(nullptr, tbytes, km.cand.as<uint64_t>(),
-                                               km.cand_alt.as<uint64_t>(), (size_t)ncand, 0, 2 * km.kshift,
+      CRISPY_HIP(ctx, rocprim::radix_sort_keys<RadixOnesweep>(nullptr, tbytes, km.cand_alt.as<uint64_t>(),
+                                               km.cand.as<uint64_t>(), (size_t)nref, 0, 2 * km.kshift,
                                                ctx->stream));
       CRISPY_TRY(ensure(ctx, ctx->temp, tbytes));
       hipEvent_t sev;
       timer_begin(ctx, "kmer_cand_sort", &sev);
-      CRISPY_HIP(ctx, rocprim::radix_sort_keys<RadixOnesweep>(ctx->temp.p, tbytes, km.cand.as<uint64_t>(),
-                                               km.cand_alt.as<uint64_t>(), (size_t)ncand, 0, 2 * km.kshift,
+      CRISPY_HIP(ctx, rocprim::radix_sort_keys<RadixOnesweep>(ctx->temp.p, tbytes, km.cand_alt.as<uint64_t>(),
+                                               km.cand.as<uint64_t>(), (size_t)nref, 0, 2 * km.kshift,
                                                ctx->stream));
-      timer_end(ctx, "kmer_cand_sort", sev, (double)ncand);
-      cand = km.cand_alt.as<uint64_t>();
+      timer_end(ctx, "kmer_cand_sort", sev, (double)nref);
+      cand = km.cand.as<uint64_t>();
     }
     CRISPY_TRY(run_counted(ctx, km.cap, km.keys, &km.dists, &km.count, [&]() -> int {
       zero_counter_kernel<<<1, 1, 0, ctx->stream>>>(km.counter.as<unsigned long long>());
       CRISPY_LAUNCHED(ctx);
-      if (ncand == 0) return CRISPY_OK;
+      if (nref == 0) return CRISPY_OK;
       hipEvent_t ev;
       timer_begin(ctx, "kmer_cand", &ev);
-      const int blocks = (int)std::min<int64_t>((ncand + 255) / 256, 8192);
+      const int blocks = (int)std::min<int64_t>((nref + 255) / 256, 8192);
       kmer_cand_kernel<<<blocks, 256, 0, ctx->stream>>>(
-          cand, ncand, km.lists.as<uint16_t>(), km.loff.as<int64_t>(), K,
+          cand, nref, km.lists.as<uint16_t>(), km.loff.as<int64_t>(), K,
           rs.len.as<int32_t>(), threshold, km.kshift, km.counter.as<unsigned long long>(), km.keys.as<uint64_t>(),
           km.dists.as<float>(), km.cap);
       CRISPY_LAUNCHED(ctx);
-      timer_end(ctx, "kmer_cand", ev, (double)ncand);
+      timer_end(ctx, "kmer_cand", ev, (double)nref);
       return CRISPY_OK;
     }));
   }

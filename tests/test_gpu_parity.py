@@ -229,6 +229,37 @@ def test_kmer_tile_ranges_partition(ctx):
     assert np.array_equal(canon(p, d), canon(fp, fd))
 
 
+def test_trim_releases_every_buffer():
+    """After runs that touch every device buffer family (the hashed k = 6
+    candidates and their segments, K = 8 lists, the fused NW, the device
+    output sort, the CSR), trimming a context gives all its device memory
+    back, and the context still runs."""
+    say("trim")
+    seqs, _, _ = synth.make("reads1k")
+    r = ca.Reads.from_list(seqs)
+    thr = ca.kmer_default_threshold(r.n)
+    with ca.Context(0) as c:
+        c.upload_reads(ca.KMER, r)
+        c.upload_reads(ca.NW, r)
+        c.kmer_exact_pass(1)
+        c.kmer_run(8, thr)
+        c.kmer_run(6, thr)
+        c.nw_run_on_kmer(10, ca.nw_default_threshold(r.n))
+        files = ca.pipeline.fused_output_files(c)
+        c.nw_csr(r.n)
+        assert c.device_bytes() > 0
+        c.trim()
+        assert c.device_bytes() == 0
+        c.upload_reads(ca.KMER, r)
+        c.upload_reads(ca.NW, r)
+        c.kmer_run(6, thr)
+        c.nw_run_on_kmer(10, ca.nw_default_threshold(r.n))
+        again = ca.pipeline.fused_output_files(c)
+    assert len(again) == len(files)
+    for (p0, d0), (p1, d1) in zip(files, again):
+        assert np.array_equal(p0, p1) and np.array_equal(bits(d0), bits(d1))
+
+
 # ------------------------------------------------------------------- genDist
 @pytest.mark.parametrize("band", [10, 5, 2, 1])
 def test_nw_clean_sample_golden(ctx, band):
