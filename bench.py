@@ -13,7 +13,11 @@ cells), the roofline of the dominant kernel (timed with HIP events on the
 stream it runs on), the CPU oracle timed on a bounded sample (all cores and
 the reference's shipped thread settings), the served form under a Poisson
 job stream (jobs/s, p50/p99 latency) and the 100,000-read k = 6 step against
-its own CPU sample.
+its own CPU sample.  Before the timed steps the library's autotuners pick
+the k-mer exact-count form and the NW slice width on the workload
+(crispy_kmer_autotune, crispy_nw_autotune); the line reports the choice and
+the default settings' step time beside the tuned one (--no-autotune times
+the defaults only).
 
     python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--workload reads250k]
 """
@@ -289,10 +293,18 @@ def served_leg(seconds=5.0, slots=4, rate_factor=2.0, pool_size=16, seed=7):
         many.close()
 
 
-def run_workload(ca, workload, K_override, steps, warmup, graphs, output_side=True, timers=True):
+KMER_FORMS = {0: "exact counts in the tile", 1: "candidates (hashed for k >= 6) + exact pass",
+              2: "candidates on the exact bitmaps + exact pass"}
+
+
+def run_workload(ca, workload, K_override, steps, warmup, graphs, output_side=True, timers=True,
+                 autotune=True):
     """Times `steps` pipeline steps of `workload` after `warmup` untimed ones
     (and, with timers, the same steps again with the per-kernel HIP-event
-    timers on).  Returns a dict of the measurements plus the context."""
+    timers on).  With autotune, the default settings' step time is taken
+    first (as many steps), then the k-mer exact-count form and the NW slice
+    width are tuned on the workload and the timed steps run tuned.  Returns
+    a dict of the measurements plus the context."""
     from crispy_amd import synth
     from crispy_amd.pipeline import fused_output_files
     t0 = time.perf_counter()
@@ -317,6 +329,25 @@ def run_workload(ca, workload, K_override, steps, warmup, graphs, output_side=Tr
     for _ in range(warmup):
         kc, nc, files = step()
     ctx.sync()
+    tune = None
+    if autotune:
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step(output_side)
+        ctx.sync()
+        default_ms = (time.perf_counter() - t0) / steps * 1e3
+        t0 = time.perf_counter()
+        kmode = ctx.kmer_autotune(K, kthr, repeats=2)
+        width = ctx.nw_autotune(band, repeats=2)
+        # the defaults: exact-count form by pair count (-1), slice width by the cost model (0)
+        default_mode = 1 if K > 6 or n * (n - 1) // 2 >= 10 ** 9 else 0
+        tune = {"kmer_exact_pass": kmode, "kmer_form": KMER_FORMS[kmode], "nw_slice_width": width,
+                "nw_slice_policy": "cost model" if width == 0 else f"{width} cells per lane",
+                "chosen_is_default": kmode == default_mode and width == 0,
+                "default_ms_per_step": default_ms, "tuning_s": time.perf_counter() - t0}
+        for _ in range(warmup):
+            step()
+        ctx.sync()
     t0 = time.perf_counter()
     for _ in range(steps):
         kc, nc, files = step(output_side)
@@ -325,6 +356,11 @@ def run_workload(ca, workload, K_override, steps, warmup, graphs, output_side=Tr
     res = dict(ctx=ctx, seqs=seqs, cfg=cfg, n=n, K=K, band=band, kthr=kthr, nthr=nthr, kc=kc, nc=nc,
                tile_bits=ctx.kmer_tile_bits(),
                ms_step=dt / steps * 1e3, nfiles=len(files) if files is not None else None)
+    if tune is not None:
+        tune["tuned_ms_per_step"] = res["ms_step"]
+        res["autotune"] = tune
+        log(f"autotune: k-mer form {kmode} ({KMER_FORMS[kmode]}), NW slice width {tune['nw_slice_policy']}; "
+            f"{default_ms:.2f} ms/step default -> {res['ms_step']:.2f} ms/step tuned")
     if output_side:
         # the same steps without the output side (file split, sort, copy out)
         t0 = time.perf_counter()
@@ -363,6 +399,8 @@ def main():
     ap.add_argument("--no-c3", action="store_true", help="skip the 100,000-read k=6 leg")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every kernel chain kernel by kernel instead of replaying captured graphs")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="time the default settings only (no k-mer form / NW slice-width tuning)")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds per CPU baseline leg")
     ap.add_argument("--served-seconds", type=float, default=5.0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -398,7 +436,8 @@ def main():
 
     import crispy_amd as ca
 
-    r = run_workload(ca, args.workload, args.k, args.steps, args.warmup, not args.no_graphs)
+    r = run_workload(ca, args.workload, args.k, args.steps, args.warmup, not args.no_graphs,
+                     autotune=not args.no_autotune)
     ctx, n, K, band, kern = r["ctx"], r["n"], r["K"], r["band"], r["kern"]
     steps = args.steps
     total_pairs = n * (n - 1) // 2
@@ -463,6 +502,8 @@ def main():
         "kernels_ms_per_step": {k: v["total_ms"] / steps for k, v in kern.items()},
         "roofline": roofline,
     }
+    if "autotune" in r:
+        out["autotune"] = r["autotune"]
     kp = None
     if not args.no_cpu:
         kp, _ = ctx.kmer_fetch(dists=False)
@@ -483,11 +524,14 @@ def main():
     if not args.no_served:
         out["served"] = served_leg(args.served_seconds)
     if not args.no_c3 and args.workload != "reads100k":
-        c3 = run_workload(ca, "reads100k", 6, steps, args.warmup, not args.no_graphs, timers=False)
+        c3 = run_workload(ca, "reads100k", 6, steps, args.warmup, not args.no_graphs, timers=False,
+                          autotune=not args.no_autotune)
         c3n = c3["n"] * (c3["n"] - 1) // 2
         entry = {"workload": "reads100k, k=6, band 10 (the same step incl. the output side)",
                  "ms_per_step": c3["ms_step"], "pairs_per_s": c3n / (c3["ms_step"] * 1e-3),
                  "nw_passing_pairs": c3["nc"]}
+        if "autotune" in c3:
+            entry["autotune"] = c3["autotune"]
         c3kp = None
         if not args.no_cpu:
             c3kp, _ = c3["ctx"].kmer_fetch(dists=False)

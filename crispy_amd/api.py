@@ -72,6 +72,8 @@ def load_library():
         "crispy_kmer_set_exact_pass": ([_vp, ctypes.c_int], ctypes.c_int),
         "crispy_kmer_tile_bits": ([_vp], ctypes.c_int),
         "crispy_nw_autotune": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+        "crispy_kmer_autotune": ([_vp, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
+                                 ctypes.c_int),
         "crispy_sort_by_dist": ([_vp, _i32p, _f32p, ctypes.c_int64], ctypes.c_int),
         "crispy_nw_dists_noalloc": ([_vp, _vp, ctypes.c_int64, ctypes.c_int, _vp], ctypes.c_int),
         "crispy_nw_file_split": ([_vp, ctypes.c_int64, ctypes.c_int64, _i64p, ctypes.c_int32,
@@ -333,9 +335,19 @@ class Context:
     def kmer_exact_pass(self, mode):
         """K <= 6: exact counts of the pairs the tile's bound keeps in the tile
         kernel (0 or False), in a second kernel over the tile's candidates
-        (1 or True), or chosen by the pair count (-1, the default); the
+        (1 or True; for K = 6 the candidate tile runs on the hashed bitmaps),
+        the same without hashing (2), or chosen by the pair count (-1, the
+        default); K = 7, 8 take candidates, hashed unless the mode is 2.  The
         results are the same."""
         self._check(self._L.crispy_kmer_set_exact_pass(self._h, int(mode)))
+
+    def kmer_autotune(self, K, threshold, repeats=2):
+        """Times the k-mer run under each exact-count form that applies to K,
+        keeps the fastest (and its result); returns the chosen mode."""
+        c = ctypes.c_int()
+        self._check(self._L.crispy_kmer_autotune(self._h, K, threshold, repeats, ctypes.byref(c)))
+        self.kmer_run(K, threshold)  # the chosen form's result and its count
+        return c.value
 
     def graphs(self, enable):
         """Kernel chains (k-mer profile and tile passes, NW DP phase, NW

@@ -31,8 +31,10 @@
 #include <rocprim/iterator/counting_iterator.hpp>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <utility>
+#include <vector>
 
 #include "common.hpp"
 #include "graph_segment.hpp"
@@ -320,25 +322,12 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
   const int lane = tid & 63, wv = tid >> 6;
   const int g = lane >> 5, r32 = lane & 31;
 
-  for (int q = tid; q < 2 * kTile; q += 128) {
-    const int r = q < kTile ? row0 + q : col0 + (q - kTile);
-    const bool ok = r < n;
-    const int T = ok ? Tn[r] : 1;
-    if (!CAND) {  // the candidate form needs only the bound
-      sT[q] = T;
-      sRc[q] = ok ? rcnt[r] : 0;
-      sRo[q] = ok ? roff[r] : 0;
-    }
-    // G = F - E (exact in fp32 when positive: E is an integer below F), >= 0
-    const float G = fmaf((float)(T - tshift), cut, -1.0f) - (float)(ok ? excess[r] : 0);
-    sG[q] = ok && G > 0.0f ? G : 0.0f;
-  }
-
   // loader: a chunk is 16 words (64 bytes) of each of the 256 reads; lane quad
   // (t >> 2) of the block loads reads (t >> 2) + 32k, k = 0..3, of the rows and
   // of the columns, lane t & 3 one 16-byte quarter, so a wave instruction reads
   // 16 whole 64-byte runs.  Rows past n load read 0 (always valid): their
-  // pairs are masked in the epilogue.
+  // pairs are masked in the epilogue.  Chunk 0 is requested first and the
+  // per-read data right after it, so their latencies overlap.
   const int lq = tid & 3, lr = tid >> 2;
   uint32_t oa[4], ob[4];  // word offsets of this lane's quarter, chunk 0
 #pragma unroll
@@ -357,6 +346,37 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
   for (int k = 0; k < 4; ++k) {
     pa[k] = ld(oa[k]);
     pb[k] = ld(ob[k]);
+  }
+
+  // per read of the tile (thread t: reads t and t + 128 of the 256)
+  static_assert(2 * kTile == 2 * 128, "128 threads cover the tile's 256 reads in two steps");
+  int pT[2], pE[2], pC[2];
+  int64_t pO[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int q = tid + 128 * h;
+    const int r = q < kTile ? row0 + q : col0 + (q - kTile);
+    const int rr = r < n ? r : 0;  // unconditional loads (read 0 stands in), so none waits on another
+    pT[h] = Tn[rr];
+    pE[h] = excess[rr];
+    if (!CAND) {  // the candidate form needs only the bound
+      pC[h] = r < n ? rcnt[rr] : 0;
+      pO[h] = r < n ? roff[rr] : 0;
+    }
+  }
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int q = tid + 128 * h;
+    const int r = q < kTile ? row0 + q : col0 + (q - kTile);
+    if (!CAND) {
+      sT[q] = r < n ? pT[h] : 1;
+      sRc[q] = pC[h];
+      sRo[q] = pO[h];
+    }
+    // G = F - E (exact in fp32 when positive: E is an integer below F), >= 0;
+    // 0 past the last read (those pairs are masked in the epilogue)
+    const float G = fmaf((float)(pT[h] - tshift), cut, -1.0f) - (float)pE[h];
+    sG[q] = r < n && G > 0.0f ? G : 0.0f;
   }
 
   mfma_v16f acc[2][4];
@@ -1255,8 +1275,48 @@ int crispy_kmer_tile_bits(crispy_ctx *ctx) {
 
 int crispy_kmer_set_exact_pass(crispy_ctx *ctx, int mode) {
   if (!ctx) return CRISPY_ERR_ARG;
-  if (mode < -1 || mode > 1) return set_err(ctx, CRISPY_ERR_ARG, "exact pass mode %d: -1, 0 or 1", mode);
+  if (mode < -1 || mode > 2) return set_err(ctx, CRISPY_ERR_ARG, "exact pass mode %d: -1, 0, 1 or 2", mode);
   ctx->km.exact_pass = mode;
+  return CRISPY_OK;
+}
+
+int crispy_kmer_autotune(crispy_ctx *ctx, int K, float threshold, int repeats, int *chosen) {
+  if (!ctx || repeats < 1) return CRISPY_ERR_ARG;
+  if (K < 1 || K > 8) return set_err(ctx, CRISPY_ERR_ARG, "unsupported K=%d (1..8)", K);
+  CRISPY_HIP(ctx, hipSetDevice(ctx->device));
+  // K <= 5 have no hashed form (2 = 1); K > 6 always take candidates (0 = 1)
+  std::vector<int> modes;
+  if (K <= 6) modes.push_back(0);
+  modes.push_back(1);
+  if (K >= 6) modes.push_back(2);
+  KmerState &km = ctx->km;
+  const int before = km.exact_pass;
+  double best = 1e300;
+  int pick = before;
+  for (int mode : modes) {
+    km.exact_pass = mode;
+    double fastest = 1e300;
+    for (int r = 0; r <= repeats; ++r) {  // the first run warms up (profiles, graphs, buffers)
+      CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      const auto t0 = std::chrono::steady_clock::now();
+      const int rc = crispy_kmer_run(ctx, K, threshold, 0, -1, nullptr);
+      if (rc != CRISPY_OK) {
+        km.exact_pass = before;
+        return rc;
+      }
+      CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      if (r > 0) fastest = std::min(fastest, sec);
+    }
+    if (fastest < best) {
+      best = fastest;
+      pick = mode;
+    }
+  }
+  km.exact_pass = pick;
+  // leave the result of the chosen setting (every setting gives the same one)
+  CRISPY_TRY(crispy_kmer_run(ctx, K, threshold, 0, -1, nullptr));
+  if (chosen) *chosen = pick;
   return CRISPY_OK;
 }
 
@@ -1296,8 +1356,11 @@ int crispy_kmer_run(crispy_ctx *ctx, int K, float threshold, int64_t tile_begin,
   // -> 44.0 ms per k-mer run); below, the pass's latency-bound merges cost more
   // than the tile saves (reads20k 0.78 -> 0.97 ms).  The candidate tiles of
   // k = 6 run on the hashed bitmaps (half the words).
-  const bool separate = km.exact_pass == 1 || (km.exact_pass == -1 && (int64_t)n * (n - 1) / 2 >= kExactPassPairs);
-  CRISPY_TRY(build_profiles(ctx, KP, K, K > 6 || separate));
+  // exact_pass 2: the candidate form on the exact bitmaps (no hashing).
+  const bool separate = km.exact_pass >= 1 || (km.exact_pass == -1 && (int64_t)n * (n - 1) / 2 >= kExactPassPairs);
+  CRISPY_TRY(build_profiles(ctx, KP, K, (K > 6 || separate) && km.exact_pass != 2));
+  // the k = 6 candidate tiles: on the hashed bitmaps, or on the exact ones
+  const TileFn tiles6 = km.hashed ? launch_tiles<6, true, true> : launch_tiles<6, true, false>;
   // first guess at the result size: the pairs in the range, at most 4 Mi
   const int64_t pairs_in_range = std::min<int64_t>(nt * kTile * kTile, (int64_t)n * (n - 1) / 2);
   const int64_t guess = std::min<int64_t>((int64_t)1 << 22, std::max<int64_t>(4096, pairs_in_range));
@@ -1311,8 +1374,8 @@ int crispy_kmer_run(crispy_ctx *ctx, int K, float threshold, int64_t tile_begin,
     km.cand_seg_cap = std::max(km.cand_seg_cap, (guess + kCandSegs - 1) / kCandSegs);
     int64_t ncand = 0;
     CRISPY_TRY(run_segmented(ctx, &ncand, [&]() {
-      return kTilesCand[K](ctx, n, nb, tile_begin, nt, threshold, 0, km.cand.as<uint64_t>(), nullptr,
-                           km.cand_seg_cap);
+      return (K == 6 ? tiles6 : kTilesCand[K])(ctx, n, nb, tile_begin, nt, threshold, 0,
+                                               km.cand.as<uint64_t>(), nullptr, km.cand_seg_cap);
     }));
     km.candidates = ncand;
     // hashed filter (K = 6): the exact presence popcount and the exact-bitmap
@@ -1350,11 +1413,12 @@ int crispy_kmer_run(crispy_ctx *ctx, int K, float threshold, int64_t tile_begin,
     km.cand_seg_cap = std::max(km.cand_seg_cap, (guess + kCandSegs - 1) / kCandSegs);
     int64_t ncand = 0;
     CRISPY_TRY(run_segmented(ctx, &ncand, [&]() {
-      return launch_tiles<6, true, true>(ctx, n, nb, tile_begin, nt, threshold, K - 6,
-                                         km.cand.as<uint64_t>(), nullptr, km.cand_seg_cap);
+      return tiles6(ctx, n, nb, tile_begin, nt, threshold, K - 6, km.cand.as<uint64_t>(), nullptr,
+                    km.cand_seg_cap);
     }));
     km.candidates = ncand;
     // the exact k = 6 bitmap bound against T_K = T_6 - (K - 6), into one run
+    // (on exact-bitmap candidates it keeps them all: then it only compacts)
     int64_t nref = 0;
     if (ncand > 0) {
       CRISPY_TRY(refine_candidates(ctx, ncand, threshold, K - 6));

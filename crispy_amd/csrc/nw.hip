@@ -73,7 +73,7 @@ static BucketKey bucket_key(int kb_lo, int kb_top, int maxlen) {
 }
 
 // ---------------------------------------------------------------- bucketing
-// Sort key per pair: kB << 11 | R, R = min(L2, L1 + kB) the last row with an
+// Sort key per pair: kB << 11 | R, R = min(L1, L2) the DP's last row with an
 // in-band cell, clamped to 2047, for the register and sliced kernels; all ones
 // for the generic kernel.  The keys are radix-sorted with the pair indices as
 // values; the sort is stable, so equal keys keep the (i, j) order of the
@@ -104,8 +104,10 @@ __global__ __launch_bounds__(256) void nw_bucket_kernel(
       const int kb = (L1 + L2) / (2 * band);
       nc = band_cells(L1, L2, kb);
       if (kb <= kKbSliceMax && clean[pr.x] && clean[pr.y]) {
-        // rows past L1 + kB hold no in-band cell: the DP ends at min(L2, L1 + kB)
-        const int rows = min(L2, L1 + kb);
+        // rows past the column read's length + kB hold no in-band cell, and
+        // the DP runs its rows over the shorter read (nw_dp.hpp), so it ends
+        // at row min(L1, L2)
+        const int rows = min(L1, L2);
         key = ((uint32_t)(kb - bk.kb_lo) << bk.rbits) | (uint32_t)min(rows, 2047);
         bucket = kb;
       }

@@ -27,7 +27,11 @@ enum { kRowFull = 0, kRowCapture = 1, kRowTop = 2, kRowTopCap = 3 };
 constexpr bool row_top(int mode) { return mode == kRowTop || mode == kRowTopCap; }
 constexpr bool row_captures(int mode) { return mode == kRowCapture || mode == kRowTopCap; }
 
-template <int KB, int MODE>
+// TR: the DP runs transposed (rows over the reference's read 1, columns over
+// its read 2), so the reference's "up" move (same row) is this frame's
+// "left" (previous row) and vice versa: of two equal gap moves the
+// previous-row one wins, which keeps the reference's choice.
+template <int KB, int MODE, bool TR>
 __device__ __forceinline__ void nw_row(const int i, int (&Mt)[2 * KB + 1], int (&Mv)[2 * KB + 1],
                                        uint32_t (&LA)[2 * KB + 1],
                                        const uint32_t (&m)[(2 * KB + 1 + 15) / 16], const int dl,
@@ -55,18 +59,21 @@ __device__ __forceinline__ void nw_row(const int i, int (&Mt)[2 * KB + 1], int (
     const int diag = Mt[d] + (int)(9u * s);
     const int left = d < W - 1 ? Mv[d + 1] : kNeg;
     const int up = d > 0 ? Mh : kNeg;
-    const int M = max(max(diag, up), left);
+    int M = max(max(diag, up), left);
+    // opaque: otherwise the transposed frame's M == left becomes
+    // left >= max(diag, up), which splits the max3 into two maxima
+    asm volatile("" : "+v"(M));
     const bool cd = M == diag;
-    const bool cu = M == up;
+    const bool c2 = M == (TR ? left : up);  // the gap move that wins a tie
     const uint32_t lad = LA[d] + (s << 16);
     const uint32_t lal = d < W - 1 ? LA[d + 1] + 1u : 0u;
-    const uint32_t la = cd ? lad : (cu ? LAu : lal);
+    const uint32_t la = cd ? lad : (c2 ? (TR ? lal : LAu) : (TR ? LAu : lal));
     // pen_h (low half) and 4 + pen_v (high half) of the winner in one select;
     // the sign-extending half reads fold into SDWA operands of the adds.
     constexpr uint32_t kPenDiag = ((uint32_t)(4 + kGO) << 16) | (uint16_t)kGO;
     constexpr uint32_t kPenUp = ((uint32_t)4 << 16) | (uint16_t)kGE;
     constexpr uint32_t kPenLeft = ((uint32_t)(4 + kGE) << 16) | 0u;
-    const uint32_t pk = cd ? kPenDiag : (cu ? kPenUp : kPenLeft);
+    const uint32_t pk = cd ? kPenDiag : (c2 ? (TR ? kPenLeft : kPenUp) : (TR ? kPenUp : kPenLeft));
     Mh = M + (int)(int16_t)(pk & 0xFFFFu);
     LAu = la;
     Mt[d] = M;
@@ -105,25 +112,15 @@ constexpr int nw_reg_waves() {
   return (3 * (2 * KB + 1) + 64) <= 128 ? 4 : (3 * (2 * KB + 1) + 64) <= 168 ? 3 : 2;
 }
 
-// One wave: pairs order[base .. base+63] of a bucket of `count` pairs.
-template <int KB>
-__device__ __forceinline__ void nw_reg_wave(const uint32_t *__restrict__ pack,
-                                            const int64_t *__restrict__ poff,
-                                            const int32_t *__restrict__ len,
-                                            const int2 *__restrict__ pairs,
-                                            const uint32_t *__restrict__ order, int32_t count,
-                                            int base, float *__restrict__ out) {
+// The DP of one wave's pairs: columns over s1 (L1 bases), rows over s2 (L2
+// bases); TR when that is the transposed frame (s1 = the reference's read 2).
+template <int KB, bool TR>
+__device__ __forceinline__ void nw_reg_dp(const uint32_t *__restrict__ s1, const uint32_t *__restrict__ s2,
+                                          const int L1, const int L2, const bool active,
+                                          float *__restrict__ out) {
   constexpr int W = 2 * KB + 1;
   constexpr int NMW = (W + 15) / 16;  // match words per row
   constexpr int NB = NMW + 1;         // window words (one of lookahead)
-  const int lane = threadIdx.x & 63;
-  const int t = base + lane;
-  const bool active = t < count;
-  const uint32_t pidx = order[active ? t : count - 1];
-  const int2 pr = pairs[pidx];
-  const int L1 = len[pr.x], L2 = len[pr.y];
-  const uint32_t *s1 = pack + poff[pr.x];
-  const uint32_t *s2 = pack + poff[pr.y];
   const int nw1 = (L1 + 15) >> 4, nw2 = (L2 + 15) >> 4;
 
   // Rows past L1 + kB hold no in-band cell: the last column is out of reach
@@ -182,7 +179,7 @@ __device__ __forceinline__ void nw_reg_wave(const uint32_t *__restrict__ pack,
     uint32_t capLA = 0;
     // rows past the pair's last in-band row R leave its registers alone, so
     // they still hold row R when the loop ends
-    if (i <= R) nw_row<KB, MODE>(i, Mt, Mv, LA, m, dl, L1max - i + KB, capM, capLA);
+    if (i <= R) nw_row<KB, MODE, TR>(i, Mt, Mv, LA, m, dl, L1max - i + KB, capM, capLA);
     if (row_captures(MODE) && dl >= 0 && dl < W && i <= L2) {
       const int Mr = capM - 4 * i;
       if (Mr >= bestJ) {
@@ -228,9 +225,12 @@ __device__ __forceinline__ void nw_reg_wave(const uint32_t *__restrict__ pack,
       }
     }
   }
+  // the reference takes its last row's best only when it beats its last
+  // column's (genCPU.cpp:461-464); transposed, those are this frame's last
+  // column and last row
   uint32_t la;
   int j;
-  if (bestI > bestJ) {
+  if (TR ? !(bestJ > bestI) : bestI > bestJ) {
     la = bestILA;
     j = bestIj;
   } else {
@@ -239,7 +239,38 @@ __device__ __forceinline__ void nw_reg_wave(const uint32_t *__restrict__ pack,
   }
   const int AL = (int)(la & 0xFFFFu) + j;
   const int ML = AL - (int)(la >> 16);
-  if (active) out[pidx] = (float)ML / (float)AL;
+  if (active) *out = (float)ML / (float)AL;
+}
+
+// Rows over the shorter read: a pair's DP ends at row min(L2, L1 + kB), so
+// with rows over the longer read up to kB rows run past column L1 with a
+// shrinking band.  The recurrence is symmetric up to its tie rules, so a wave
+// runs transposed (rows over read 1) when none of its pairs has read 1 longer
+// than read 2, which is every pair of a length-sorted read set (i < j).
+__device__ __forceinline__ bool wave_transposed(const bool lane_has_pair, const int La, const int Lb) {
+  return __ballot(lane_has_pair && La > Lb) == 0;
+}
+
+// One wave: pairs order[base .. base+63] of a bucket of `count` pairs.
+template <int KB>
+__device__ __forceinline__ void nw_reg_wave(const uint32_t *__restrict__ pack,
+                                            const int64_t *__restrict__ poff,
+                                            const int32_t *__restrict__ len,
+                                            const int2 *__restrict__ pairs,
+                                            const uint32_t *__restrict__ order, int32_t count,
+                                            int base, float *__restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const int t = base + lane;
+  const bool active = t < count;
+  const uint32_t pidx = order[active ? t : count - 1];
+  const int2 pr = pairs[pidx];
+  const int La = len[pr.x], Lb = len[pr.y];
+  const uint32_t *sa = pack + poff[pr.x];
+  const uint32_t *sb = pack + poff[pr.y];
+  if (wave_transposed(active, La, Lb))
+    nw_reg_dp<KB, true>(sb, sa, Lb, La, active, out + pidx);
+  else
+    nw_reg_dp<KB, false>(sa, sb, La, Lb, active, out + pidx);
 }
 
 // -------------------------------------------------------- sliced kernel
@@ -257,7 +288,7 @@ constexpr int kDppWaveShr1 = 0x138;  // lane l takes lane l-1
 //     which lane g+1 computes first in this step and sends mid-step.
 // Positions >= W in the last lane are dead; the last real cell (KEND in the
 // last lane, KEND = (W - 1) mod C) sees an out-of-band left neighbour.
-template <int KEND, int MODE>
+template <int KEND, int MODE, bool TR>
 __device__ __forceinline__ void nw_slice_cell(const int k, const int i, const uint32_t s,
                                               int &Mt_k, int &Mv_k, uint32_t &LA_k,
                                               const int leftM, const uint32_t leftLA, int &Mh,
@@ -265,13 +296,14 @@ __device__ __forceinline__ void nw_slice_cell(const int k, const int i, const ui
                                               int &capM, uint32_t &capLA) {
   const int diag = Mt_k + (int)(9u * s);
   int M = max(max(diag, Mh), leftM);
+  asm volatile("" : "+v"(M));  // keeps the max3 whole (see nw_row)
   const bool cd = M == diag;
-  const bool cu = M == Mh;
-  uint32_t la = cd ? LA_k + (s << 16) : (cu ? LAu : leftLA + 1u);
+  const bool c2 = M == (TR ? leftM : Mh);  // the gap move that wins a tie (see nw_row)
+  uint32_t la = cd ? LA_k + (s << 16) : (c2 ? (TR ? leftLA + 1u : LAu) : (TR ? LAu : leftLA + 1u));
   constexpr uint32_t kPenDiag = ((uint32_t)(4 + kGO) << 16) | (uint16_t)kGO;
   constexpr uint32_t kPenUp = ((uint32_t)4 << 16) | (uint16_t)kGE;
   constexpr uint32_t kPenLeft = ((uint32_t)(4 + kGE) << 16) | 0u;
-  uint32_t pk = cd ? kPenDiag : (cu ? kPenUp : kPenLeft);
+  uint32_t pk = cd ? kPenDiag : (c2 ? (TR ? kPenLeft : kPenUp) : (TR ? kPenUp : kPenLeft));
   if (row_top(MODE)) {  // boundary cell (i, 0): M 0, ML 0, AL i; pen_h = pen_v = 0
     constexpr uint32_t kPenBoundary = (uint32_t)4 << 16;
     const bool b = k == k0;
@@ -294,31 +326,18 @@ __device__ __forceinline__ void nw_slice_cell(const int k, const int i, const ui
   }
 }
 
-// One wave: 64 / G pairs starting at order[wave * (64 / G)]; `red` is the
-// wave's 64 x 6 LDS reduction buffer.
-template <int C, int KEND>
-__device__ __forceinline__ void nw_sliced_wave(const uint32_t *__restrict__ pack,
-                                               const int64_t *__restrict__ poff,
-                                               const int32_t *__restrict__ len,
-                                               const int2 *__restrict__ pairs,
-                                               const uint32_t *__restrict__ order, int32_t count,
-                                               int KB, int G, int wave, int (*red)[6],
-                                               float *__restrict__ out) {
+// The DP of one wave's 64 / G pairs (lane g of group grp; columns over s1,
+// rows over s2, L2 = 0 for a lane without a pair); TR as in nw_reg_dp.  `red`
+// is the wave's 64 x 6 LDS reduction buffer.
+template <int C, int KEND, bool TR>
+__device__ __forceinline__ void nw_sliced_dp(const uint32_t *__restrict__ s1, const uint32_t *__restrict__ s2,
+                                             const int L1, const int L2, const bool has_pair, const int KB,
+                                             const int G, const int g, int (*red)[6],
+                                             float *__restrict__ out) {
   constexpr int NMW = (C + 15) / 16;
   constexpr int NB = NMW + 1;
   const int W = 2 * KB + 1;
   const int lane = threadIdx.x & 63;
-  const int groups = 64 / G;
-  const int grp = lane / G;
-  const int g = lane - grp * G;
-  const int slot = wave * groups + grp;
-  const bool has_pair = grp < groups && slot < count;
-  const uint32_t pidx = order[has_pair ? slot : count - 1];
-  const int2 pr = pairs[pidx];
-  const int L1 = len[pr.x];
-  const int L2 = has_pair ? len[pr.y] : 0;
-  const uint32_t *s1 = pack + poff[pr.x];
-  const uint32_t *s2 = pack + poff[pr.y];
   const int nw1 = (L1 + 15) >> 4;
   const bool is_end = g == G - 1;
   const int base_d = g * C;
@@ -388,8 +407,8 @@ __device__ __forceinline__ void nw_sliced_wave(const uint32_t *__restrict__ pack
     uint32_t LAu = upLA;
     if (active) {
       const bool last0 = KEND == 0 && is_end;
-      nw_slice_cell<KEND, MODE>(0, i, m[0] & 1u, Mt[0], Mv[0], LA[0], last0 ? kNeg : Mv[1], LA[1],
-                                Mh, LAu, k0, kc, capM, capLA);
+      nw_slice_cell<KEND, MODE, TR>(0, i, m[0] & 1u, Mt[0], Mv[0], LA[0], last0 ? kNeg : Mv[1], LA[1],
+                                    Mh, LAu, k0, kc, capM, capLA);
     }
     // lane g+1's first cell of row i-1 is this lane's "left" for its last cell
     // (a DPP wave shift: lane l reads lane l+1; the last lane of a group ignores it)
@@ -409,8 +428,8 @@ __device__ __forceinline__ void nw_sliced_wave(const uint32_t *__restrict__ pack
           lM = (k == KEND && is_end) ? kNeg : Mv[k + 1];
           lLA = LA[k + 1];
         }
-        nw_slice_cell<KEND, MODE>(k, i, s, Mt[k], Mv[k], LA[k], lM, lLA, Mh, LAu, k0, kc, capM,
-                                  capLA);
+        nw_slice_cell<KEND, MODE, TR>(k, i, s, Mt[k], Mv[k], LA[k], lM, lLA, Mh, LAu, k0, kc, capM,
+                                      capLA);
       }
       if (row_captures(MODE) && kc >= 0 && kc < C && i >= L1 - KB) {
         const int Mr = capM - 4 * i;
@@ -491,7 +510,7 @@ __device__ __forceinline__ void nw_sliced_wave(const uint32_t *__restrict__ pack
     }
     uint32_t la;
     int j;
-    if (iM > jM) {
+    if (TR ? !(jM > iM) : iM > jM) {  // see nw_reg_dp
       la = iLA;
       j = ij;
     } else {
@@ -500,8 +519,34 @@ __device__ __forceinline__ void nw_sliced_wave(const uint32_t *__restrict__ pack
     }
     const int AL = (int)(la & 0xFFFFu) + j;
     const int ML = AL - (int)(la >> 16);
-    out[pidx] = (float)ML / (float)AL;
+    *out = (float)ML / (float)AL;
   }
+}
+
+// One wave: 64 / G pairs starting at order[wave * (64 / G)].
+template <int C, int KEND>
+__device__ __forceinline__ void nw_sliced_wave(const uint32_t *__restrict__ pack,
+                                               const int64_t *__restrict__ poff,
+                                               const int32_t *__restrict__ len,
+                                               const int2 *__restrict__ pairs,
+                                               const uint32_t *__restrict__ order, int32_t count,
+                                               int KB, int G, int wave, int (*red)[6],
+                                               float *__restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const int groups = 64 / G;
+  const int grp = lane / G;
+  const int g = lane - grp * G;
+  const int slot = wave * groups + grp;
+  const bool has_pair = grp < groups && slot < count;
+  const uint32_t pidx = order[has_pair ? slot : count - 1];
+  const int2 pr = pairs[pidx];
+  const int La = len[pr.x], Lb = len[pr.y];
+  const uint32_t *sa = pack + poff[pr.x];
+  const uint32_t *sb = pack + poff[pr.y];
+  if (wave_transposed(has_pair, La, Lb))
+    nw_sliced_dp<C, KEND, true>(sb, sa, Lb, has_pair ? La : 0, has_pair, KB, G, g, red, out + pidx);
+  else
+    nw_sliced_dp<C, KEND, false>(sa, sb, La, has_pair ? Lb : 0, has_pair, KB, G, g, red, out + pidx);
 }
 
 // ------------------------------------------------------------ fused launches

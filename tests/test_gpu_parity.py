@@ -188,21 +188,24 @@ def test_kmer_long_reads_vs_oracle(ctx, K):
         assert np.array_equal(bits(d), bits(od))
 
 
-@pytest.mark.parametrize("K", range(1, 7))
+@pytest.mark.parametrize("K", range(1, 9))
 def test_kmer_exact_pass_forms(ctx, K):
-    """K <= 6: the exact counts in the tile kernel and in a pass of their own
-    over the tile's candidates give the oracle's pairs and distances, at a
-    threshold where most candidates pass and at one where most fail."""
+    """Every exact-count form that applies to K gives the oracle's pairs and
+    distances: in the tile kernel (K <= 6), in a pass of its own over the
+    tile's candidates (hashed for K >= 6) and over exact-bitmap candidates
+    (K >= 6), at a threshold where most candidates pass and at one where
+    most fail."""
     say("kmer exact pass forms K", K)
     rng = np.random.default_rng(3000 + K)
     seqs = random_reads(rng, 300, max(K, 8), 300, repeats=True)
     r = ca.Reads.from_list(seqs)
     ro = O.Reads.from_list(seqs)
     ctx.upload_reads(ca.KMER, r)
+    modes = ([0] if K <= 6 else []) + [1] + ([2] if K >= 6 else [])
     try:
         for thr in (0.25, 0.6):
             op, od = O.kmer_all(ro, K, thr, nthreads=1)
-            for mode in (0, 1):
+            for mode in modes:
                 ctx.kmer_exact_pass(mode)
                 ctx.kmer_run(K, thr)
                 p, d = ctx.kmer_fetch()
@@ -210,6 +213,27 @@ def test_kmer_exact_pass_forms(ctx, K):
                 assert np.array_equal(bits(d), bits(od)), (thr, mode)
     finally:
         ctx.kmer_exact_pass(-1)
+
+
+@pytest.mark.parametrize("K", [6, 8])
+def test_kmer_autotune(ctx, K):
+    """The k-mer autotuner picks a form that applies to K and leaves the
+    oracle's result."""
+    say("kmer autotune K", K)
+    seqs, _, _ = synth.make("reads1k")
+    r = ca.Reads.from_list(seqs)
+    ro = O.Reads.from_list(seqs)
+    thr = ca.kmer_default_threshold(r.n)
+    ctx.upload_reads(ca.KMER, r)
+    try:
+        mode = ctx.kmer_autotune(K, thr, repeats=1)
+        assert mode in ((0, 1, 2) if K == 6 else (1, 2))
+        p, d = ctx.kmer_fetch()
+    finally:
+        ctx.kmer_exact_pass(-1)
+    op, od = O.kmer_all(ro, K, thr, nthreads=O.num_threads())
+    order = np.lexsort((op[:, 1], op[:, 0]))
+    assert np.array_equal(p, op[order]) and np.array_equal(bits(d), bits(od[order]))
 
 
 def test_kmer_tile_ranges_partition(ctx):

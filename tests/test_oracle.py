@@ -418,6 +418,23 @@ def test_nw_as_shipped_settings_same_distances():
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
+@pytest.mark.parametrize("K", [4, 6, 8])
+def test_kmer_rows_is_kmer_all_on_those_rows(K):
+    """The row-list form (used to sample full-size runs) gives exactly the
+    pairs and distances of the all-pairs form whose first read is listed;
+    repeated and unsorted rows are taken once."""
+    seqs, _, _ = synth.make("reads1k", n=400)
+    r = O.Reads.from_list(seqs)
+    thr = 0.3
+    ap, ad = O.kmer_all(r, K, thr, nthreads=1)
+    rows = np.array([399, 0, 7, 7, 200, 398, 123], np.int32)
+    p, d = O.kmer_rows(r, K, thr, rows, nthreads=3)
+    order = np.lexsort((p[:, 1], p[:, 0]))
+    sel = np.isin(ap[:, 0], rows)
+    assert sel.sum() > 0
+    assert np.array_equal(p[order], ap[sel]) and np.array_equal(d[order].view(np.uint32), ad[sel].view(np.uint32))
+
+
 def _bucket(codes):
     """The k = 6 candidate filter's bucket of a code: the top 11 bits of
     code * 2654435761 mod 2^32 (kmer_bucket in crispy_amd/csrc/kmer.hip)."""

@@ -453,3 +453,59 @@ def test_hand_cases_kill_mutations(tmp_path):
         if _case_result(L, s1, s2, band) == (ml, al, sc):
             survivors.append((nm, killer))
     assert not survivors, f"mutations not told apart by their named case: {survivors}"
+
+
+# The device DP may run transposed: rows over s1 and columns over s2 (the
+# shorter read as rows costs fewer band rows).  Transposition turns the
+# reference's "up" (same row) into the "left" (previous row) of the new frame
+# and its last row into the new frame's last column.  The penalties, band
+# and boundary are symmetric, so the reference's result comes out when the
+# new frame (a) lets its "left" win a tie between the two gap moves and (b)
+# takes its last row's best unless its last column's is strictly larger.
+TRANSPOSED = [
+    ("""            } else if (m == up) {
+                U[cur][j] = 4; ML[cur][j] = ML[cur][j - 1] + 1; AL[cur][j] = AL[cur][j - 1] + 1;
+            } else {
+                U[cur][j] = 2; ML[cur][j] = ML[pre][j] + 1; AL[cur][j] = AL[pre][j] + 1;
+            }""",
+     """            } else if (m == left) {
+                U[cur][j] = 2; ML[cur][j] = ML[pre][j] + 1; AL[cur][j] = AL[pre][j] + 1;
+            } else {
+                U[cur][j] = 4; ML[cur][j] = ML[cur][j - 1] + 1; AL[cur][j] = AL[cur][j - 1] + 1;
+            }"""),
+    ("    if (bestCol > bestRow) {", "    if (!(bestRow > bestCol)) {"),
+]
+
+
+def _tie_heavy_pairs(rng, count, max_len):
+    for _ in range(count):
+        alph = np.frombuffer(b"AGTC", np.uint8)[:int(rng.integers(1, 5))]
+        l1, l2 = (int(x) for x in rng.integers(1, max_len + 1, 2))
+        yield (alph[rng.integers(0, len(alph), l1)].tobytes(), alph[rng.integers(0, len(alph), l2)].tobytes(),
+               int(rng.integers(1, 13)))
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="gcc builds the transposed copies")
+def test_transposed_frame_gives_the_reference_result(tmp_path):
+    """A copy of the oracle with (a) and (b), called with the reads exchanged,
+    returns the oracle's (ML, AL, score) on 6,000 random pairs over 1- to
+    4-letter alphabets (ties everywhere) and lengths 1-40; each of (a) and (b)
+    alone gives a different result on some of them."""
+    built = {}
+    for name, subs in (("transposed", TRANSPOSED), ("tie_only", TRANSPOSED[:1]), ("final_only", TRANSPOSED[1:])):
+        L = ctypes.CDLL(_build_mutant(name, subs, str(tmp_path)))
+        L.oracle_nw_pair.restype = ctypes.c_float
+        built[name] = L
+    ref = ctypes.CDLL(_build_mutant("reference_frame", [], str(tmp_path)))
+    ref.oracle_nw_pair.restype = ctypes.c_float
+    rng = np.random.default_rng(4242)
+    differ = {"tie_only": 0, "final_only": 0}
+    n = 0
+    for s1, s2, band in _tie_heavy_pairs(rng, 6000, 40):
+        want = _case_result(ref, s1, s2, band)
+        assert _case_result(built["transposed"], s2, s1, band) == want, (s1, s2, band)
+        for k in differ:
+            differ[k] += _case_result(built[k], s2, s1, band) != want
+        n += 1
+    print("transposed frame: pairs where (a) or (b) alone differ", differ, "of", n)
+    assert differ["tie_only"] > 0 and differ["final_only"] > 0, differ
