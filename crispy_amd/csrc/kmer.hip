@@ -444,10 +444,80 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
     }
   }
 
-  // ---- bound: a 16-bit mask of possible pairs per fragment, branch-free
   const bool diag = bi == bj;
   const int rlim = n - row0, clim = n - col0;  // valid rows / columns of the tile
   const bool partial = diag || rlim < kTile || clim < kTile;  // block-uniform
+  if constexpr (CAND) {
+    // ---- candidate form: each accumulator value's bound compare is a lane
+    // mask of the wave; a value with any survivor queues them (entry
+    // li | lj' << 7 | popcount << 14), one atomic per flush reserves their
+    // place in this block's output segment.  The queue is the wave's own
+    // rows of the chunk buffer: after the main loop only this wave reads them.
+    constexpr int kQueue = 64 * kRow;
+    static_assert(kQueue >= 1024 + 256, "a fragment adds at most 16 x 64 entries");
+    uint32_t *queue = Ps + wv * 64 * kRow;
+    int qcount = 0;  // wave-uniform
+    auto flush = [&]() {
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      unsigned long long base = 0;
+      if (lane == 0) base = atomicAdd(counter, (unsigned long long)qcount);
+      const int64_t b0 = (int64_t)__shfl(base, 0, 64);
+      for (int e = lane; e < qcount; e += 64) {
+        const uint32_t q = queue[e];
+        const int64_t pos = b0 + e;
+        if (pos < cap)
+          out_keys[pos] = ((uint64_t)(row0 + (int)(q & 127u)) << kshift) | (uint64_t)(col0 + (int)((q >> 7) & 127u)) |
+                          ((uint64_t)((q >> 14) & 0x1FFFu) << kCandPopShift);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      qcount = 0;
+    };
+    // PART: the tile is on the diagonal or the edge (i < j, i < n, j < n
+    // checked per pair); otherwise the mask is the compare alone
+    auto fragments = [&](auto PART) {
+      constexpr bool P = decltype(PART)::value;
+      static_for<2>([&](auto FA) {
+        constexpr int fa = decltype(FA)::value;
+        const int rb = wv * 64 + fa * 32 + 4 * g;  // rows rb + 8q + t hold accumulators 4q + t
+        lds_f4 gr[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) gr[q] = *reinterpret_cast<const lds_f4 *>(&sG[rb + 8 * q]);
+        static_for<4>([&](auto FB) {
+          constexpr int fb = decltype(FB)::value;
+          if (qcount > kQueue - 1024) flush();
+          const int cj = fb * 32 + r32;  // the lane's column in the tile
+          const float gc = sG[kTile + cj];
+          const uint32_t ebase = (uint32_t)rb | ((uint32_t)cj << 7);
+          static_for<16>([&](auto R) {
+            constexpr int r = decltype(R)::value;
+            constexpr int dr = 8 * (r >> 2) + (r & 3);
+            // popcount >= min(G_i, G_j): the pair may pass
+            bool keep = acc[fa][fb][r] >= umin_f(gr[r >> 2][r & 3], gc);
+            if constexpr (P) {
+              const int ri = rb + dr;
+              keep = keep && (!diag || ri < cj) && ri < rlim && cj < clim;
+            }
+            const uint64_t ball = __ballot(keep);
+            if (ball) {  // wave-uniform
+              if (keep) {  // slot: the kept lanes below this one
+                const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(ball >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)ball, 0u));
+                queue[qcount + (int)below] = (ebase + (uint32_t)dr) | ((uint32_t)acc[fa][fb][r] << 14);
+              }
+              qcount += __popcll(ball);
+            }
+          });
+        });
+      });
+    };
+    if (partial)
+      fragments(std::true_type{});
+    else
+      fragments(std::false_type{});
+    if (qcount > 0) flush();
+    return;
+  }
+  // ---- bound: a 16-bit mask of possible pairs per fragment, branch-free
   uint32_t keep[8];
   int nkeep = 0;
   static_for<2>([&](auto FA) {
