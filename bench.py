@@ -306,7 +306,7 @@ def run_workload(ca, workload, K_override, steps, warmup, graphs, output_side=Tr
     width are tuned on the workload and the timed steps run tuned.  Returns
     a dict of the measurements plus the context."""
     from crispy_amd import synth
-    from crispy_amd.pipeline import fused_output_files
+    from crispy_amd.pipeline import OutputBuffers, fused_output_files
     t0 = time.perf_counter()
     seqs, _, cfg = synth.make(workload)
     reads = ca.Reads.from_list(seqs)
@@ -320,10 +320,12 @@ def run_workload(ca, workload, K_override, steps, warmup, graphs, output_side=Tr
     ctx.upload_reads(ca.NW, reads)
     ctx.graphs(graphs)
 
+    host_out = OutputBuffers()  # the output files' host arrays, reused from step to step
+
     def step(out=True):
         kc = ctx.kmer_run(K, kthr)
         nc = ctx.nw_run_on_kmer(band, nthr)
-        files = fused_output_files(ctx) if out else None
+        files = fused_output_files(ctx, host_out) if out else None
         return kc, nc, files
 
     for _ in range(warmup):

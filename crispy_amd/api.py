@@ -111,6 +111,8 @@ def load_library():
         "crispy_server_fetch_csr": ([_vp, ctypes.c_int64, ctypes.c_int32, _i64p, _i32p, _f32p], ctypes.c_int),
         "crispy_server_status": ([_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                   _i64p, _i64p], ctypes.c_int),
+        "crispy_server_job_solo": ([_vp, ctypes.c_int64], ctypes.c_int),
+        "crispy_ctx_set_mem_limit": ([_vp, ctypes.c_int64], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -401,13 +403,18 @@ class Context:
                                                  max_files, ctypes.byref(nf)))
         return ends[:nf.value]
 
-    def nw_fetch_sorted(self, first=0, count=None):
+    def nw_fetch_sorted(self, first=0, count=None, pairs=None, dists=None):
         """Result entries [first, first + count) ordered by (dist, i, j), sorted
-        on the device: one output file's contents."""
+        on the device: one output file's contents.  `pairs` ((count, 2) int32)
+        and `dists` ((count,) float32), C-contiguous, receive them when given
+        (so a caller can reuse its host buffers); returns the two arrays."""
         if count is None:
             count = self.nw_count - first
-        p = np.zeros((count, 2), np.int32)
-        d = np.zeros(count, np.float32)
+        p = np.empty((count, 2), np.int32) if pairs is None else pairs
+        d = np.empty(count, np.float32) if dists is None else dists
+        if p.shape != (count, 2) or p.dtype != np.int32 or not p.flags.c_contiguous or \
+                d.shape != (count,) or d.dtype != np.float32 or not d.flags.c_contiguous:
+            raise ValueError("pairs must be C-contiguous (count, 2) int32 and dists (count,) float32")
         self._check(self._L.crispy_nw_fetch_sorted(self._h, first, count, _p(p, _i32p), _p(d, _f32p)))
         return p, d
 
@@ -449,6 +456,10 @@ class Context:
 
     def sync(self):
         self._check(self._L.crispy_ctx_sync(self._h))
+
+    def mem_limit(self, nbytes):
+        """Caps the device bytes the context's buffers may grow to (0: no cap)."""
+        self._check(self._L.crispy_ctx_set_mem_limit(self._h, int(nbytes)))
 
     def device_bytes(self):
         return int(self._L.crispy_ctx_device_bytes(self._h))
@@ -520,9 +531,10 @@ class Server:
         self._job_check(jid, self._L.crispy_server_job_times(
             self._h, jid, ctypes.byref(s), ctypes.byref(b), ctypes.byref(e), ctypes.byref(est),
             ctypes.byref(held)))
+        solo = self._L.crispy_server_job_solo(self._h, jid)
         return dict(kmer_pairs=kp.value, nw_pairs=npairs.value, queue_ms=q.value, run_ms=r.value,
                     submit_ms=s.value, start_ms=b.value, end_ms=e.value, est_bytes=est.value,
-                    held_bytes=held.value)
+                    held_bytes=held.value, ran_alone_again=solo == 1)
 
     def fetch(self, jid):
         """Waits for job `jid`, returns (pairs (n, 2) int32, dists float32) and forgets it."""

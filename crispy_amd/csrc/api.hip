@@ -22,6 +22,10 @@ int set_err(crispy_ctx *ctx, int code, const char *fmt, ...) {
 int ensure(crispy_ctx *ctx, DevBuf &b, size_t bytes) {
   if (bytes == 0) bytes = 256;
   if (b.bytes >= bytes) return CRISPY_OK;
+  if (ctx->mem_limit > 0 && ctx->dev_bytes - (int64_t)b.bytes + (int64_t)bytes > ctx->mem_limit)
+    return set_err(ctx, CRISPY_ERR_NOMEM, "growing a buffer to %zu bytes would take the context past its "
+                   "device memory limit (%lld of %lld bytes held)", bytes, (long long)ctx->dev_bytes,
+                   (long long)ctx->mem_limit);
   if (b.p) {
     CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     CRISPY_HIP(ctx, hipFree(b.p));
@@ -285,6 +289,12 @@ int crispy_ctx_sync(crispy_ctx *ctx) {
 }
 
 int64_t crispy_ctx_device_bytes(const crispy_ctx *ctx) { return ctx ? ctx->dev_bytes : 0; }
+
+int crispy_ctx_set_mem_limit(crispy_ctx *ctx, int64_t bytes) {
+  if (!ctx || bytes < 0) return CRISPY_ERR_ARG;
+  ctx->mem_limit = bytes;
+  return CRISPY_OK;
+}
 
 int64_t crispy_band_cells(int32_t len1, int32_t len2, int32_t band) {
   if (len1 <= 0 || len2 <= 0) return 0;

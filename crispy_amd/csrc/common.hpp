@@ -198,6 +198,7 @@ struct crispy_ctx {
   std::vector<hipEvent_t> pool;
   std::map<std::string, KStat> stats;
   int64_t dev_bytes = 0;
+  int64_t mem_limit = 0;  // device bytes the buffers may grow to (0: no limit)
 };
 
 namespace crispy {

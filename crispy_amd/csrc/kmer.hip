@@ -69,7 +69,7 @@ __global__ __launch_bounds__(256) void kmer_profile_kernel(
   __shared__ uint32_t hist[NBINS];
   __shared__ uint32_t bm[WP];
   __shared__ uint32_t hbm[HASH ? kHashWords : 1];
-  __shared__ int scan[256];
+  __shared__ int wsum[4];
   __shared__ int s_distinct, s_hdistinct;
   const int r = blockIdx.x;
   const int tid = threadIdx.x;
@@ -110,30 +110,36 @@ __global__ __launch_bounds__(256) void kmer_profile_kernel(
     if (bits) atomicOr(&bm[b0 >> 5], bits << (b0 & 31));
   }
   if (ndist) atomicAdd(&s_distinct, ndist);
-  scan[tid] = nrep;
-  __syncthreads();
-  for (int off = 1; off < 256; off <<= 1) {
-    const int add = tid >= off ? scan[tid - off] : 0;
-    __syncthreads();
-    scan[tid] += add;
-    __syncthreads();
+  // where this thread's repeat entries go: an inclusive scan of nrep within
+  // the wave, then the totals of the waves before it
+  const int lane = tid & 63, wv = tid >> 6;
+  int incl = nrep;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int v = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += v;
   }
+  if (lane == 63) wsum[wv] = incl;
+  __syncthreads();
+  int before = 0;
+#pragma unroll
+  for (int w = 0; w < 3; ++w) before += w < wv ? wsum[w] : 0;
   if (nrep) {
-    uint32_t *out = rlist + roff[r] + (scan[tid] - nrep);
+    uint32_t *out = rlist + roff[r] + (before + incl - nrep);
 #pragma unroll
     for (int k = 0; k < BPT; ++k) {
       const uint32_t c = hist[b0 + k];
       if (c >= 2) *out++ = ((uint32_t)(b0 + k) << 16) | (c - 1);
     }
   }
-  if (tid == 255) rcnt[r] = scan[255];
+  if (tid == 255) rcnt[r] = before + incl;  // the last thread's inclusive sum: every entry
   if (tid == 0) {
     Tout[r] = T;
     excess[r] = T - s_distinct;  // sum over codes of (count - 1)
   }
   uint32_t *dst = bitmap + (int64_t)r * WP;
   for (int w = tid; w < WP; w += 256) dst[w] = bm[w];
-  if (HASH) {  // the scan's barriers ordered the bucket bits before these reads
+  if (HASH) {  // the barrier after the scan ordered the bucket bits before these reads
     if (tid < kHashWords) {
       const uint32_t w = hbm[tid];
       hbitmap[(int64_t)r * kHashWords + tid] = w;

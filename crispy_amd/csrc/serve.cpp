@@ -32,6 +32,9 @@ struct Job {
   float kthr = 0.f, nthr = 0.f;
   int64_t est = 0;
   int64_t held = 0;  // device bytes the slot's context held after the job
+  // outgrew its charge while other jobs ran: queued again at the head,
+  // charged the whole budget, so it runs alone
+  bool solo = false;
   bool done = false;
   bool consumed = false;  // fetched: results handed out, the job forgotten
   int status = CRISPY_OK;
@@ -115,7 +118,9 @@ void worker_loop(crispy_server *srv, int slot) {
             srv->peak_reserved = std::max(srv->peak_reserved, srv->reserved);
             srv->charge[slot] = need;
             srv->running += 1;
-            job->t_start = Clock::now();
+            if (!job->solo) job->t_start = Clock::now();
+            // the job's buffers may not outgrow what it is charged
+            crispy_ctx_set_mem_limit(ctx, need);
             break;
           }
           if (held > 0) {
@@ -135,17 +140,31 @@ void worker_loop(crispy_server *srv, int slot) {
     run_job(ctx, *job);
     {
       std::lock_guard<std::mutex> lk(srv->mu);
-      job->t_end = Clock::now();
-      job->done = true;
       const int64_t held = crispy_ctx_device_bytes(ctx);
-      job->held = held;
       srv->reserved += held - srv->charge[slot];
       srv->peak_reserved = std::max(srv->peak_reserved, srv->reserved);
       srv->charge[slot] = held;
       srv->running -= 1;
-      // the input copies are no longer needed
-      std::string().swap(job->kbases);
-      std::string().swap(job->nbases);
+      if (job->status == CRISPY_ERR_NOMEM && !job->solo) {
+        // it needed more than its estimate: run it again alone, with the
+        // whole budget, before the jobs behind it
+        job->solo = true;
+        job->est = srv->budget;
+        job->status = CRISPY_OK;
+        job->error.clear();
+        job->pairs.clear();
+        job->dists.clear();
+        job->row_ptr.clear();
+        srv->queue.push_front(job);
+        job.reset();
+      } else {
+        job->t_end = Clock::now();
+        job->done = true;
+        job->held = held;
+        // the input copies are no longer needed
+        std::string().swap(job->kbases);
+        std::string().swap(job->nbases);
+      }
     }
     srv->cv_done.notify_all();
     srv->cv_work.notify_all();
@@ -361,6 +380,14 @@ int crispy_server_fetch_csr(crispy_server *srv, int64_t job_id, int32_t n_rows, 
   srv->jobs.erase(job_id);
   srv->cv_done.notify_all();
   return status;
+}
+
+int crispy_server_job_solo(crispy_server *srv, int64_t job_id) {
+  if (!srv) return CRISPY_ERR_ARG;
+  std::lock_guard<std::mutex> lk(srv->mu);
+  auto it = srv->jobs.find(job_id);
+  if (it == srv->jobs.end() || it->second->consumed) return CRISPY_ERR_ARG;
+  return it->second->solo ? 1 : 0;
 }
 
 int crispy_server_status(crispy_server *srv, int *running, int *queued, int64_t *reserved_bytes,

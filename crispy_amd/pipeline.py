@@ -79,17 +79,42 @@ def compute_gen_dist(ctx: Context, reads: Reads, in_pair_path: str, pair_prefix:
     return nfiles, total
 
 
-def fused_output_files(ctx: Context):
+class OutputBuffers:
+    """Grow-only host arrays that fused_output_files can fill instead of
+    allocating new ones per call (fresh host pages cost a page fault each on
+    the first copy); the files it returns are then views into them, valid
+    until the next call with the same buffers."""
+
+    def __init__(self):
+        self.pairs = np.empty((0, 2), np.int32)
+        self.dists = np.empty(0, np.float32)
+
+    def reserve(self, n):
+        if len(self.dists) < n:
+            self.pairs = np.empty((n, 2), np.int32)
+            self.dists = np.empty(n, np.float32)
+            self.pairs.fill(0)  # touch the pages once
+            self.dists.fill(0)
+
+
+def fused_output_files(ctx: Context, buffers: OutputBuffers = None):
     """The output files of the current NW result (crispy_nw_run_on_kmer): the
     reference's file split of its input (the k-mer pairs read in
     CHUNK_PAIRS chunks, a file closed at >= FLUSH_PAIRS, genCPU.cpp:482-505)
     and each file's pairs in (dist, i, j) order, sorted on the device.
-    Returns a list of (pairs (n, 2) int32, dists float32)."""
+    Returns a list of (pairs (n, 2) int32, dists float32); with `buffers`,
+    views into them."""
     out = []
     first = 0
+    if buffers is not None:
+        buffers.reserve(ctx.nw_count)
     for end in ctx.nw_file_split(CHUNK_PAIRS, FLUSH_PAIRS):
-        out.append(ctx.nw_fetch_sorted(first, int(end) - first))
-        first = int(end)
+        end = int(end)
+        if buffers is None:
+            out.append(ctx.nw_fetch_sorted(first, end - first))
+        else:
+            out.append(ctx.nw_fetch_sorted(first, end - first, buffers.pairs[first:end], buffers.dists[first:end]))
+        first = end
     return out
 
 

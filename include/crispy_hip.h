@@ -60,6 +60,11 @@ const char *crispy_ctx_error(const crispy_ctx *ctx);
 int crispy_ctx_sync(crispy_ctx *ctx);
 /* Device bytes currently held by the context. */
 int64_t crispy_ctx_device_bytes(const crispy_ctx *ctx);
+/* Caps the device bytes the context's buffers may grow to (0 = no cap): a
+ * call that would grow them past it fails with CRISPY_ERR_NOMEM and leaves
+ * them as they were.  No counterpart in the reference: the job server charges
+ * each running job against its HBM budget with it. */
+int crispy_ctx_set_mem_limit(crispy_ctx *ctx, int64_t bytes);
 /* Frees every device buffer of the context (reads, profiles, results); the
  * streams stay.  Buffers otherwise only grow, so a long-lived context holds
  * the largest job it has run. */
@@ -324,6 +329,10 @@ int crispy_server_fetch(crispy_server *srv, int64_t job_id, int32_t *pairs, floa
  * job is kept). */
 int crispy_server_fetch_csr(crispy_server *srv, int64_t job_id, int32_t n_rows, int64_t *row_ptr,
                             int32_t *cols, float *vals);
+/* 1 if the job outgrew its admission charge while other jobs ran and was run
+ * again alone (charged the whole budget), 0 if not; CRISPY_ERR_ARG for an
+ * unknown or fetched job. */
+int crispy_server_job_solo(crispy_server *srv, int64_t job_id);
 /* Jobs running and queued, device bytes charged to the slots now and at most
  * so far (idle slots are charged what their contexts hold and give it back
  * when the queue head would not fit otherwise). */

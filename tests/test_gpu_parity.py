@@ -602,6 +602,14 @@ def test_device_file_split_and_sorted_fetch(ctx):
     nthr = ca.nw_default_threshold(r.n)
     ctx.nw_run_on_kmer(10, nthr)
     files = ca.pipeline.fused_output_files(ctx)
+    # into reused host buffers (grown once, then views): the same files
+    buf = ca.pipeline.OutputBuffers()
+    buf.reserve(7)
+    for _ in range(2):
+        again = ca.pipeline.fused_output_files(ctx, buf)
+        assert len(again) == len(files)
+        for (p0, d0), (p1, d1) in zip(files, again):
+            assert np.array_equal(p0, p1) and np.array_equal(bits(d0), bits(d1))
     ro = O.Reads.from_list(seqs)
     import tempfile
     with tempfile.TemporaryDirectory() as td:

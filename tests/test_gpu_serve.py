@@ -217,3 +217,70 @@ def test_csr_views(jobs, solo):
             srv.fetch(jid)
     assert np.array_equal(rp2, want_rp) and np.array_equal(cols2, p[:, 1])
     assert np.array_equal(vals2.view(np.uint32), d.view(np.uint32))
+
+
+def near_copies(n, L, seed):
+    """n reads that are one L-bp read with 0-2 substitutions each: every pair
+    passes both filters, far more pairs per read than a job estimate allows."""
+    rng = np.random.default_rng(seed)
+    alph = np.frombuffer(b"ACGT", np.uint8)
+    base = alph[rng.integers(0, 4, L)]
+    seqs = set()
+    while len(seqs) < n:
+        s = base.copy()
+        for _ in range(int(rng.integers(0, 3))):
+            s[int(rng.integers(0, L))] = alph[int(rng.integers(0, 4))]
+        seqs.add(s.tobytes())
+    return ca.Reads.from_list(sorted(seqs))
+
+
+def test_context_memory_limit():
+    """A context whose buffers would grow past its limit fails the call with
+    CRISPY_ERR_NOMEM and keeps what it held; with the limit lifted the same
+    run gives the same result as a context that never had one."""
+    reads = near_copies(1500, 300, 5)
+    job = dict(reads=reads, K=6, band=10, kthr=ca.kmer_default_threshold(reads.n),
+               nthr=ca.nw_default_threshold(reads.n))
+    want = alone(job)
+    with ca.Context(0) as c:
+        c.upload_reads(ca.KMER, reads)
+        c.upload_reads(ca.NW, reads)
+        held = c.device_bytes()
+        c.mem_limit(held + (1 << 20))
+        with pytest.raises(ca.CrispyError) as e:
+            c.kmer_run(job["K"], job["kthr"])
+            c.nw_run_on_kmer(job["band"], job["nthr"])
+        assert e.value.code == -3
+        assert c.device_bytes() <= held + (1 << 20)
+        c.mem_limit(0)
+        c.kmer_run(job["K"], job["kthr"])
+        c.nw_run_on_kmer(job["band"], job["nthr"])
+        assert same(c.nw_fetch(), want)
+
+
+def test_job_outgrowing_its_charge_runs_alone(jobs):
+    """A job whose buffers outgrow its admission estimate (3,000 near-copies of
+    one read: every pair passes, ~1,500 per read against the estimate's 256)
+    is held to its charge while other jobs run: it stops at the cap, runs again
+    alone charged the whole budget, and gives the result it gives alone; the
+    charged bytes never exceed the budget and the other jobs are unaffected."""
+    reads = near_copies(3000, 300, 6)
+    big = dict(reads=reads, K=6, band=10, kthr=ca.kmer_default_threshold(reads.n),
+               nthr=ca.nw_default_threshold(reads.n))
+    want = alone(big)
+    say(f"near copies: {len(want[1])} NW pairs, estimate {ca.job_bytes(reads, 6, 10)} bytes")
+    budget = 4 << 30
+    with ca.Server(0, 3, budget) as srv:
+        before = [submit(srv, j) for j in jobs[1:4]]
+        jid = submit(srv, big)
+        after = [submit(srv, j) for j in jobs[4:6]]
+        info = srv.wait(jid)
+        p, d = srv.fetch(jid)
+        others = [srv.fetch(i) for i in before + after]
+        st = srv.status()
+    say("big job:", info, "peak", st["peak_reserved_bytes"])
+    assert info["ran_alone_again"]
+    assert same((p, d), want)
+    assert st["peak_reserved_bytes"] <= budget
+    for k, r in zip([1, 2, 3, 4, 5], others):
+        assert same(r, alone(jobs[k]))
