@@ -443,10 +443,55 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
 #pragma unroll
       for (int fb = 0; fb < 4; ++fb)
         xb[fb] = *reinterpret_cast<const uint4 *>(&Ps[(kTile + fb * 32 + r32) * kRow + 8 * m + 4 * g]);
+      if constexpr (CAND) {
+      // the 4 bit planes software-pipelined in the wave: a column operand is
+      // re-extracted for the next plane right after the two MFMAs that read it,
+      // the row operands into a second set, so the wave's own VALU work runs
+      // between its MFMAs instead of in front of them (tile -4 % at 2.5e5 and
+      // 1e5 reads; the exact-count form's larger epilogue state spills with it)
+      auto ext = [](const uint4 &x, int pl) {
+        const uint32_t M = pl < 3 ? 0x11111111u << pl : 0x22222222u;
+        const int sh = pl < 3 ? 0 : 2;
+        mfma_v8i e;
+        e[0] = (int)((x.x >> sh) & M);
+        e[1] = (int)((x.y >> sh) & M);
+        e[2] = (int)((x.z >> sh) & M);
+        e[3] = (int)((x.w >> sh) & M);
+        e[4] = e[5] = e[6] = e[7] = 0;  // unused by FP4 operands
+        return e;
+      };
+      mfma_v8i ea[2], en[2], eb[4];
+#pragma unroll
+      for (int fa = 0; fa < 2; ++fa) ea[fa] = ext(xa[fa], 0);
+#pragma unroll
+      for (int fb = 0; fb < 4; ++fb) eb[fb] = ext(xb[fb], 0);
+      static_for<4>([&](auto PL) {
+        constexpr int pl = decltype(PL)::value;
+        constexpr int scale = pl < 3 ? kUnitScale + 2 - 2 * pl : kUnitScale;
+        static_for<4>([&](auto FB) {
+          constexpr int fb = decltype(FB)::value;
+#pragma unroll
+          for (int fa = 0; fa < 2; ++fa)
+            acc[fa][fb] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(ea[fa], eb[fb], acc[fa][fb], kFp4, kFp4,
+                                                                          0, scale, 0, kUnitScale);
+          if constexpr (pl < 3) {
+            eb[fb] = ext(xb[fb], pl + 1);
+            if constexpr (fb >= 2) en[fb - 2] = ext(xa[fb - 2], pl + 1);
+          }
+          __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);  // 2 MFMAs
+          __builtin_amdgcn_sched_group_barrier(0x002, pl < 2 ? (fb >= 2 ? 8 : 4) : (pl == 2 ? (fb >= 2 ? 16 : 8) : 0), 0);
+        });
+        if constexpr (pl < 3) {
+          ea[0] = en[0];
+          ea[1] = en[1];
+        }
+      });
+      } else {
       // bit planes 0..2 in a rolled loop (one plane's operands live at a time)
 #pragma unroll 1
       for (int s = 0; s < 3; ++s) mfma_step(xa, xb, 0x11111111u << s, 0, kUnitScale + 2 - 2 * s);
       mfma_step(xa, xb, 0x22222222u, 2, kUnitScale);
+      }
     }
   }
 
