@@ -447,8 +447,9 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
       // the 4 bit planes software-pipelined in the wave: a column operand is
       // re-extracted for the next plane right after the two MFMAs that read it,
       // the row operands into a second set, so the wave's own VALU work runs
-      // between its MFMAs instead of in front of them (tile -4 % at 2.5e5 and
-      // 1e5 reads; the exact-count form's larger epilogue state spills with it)
+      // between its MFMAs instead of in front of them (tile -6 % at 2.5e5
+      // reads; pinning the order with sched_group_barrier measured slower; the
+      // exact-count form's larger epilogue state spills with it)
       auto ext = [](const uint4 &x, int pl) {
         const uint32_t M = pl < 3 ? 0x11111111u << pl : 0x22222222u;
         const int sh = pl < 3 ? 0 : 2;
@@ -478,8 +479,6 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
             eb[fb] = ext(xb[fb], pl + 1);
             if constexpr (fb >= 2) en[fb - 2] = ext(xa[fb - 2], pl + 1);
           }
-          __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);  // 2 MFMAs
-          __builtin_amdgcn_sched_group_barrier(0x002, pl < 2 ? (fb >= 2 ? 8 : 4) : (pl == 2 ? (fb >= 2 ? 16 : 8) : 0), 0);
         });
         if constexpr (pl < 3) {
           ea[0] = en[0];
