@@ -338,14 +338,17 @@ def run_workload(ca, workload, K_override, steps, warmup, graphs, output_side=Tr
             step(output_side)
         ctx.sync()
         default_ms = (time.perf_counter() - t0) / steps * 1e3
+        default_bits = ctx.kmer_tile_bits()
         t0 = time.perf_counter()
         kmode = ctx.kmer_autotune(K, kthr, repeats=2)
         width = ctx.nw_autotune(band, repeats=2)
         # the defaults: exact-count form by pair count (-1), slice width by the cost model (0)
         default_mode = 1 if K > 6 or n * (n - 1) // 2 >= 10 ** 9 else 0
-        tune = {"kmer_exact_pass": kmode, "kmer_form": KMER_FORMS[kmode], "nw_slice_width": width,
+        tuned_bits = ctx.kmer_tile_bits()
+        tune = {"kmer_exact_pass": kmode, "kmer_form": KMER_FORMS[kmode], "kmer_tile_bits": tuned_bits,
+                "nw_slice_width": width,
                 "nw_slice_policy": "cost model" if width == 0 else f"{width} cells per lane",
-                "chosen_is_default": kmode == default_mode and width == 0,
+                "chosen_is_default": kmode == default_mode and width == 0 and tuned_bits == default_bits,
                 "default_ms_per_step": default_ms, "tuning_s": time.perf_counter() - t0}
         for _ in range(warmup):
             step()
@@ -455,8 +458,8 @@ def main():
 
     kt = kern["kmer_tile"]
     # the tile is an FP4 matrix product over the presence vectors (the k <= 6
-    # codes, or their 2,048 hash buckets for the k = 6 candidate filter): 2 ops
-    # x kdim per pair
+    # codes, or their 1,536 or 2,048 hash buckets for the k = 6 candidate
+    # filter): 2 ops x kdim per pair
     kdim = r["tile_bits"]
     kmer_tile_tops = kt["units"] * 2 * kdim / (kt["total_ms"] * 1e-3) / 1e12 if kt["total_ms"] else 0
     dp_launches = sum(kern[k]["launches"] for k in ("nw_band_reg", "nw_band_sliced", "nw_generic"))
@@ -480,7 +483,7 @@ def main():
                     "peak": MFMA_FP4_PEAK_TOPS, "unit": "TFLOP/s (FP4 MFMA)",
                     "frac": kmer_tile_tops / MFMA_FP4_PEAK_TOPS,
                     "work": (f"2 x {kdim} FP4 ops (presence-vector dot product"
-                             f"{', 4,096 6-mer codes hashed into 2,048 buckets' if kdim == 2048 else ''}) per read pair, "
+                             f"{f', 4,096 6-mer codes hashed into {kdim:,} buckets' if kdim in (1536, 2048) else ''}) per read pair, "
                              f"{int(kt['units'] / max(1, kt['launches']))} pairs per launch"),
                     "avg_launch_ms": kt["avg_ms"], "traffic": traffic, "traffic_source": src}
 

@@ -70,6 +70,7 @@ def load_library():
         "crispy_nw_set_slice_width": ([_vp, ctypes.c_int], ctypes.c_int),
         "crispy_set_graphs": ([_vp, ctypes.c_int], ctypes.c_int),
         "crispy_kmer_set_exact_pass": ([_vp, ctypes.c_int], ctypes.c_int),
+        "crispy_kmer_set_hash_buckets": ([_vp, ctypes.c_int], ctypes.c_int),
         "crispy_kmer_tile_bits": ([_vp], ctypes.c_int),
         "crispy_nw_autotune": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "crispy_kmer_autotune": ([_vp, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
@@ -331,7 +332,8 @@ class Context:
 
     def kmer_tile_bits(self):
         """Presence bits per read the last k-mer tile pass multiplied (4^min(K,6)
-        padded to >= 512, or 2,048 for the hashed k = 6 candidate filter)."""
+        padded to >= 512, or the bucket count of the hashed k = 6 candidate
+        filter)."""
         return int(self._L.crispy_kmer_tile_bits(self._h))
 
     def kmer_exact_pass(self, mode):
@@ -343,9 +345,16 @@ class Context:
         results are the same."""
         self._check(self._L.crispy_kmer_set_exact_pass(self._h, int(mode)))
 
+    def kmer_hash_buckets(self, buckets):
+        """Buckets of the hashed k = 6 candidate filter: 1536, 2048 or 0 (the
+        default, 1536).  The results are the same."""
+        self._check(self._L.crispy_kmer_set_hash_buckets(self._h, int(buckets)))
+
     def kmer_autotune(self, K, threshold, repeats=2):
-        """Times the k-mer run under each exact-count form that applies to K,
-        keeps the fastest (and its result); returns the chosen mode."""
+        """Times the k-mer run under each exact-count form that applies to K
+        (and each hash bucket count for the hashed form), keeps the fastest
+        (and its result); returns the chosen mode (kmer_tile_bits() tells the
+        bucket count)."""
         c = ctypes.c_int()
         self._check(self._L.crispy_kmer_autotune(self._h, K, threshold, repeats, ctypes.byref(c)))
         self.kmer_run(K, threshold)  # the chosen form's result and its count

@@ -36,8 +36,11 @@ constexpr int kGenericBucket = kKbSliceMax + 1;
 // k-mer tile geometry.
 constexpr int kTile = 128;        // reads per tile side
 constexpr int kChunkWords = 16;   // bitmap words staged in LDS per step
-constexpr int kHashBits = 2048;   // buckets of the hashed k = 6 candidate filter
-constexpr int kHashWords = kHashBits / 32;
+// Buckets of the hashed k = 6 candidate filter: 1,536 or 2,048 (48 or 64
+// bitmap words per read, whole chunks of kChunkWords); fewer buckets mean less
+// tile work and more candidates for the exact passes.
+constexpr int kHashWordsMax = 64;
+constexpr int kHashWordsMin = 48;
 // The candidate tiles append to kCandSegs output segments, each with its own
 // counter kSegStride u64 words (4 KB) from the next: one counter for the
 // whole chip serialises the per-wave reservations (~600k atomics per
@@ -112,10 +115,13 @@ struct KmerState {
   int K = 0;
   int profile_gen = -1;
   DevBuf bitmap, T, rcnt, roff, rlist, excess;
-  // k = 6 candidate filter: presence of the 4,096 codes hashed into 2,048
-  // buckets, and T - (distinct buckets); `hashed` when the profile holds them
+  // k = 6 candidate filter: presence of the 4,096 codes hashed into
+  // 32 x hash_words buckets, and T - (distinct buckets); `hashed` when the
+  // profile holds them.  hash_buckets: the setting (0 = default).
   DevBuf hbitmap, hexcess;
   bool hashed = false;
+  int hash_words = crispy::kHashWordsMax;
+  int hash_buckets = 0;
   // the repeat lists back to back in read order (rlistc + roffc[r])
   DevBuf roffc, rlistc;
   // K = 7, 8: signed-sorted K-mer lists and the k = 6 candidate pairs

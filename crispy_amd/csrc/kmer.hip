@@ -43,20 +43,25 @@
 namespace crispy {
 
 // --------------------------------------------------------------- profiles
-// Bucket of a k = 6 code in the hashed candidate filter (multiplicative hash,
-// top 11 bits).  Merging codes into buckets can only raise
-// sum_b min(count_a(b), count_b(b)), so the filter's bound stays an upper
-// bound of the k-mer matches.
-__host__ __device__ inline uint32_t kmer_bucket(uint32_t code) { return (code * 2654435761u) >> 21; }
-static_assert(kHashBits == 2048, "kmer_bucket yields 11 bits");
+// Bucket of a k = 6 code in the hashed candidate filter of 32 x HW buckets:
+// the multiplicative hash h = code * 2654435761 mod 2^32 scaled to the bucket
+// range, (h * buckets) >> 32 (for 2,048 buckets its top 11 bits).  Merging
+// codes into buckets can only raise sum_b min(count_a(b), count_b(b)), so the
+// filter's bound stays an upper bound of the k-mer matches.
+constexpr int kDefaultHashWords = kHashWordsMin;  // 1,536 buckets: the tile work saved outweighs the extra candidates (reads of 400-900 bp)
+template <int HW>
+__host__ __device__ inline uint32_t kmer_bucket(uint32_t code) {
+  static_assert(HW == kHashWordsMin || HW == kHashWordsMax, "1,536 or 2,048 buckets");
+  return (uint32_t)(((uint64_t)(code * 2654435761u) * (uint64_t)(32 * HW)) >> 32);
+}
 
 // One block per read: the codes of its T = L-K+1 k-mers (first base most
 // significant, as READ::formTuples kmerMain.cpp:117-154) counted in an LDS
 // histogram over all 4^K codes; the bins then give the presence bitmap, the
-// repeat entries in code order and excess = T - (distinct codes).  HASH
-// (K = 6) also writes the presence of the codes' buckets (kmer_bucket) and
-// hexcess = T - (distinct buckets).
-template <int K, bool HASH = false>
+// repeat entries in code order and excess = T - (distinct codes).  HW > 0
+// (K = 6) also writes the presence of the codes' buckets (kmer_bucket<HW>,
+// HW words per read) and hexcess = T - (distinct buckets).
+template <int K, int HW = 0>
 __global__ __launch_bounds__(256) void kmer_profile_kernel(
     const uint32_t *__restrict__ pack, const int64_t *__restrict__ poff,
     const int32_t *__restrict__ len, uint32_t *__restrict__ bitmap, int32_t *__restrict__ Tout,
@@ -65,10 +70,11 @@ __global__ __launch_bounds__(256) void kmer_profile_kernel(
   constexpr int WP = kmer_words(K);
   constexpr int NBINS = 1 << (2 * K);
   constexpr int BPT = NBINS >= 256 ? NBINS / 256 : 1;  // bins per thread (within one bitmap word)
+  constexpr bool HASH = HW > 0;
   static_assert(!HASH || K == 6, "the hashed filter covers k = 6");
   __shared__ uint32_t hist[NBINS];
   __shared__ uint32_t bm[WP];
-  __shared__ uint32_t hbm[HASH ? kHashWords : 1];
+  __shared__ uint32_t hbm[HASH ? HW : 1];
   __shared__ int wsum[4];
   __shared__ int s_distinct, s_hdistinct;
   const int r = blockIdx.x;
@@ -78,7 +84,7 @@ __global__ __launch_bounds__(256) void kmer_profile_kernel(
   for (int v = tid; v < NBINS; v += 256) hist[v] = 0;
   for (int w = tid; w < WP; w += 256) bm[w] = 0;
   if (HASH)
-    for (int w = tid; w < kHashWords; w += 256) hbm[w] = 0;
+    for (int w = tid; w < HW; w += 256) hbm[w] = 0;
   if (tid == 0) s_distinct = s_hdistinct = 0;
   __syncthreads();
   for (int t = tid; t < T; t += 256) {
@@ -103,7 +109,7 @@ __global__ __launch_bounds__(256) void kmer_profile_kernel(
       ndist += c > 0;
       nrep += c >= 2;
       if (HASH && c > 0) {
-        const uint32_t h = kmer_bucket((uint32_t)(b0 + k));
+        const uint32_t h = kmer_bucket<HASH ? HW : kHashWordsMax>((uint32_t)(b0 + k));
         atomicOr(&hbm[h >> 5], 1u << (h & 31));
       }
     }
@@ -140,9 +146,9 @@ __global__ __launch_bounds__(256) void kmer_profile_kernel(
   uint32_t *dst = bitmap + (int64_t)r * WP;
   for (int w = tid; w < WP; w += 256) dst[w] = bm[w];
   if (HASH) {  // the barrier after the scan ordered the bucket bits before these reads
-    if (tid < kHashWords) {
+    if (tid < HW) {
       const uint32_t w = hbm[tid];
-      hbitmap[(int64_t)r * kHashWords + tid] = w;
+      hbitmap[(int64_t)r * HW + tid] = w;
       atomicAdd(&s_hdistinct, __popc(w));
     }
     __syncthreads();
@@ -532,11 +538,28 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
         lds_f4 gr[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) gr[q] = *reinterpret_cast<const lds_f4 *>(&sG[rb + 8 * q]);
+        // the smallest row bound of the lane's 16 rows (non-negative floats
+        // order as their bits)
+        uint32_t grmin = __float_as_uint(gr[0][0]);
+        static_for<15>([&](auto R) {
+          constexpr int r = decltype(R)::value + 1;
+          grmin = min(grmin, __float_as_uint(gr[r >> 2][r & 3]));
+        });
         static_for<4>([&](auto FB) {
           constexpr int fb = decltype(FB)::value;
           if (qcount > kQueue - 1024) flush();
           const int cj = fb * 32 + r32;  // the lane's column in the tile
           const float gc = sG[kTile + cj];
+          // whole-fragment test first: a value can only be kept if the
+          // fragment's largest popcount in the lane reaches the lane's
+          // smallest bound min(G_i, G_j); most fragments of a large run
+          // keep nothing, and then cost 10 VALU instead of 32
+          uint32_t amax = __float_as_uint(acc[fa][fb][0]);
+          static_for<15>([&](auto R) {
+            constexpr int r = decltype(R)::value + 1;
+            amax = max(amax, __float_as_uint(acc[fa][fb][r]));
+          });
+          if (__ballot(amax >= min(grmin, __float_as_uint(gc))) == 0) return;  // wave-uniform
           const uint32_t ebase = (uint32_t)rb | ((uint32_t)cj << 7);
           static_for<16>([&](auto R) {
             constexpr int r = decltype(R)::value;
@@ -1064,7 +1087,9 @@ static int launch_profile(crispy_ctx *ctx, ReadStore &rs) {
   KmerState &km = ctx->km;
   if constexpr (K == 6) {
     if (km.hashed) {
-      kmer_profile_kernel<6, true><<<rs.n, 256, 0, ctx->stream>>>(
+      auto kernel = km.hash_words == kHashWordsMin ? kmer_profile_kernel<6, kHashWordsMin>
+                                                   : kmer_profile_kernel<6, kHashWordsMax>;
+      kernel<<<rs.n, 256, 0, ctx->stream>>>(
           rs.pack.as<uint32_t>(), rs.poff.as<int64_t>(), rs.len.as<int32_t>(), km.bitmap.as<uint32_t>(),
           km.T.as<int32_t>(), km.rcnt.as<int32_t>(), km.roff.as<int64_t>(), km.rlist.as<uint32_t>(),
           km.excess.as<int32_t>(), km.hbitmap.as<uint32_t>(), km.hexcess.as<int32_t>());
@@ -1133,16 +1158,17 @@ static double tile_units(int32_t n, int nb, int64_t tb, int64_t nt) {
 }
 
 // The tile pass: zero the result counter, then the tile kernel over slots
-// [tb, tb + nt) (a captured graph with graphs on).  HASHED: the candidate
-// form on the k = 6 bucket bitmaps (kHashWords words per read) and their
-// excess; otherwise the presence bitmaps of K.  CAND: the output is
-// kCandSegs segments of `cap` keys (counters in km.seg, their total and
-// largest count in km.seg_stat after the pass).
-template <int K, bool CAND, bool HASHED = false>
+// [tb, tb + nt) (a captured graph with graphs on).  HW > 0: the candidate
+// form on the k = 6 bucket bitmaps (HW words per read) and their excess;
+// otherwise the presence bitmaps of K.  CAND: the output is kCandSegs
+// segments of `cap` keys (counters in km.seg, their total and largest count
+// in km.seg_stat after the pass).
+template <int K, bool CAND, int HW = 0>
 static int launch_tiles(crispy_ctx *ctx, int32_t n, int nb, int64_t tb, int64_t nt, float thr,
                         int tshift, uint64_t *out_keys, float *out_dists, int64_t cap) {
+  constexpr bool HASHED = HW > 0;
   static_assert(!HASHED || (K == 6 && CAND), "the hashed filter is the k = 6 candidate form");
-  constexpr int WP = HASHED ? kHashWords : kmer_words(K);
+  constexpr int WP = HASHED ? HW : kmer_words(K);
   KmerState &km = ctx->km;
   const DevBuf &bits = HASHED ? km.hbitmap : km.bitmap;
   const DevBuf &excess = HASHED ? km.hexcess : km.excess;
@@ -1200,7 +1226,7 @@ static const TileFn kTilesCand[7] = {nullptr,
                                      launch_tiles<3, true>,
                                      launch_tiles<4, true>,
                                      launch_tiles<5, true>,
-                                     launch_tiles<6, true, true>};
+                                     launch_tiles<6, true>};
 
 // Profiles of the bitmap word size KP (KP = K for K <= 6, 6 above); every
 // read must hold at least one K-mer.
@@ -1208,6 +1234,7 @@ static int build_profiles(crispy_ctx *ctx, int KP, int K, bool hash) {
   ReadStore &rs = ctx->reads[CRISPY_READS_KMER];
   KmerState &km = ctx->km;
   km.hashed = hash && KP == 6;
+  km.hash_words = km.hash_buckets ? km.hash_buckets / 32 : kDefaultHashWords;
   // Rebuilt on every run: the per-read profiles are part of the stage's work.
   const int32_t n = rs.n;
   for (int32_t r = 0; r < n; ++r)
@@ -1226,7 +1253,7 @@ static int build_profiles(crispy_ctx *ctx, int KP, int K, bool hash) {
   CRISPY_TRY(ensure(ctx, km.rcnt, sizeof(int32_t) * (n + 1)));
   CRISPY_TRY(ensure(ctx, km.excess, sizeof(int32_t) * (n + 1)));
   if (km.hashed) {
-    CRISPY_TRY(ensure(ctx, km.hbitmap, sizeof(uint32_t) * (size_t)n * kHashWords));
+    CRISPY_TRY(ensure(ctx, km.hbitmap, sizeof(uint32_t) * (size_t)n * km.hash_words));
     CRISPY_TRY(ensure(ctx, km.hexcess, sizeof(int32_t) * (n + 1)));
   }
   CRISPY_TRY(ensure(ctx, km.roff, sizeof(int64_t) * (n + 1)));
@@ -1258,7 +1285,8 @@ static int build_profiles(crispy_ctx *ctx, int KP, int K, bool hash) {
        (uintptr_t)km.bitmap.p, (uintptr_t)km.T.p, (uintptr_t)km.rcnt.p, (uintptr_t)km.roff.p, (uintptr_t)km.rlist.p,
        (uintptr_t)km.excess.p, (uintptr_t)km.roffc.p, (uintptr_t)km.rlistc.p, (uintptr_t)ctx->temp.p,
        (uintptr_t)tbytes, (uintptr_t)(K > 6 ? km.loff.p : nullptr), (uintptr_t)(K > 6 ? km.lists.p : nullptr),
-       (uintptr_t)list_lds, (uintptr_t)km.hashed, (uintptr_t)(km.hashed ? km.hbitmap.p : nullptr),
+       (uintptr_t)list_lds, (uintptr_t)(km.hashed ? km.hash_words : 0),
+       (uintptr_t)(km.hashed ? km.hbitmap.p : nullptr),
        (uintptr_t)(km.hashed ? km.hexcess.p : nullptr)},
       "k-mer profile pass", [&](bool) -> int {
         CRISPY_TRY(kProfile[KP](ctx, rs));
@@ -1390,7 +1418,7 @@ int64_t crispy_kmer_num_tiles(int32_t n) { return num_slots((n + kTile - 1) / kT
 
 int crispy_kmer_tile_bits(crispy_ctx *ctx) {
   if (!ctx || ctx->km.K == 0) return 0;
-  return ctx->km.hashed ? kHashBits : 32 * kmer_words(ctx->km.K <= 6 ? ctx->km.K : 6);
+  return 32 * (ctx->km.hashed ? ctx->km.hash_words : kmer_words(ctx->km.K <= 6 ? ctx->km.K : 6));
 }
 
 int crispy_kmer_set_exact_pass(crispy_ctx *ctx, int mode) {
@@ -1400,28 +1428,45 @@ int crispy_kmer_set_exact_pass(crispy_ctx *ctx, int mode) {
   return CRISPY_OK;
 }
 
+int crispy_kmer_set_hash_buckets(crispy_ctx *ctx, int buckets) {
+  if (!ctx) return CRISPY_ERR_ARG;
+  if (buckets != 0 && buckets != 32 * kHashWordsMin && buckets != 32 * kHashWordsMax)
+    return set_err(ctx, CRISPY_ERR_ARG, "hash buckets %d: 0 (default), %d or %d", buckets, 32 * kHashWordsMin,
+                   32 * kHashWordsMax);
+  ctx->km.hash_buckets = buckets;
+  return CRISPY_OK;
+}
+
 int crispy_kmer_autotune(crispy_ctx *ctx, int K, float threshold, int repeats, int *chosen) {
   if (!ctx || repeats < 1) return CRISPY_ERR_ARG;
   if (K < 1 || K > 8) return set_err(ctx, CRISPY_ERR_ARG, "unsupported K=%d (1..8)", K);
   CRISPY_HIP(ctx, hipSetDevice(ctx->device));
-  // K <= 5 have no hashed form (2 = 1); K > 6 always take candidates (0 = 1)
-  std::vector<int> modes;
-  if (K <= 6) modes.push_back(0);
-  modes.push_back(1);
-  if (K >= 6) modes.push_back(2);
+  // K <= 5 have no hashed form (2 = 1); K > 6 always take candidates (0 = 1);
+  // the hashed form (mode 1, K >= 6) with each bucket count
+  std::vector<std::pair<int, int>> settings;  // (exact pass mode, hash buckets)
+  if (K <= 6) settings.push_back({0, 0});
+  if (K >= 6) {
+    settings.push_back({1, 32 * kHashWordsMax});
+    settings.push_back({1, 32 * kHashWordsMin});
+    settings.push_back({2, 0});
+  } else {
+    settings.push_back({1, 0});
+  }
   KmerState &km = ctx->km;
-  const int before = km.exact_pass;
+  const std::pair<int, int> before{km.exact_pass, km.hash_buckets};
   double best = 1e300;
-  int pick = before;
-  for (int mode : modes) {
-    km.exact_pass = mode;
+  std::pair<int, int> pick = before;
+  for (const auto &st : settings) {
+    km.exact_pass = st.first;
+    km.hash_buckets = st.second;
     double fastest = 1e300;
     for (int r = 0; r <= repeats; ++r) {  // the first run warms up (profiles, graphs, buffers)
       CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
       const auto t0 = std::chrono::steady_clock::now();
       const int rc = crispy_kmer_run(ctx, K, threshold, 0, -1, nullptr);
       if (rc != CRISPY_OK) {
-        km.exact_pass = before;
+        km.exact_pass = before.first;
+        km.hash_buckets = before.second;
         return rc;
       }
       CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -1430,13 +1475,14 @@ int crispy_kmer_autotune(crispy_ctx *ctx, int K, float threshold, int repeats, i
     }
     if (fastest < best) {
       best = fastest;
-      pick = mode;
+      pick = st;
     }
   }
-  km.exact_pass = pick;
+  km.exact_pass = pick.first;
+  km.hash_buckets = pick.second;
   // leave the result of the chosen setting (every setting gives the same one)
   CRISPY_TRY(crispy_kmer_run(ctx, K, threshold, 0, -1, nullptr));
-  if (chosen) *chosen = pick;
+  if (chosen) *chosen = pick.first;
   return CRISPY_OK;
 }
 
@@ -1480,7 +1526,9 @@ int crispy_kmer_run(crispy_ctx *ctx, int K, float threshold, int64_t tile_begin,
   const bool separate = km.exact_pass >= 1 || (km.exact_pass == -1 && (int64_t)n * (n - 1) / 2 >= kExactPassPairs);
   CRISPY_TRY(build_profiles(ctx, KP, K, (K > 6 || separate) && km.exact_pass != 2));
   // the k = 6 candidate tiles: on the hashed bitmaps, or on the exact ones
-  const TileFn tiles6 = km.hashed ? launch_tiles<6, true, true> : launch_tiles<6, true, false>;
+  const TileFn tiles6 = !km.hashed                          ? launch_tiles<6, true>
+                        : km.hash_words == kHashWordsMin ? launch_tiles<6, true, kHashWordsMin>
+                                                         : launch_tiles<6, true, kHashWordsMax>;
   // first guess at the result size: the pairs in the range, at most 4 Mi
   const int64_t pairs_in_range = std::min<int64_t>(nt * kTile * kTile, (int64_t)n * (n - 1) / 2);
   const int64_t guess = std::min<int64_t>((int64_t)1 << 22, std::max<int64_t>(4096, pairs_in_range));

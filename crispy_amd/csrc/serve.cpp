@@ -211,7 +211,7 @@ int64_t crispy_server_job_bytes(const char *bases, const int64_t *offsets, int32
   const int64_t pairs = std::min<int64_t>(all_pairs, 256 * (int64_t)n);
   const int64_t reads = 2 * (nbases + nbases / 4 + 48 * (int64_t)n);
   int64_t prof = (int64_t)n * (4 * words_per_read(K) + 48) + 6 * nbases;
-  // K > 6: K-mer lists, the hashed k = 6 bitmaps (2,048 bits) and their
+  // K > 6: K-mer lists, the hashed k = 6 bitmaps (at most 2,048 bits) and their
   // excess, candidates + their refined copy, the 256 x 4 KB segment counters
   if (K > 6) prof += 2 * nbases + 16 * (int64_t)n + 260 * (int64_t)n + 16 * (std::max(guess, pairs) + 256) + (1 << 20);
   int64_t est = reads + prof + 12 * guess + 72 * pairs + (int64_t)(8 << 20);

@@ -120,22 +120,29 @@ int64_t crispy_kmer_num_tiles(int32_t num_reads);
 /* Where K <= 6 pairs that the tile's bound cannot reject get their exact
  * count: 0 = in the tile kernel itself; 1 = the tile writes them out as
  * candidates and a second kernel counts them (for K = 6 the candidate tile
- * runs on the 4,096 codes hashed into 2,048 buckets, refined by the exact
- * bitmaps); 2 = as 1 without the hashing; -1 (the default) = form 1 from
+ * runs on the 4,096 codes hashed into 1,536 or 2,048 buckets, refined by the
+ * exact bitmaps); 2 = as 1 without the hashing; -1 (the default) = form 1 from
  * 10^9 read pairs up, form 0 below.  K = 7, 8 always take candidates: hashed
  * unless the mode is 2.  Results do not depend on it.  No counterpart in the
  * reference: a tuning and test switch. */
 int crispy_kmer_set_exact_pass(crispy_ctx *ctx, int mode);
+/* Buckets of the hashed k = 6 candidate filter: 1536, 2048, or 0 for the
+ * default (1536).  Fewer buckets: less tile work, more candidates for the
+ * exact passes.  Results do not depend on it.  No counterpart in the
+ * reference: a tuning and test switch. */
+int crispy_kmer_set_hash_buckets(crispy_ctx *ctx, int buckets);
 /* Times crispy_kmer_run over the whole tile range under each form of
- * crispy_kmer_set_exact_pass that applies to K (0, 1, 2 for K = 6), `repeats`
- * runs each after one warm-up run, keeps the fastest for the context and
- * leaves its result (the same under every form); *chosen = the mode.  No
- * counterpart in the reference (its tile shapes are fixed, kmerMain.cpp:50-53). */
+ * crispy_kmer_set_exact_pass that applies to K (0, 1, 2 for K = 6; form 1
+ * with each bucket count of crispy_kmer_set_hash_buckets for K >= 6),
+ * `repeats` runs each after one warm-up run, keeps the fastest for the
+ * context and leaves its result (the same under every form); *chosen = the
+ * mode (crispy_kmer_tile_bits tells the bucket count).  No counterpart in the
+ * reference (its tile shapes are fixed, kmerMain.cpp:50-53). */
 int crispy_kmer_autotune(crispy_ctx *ctx, int K, float threshold, int repeats, int *chosen);
 /* Presence bits per read that the last crispy_kmer_run's tile pass multiplied
- * (FP4 k-slots per pair): 4^min(K, 6) padded to at least 512, or 2,048 when
- * the k = 6 candidate filter ran on bucket-hashed bitmaps (K > 6, or K = 6 with
- * the exact counts in a pass of their own); 0 before any run.  No counterpart
+ * (FP4 k-slots per pair): 4^min(K, 6) padded to at least 512, or the bucket
+ * count when the k = 6 candidate filter ran on bucket-hashed bitmaps (K > 6,
+ * or K = 6 with the exact counts in a pass of their own); 0 before any run.  No counterpart
  * in the reference (measurement). */
 int crispy_kmer_tile_bits(crispy_ctx *ctx);
 /* Copies result entries [first, first+count) to host: pairs as int32 (i, j),

@@ -192,27 +192,33 @@ def test_kmer_long_reads_vs_oracle(ctx, K):
 def test_kmer_exact_pass_forms(ctx, K):
     """Every exact-count form that applies to K gives the oracle's pairs and
     distances: in the tile kernel (K <= 6), in a pass of its own over the
-    tile's candidates (hashed for K >= 6) and over exact-bitmap candidates
-    (K >= 6), at a threshold where most candidates pass and at one where
-    most fail."""
+    tile's candidates (hashed for K >= 6, into 1,536 and into 2,048 buckets)
+    and over exact-bitmap candidates (K >= 6), at a threshold where most
+    candidates pass and at one where most fail."""
     say("kmer exact pass forms K", K)
     rng = np.random.default_rng(3000 + K)
     seqs = random_reads(rng, 300, max(K, 8), 300, repeats=True)
     r = ca.Reads.from_list(seqs)
     ro = O.Reads.from_list(seqs)
     ctx.upload_reads(ca.KMER, r)
-    modes = ([0] if K <= 6 else []) + [1] + ([2] if K >= 6 else [])
+    forms = [(0, 0)] if K <= 6 else []
+    forms += [(1, 1536), (1, 2048)] if K >= 6 else [(1, 0)]
+    forms += [(2, 0)] if K >= 6 else []
     try:
         for thr in (0.25, 0.6):
             op, od = O.kmer_all(ro, K, thr, nthreads=1)
-            for mode in modes:
+            for mode, buckets in forms:
                 ctx.kmer_exact_pass(mode)
+                ctx.kmer_hash_buckets(buckets)
                 ctx.kmer_run(K, thr)
+                if mode == 1 and K >= 6:
+                    assert ctx.kmer_tile_bits() == buckets
                 p, d = ctx.kmer_fetch()
-                assert np.array_equal(p, op), (thr, mode)
-                assert np.array_equal(bits(d), bits(od)), (thr, mode)
+                assert np.array_equal(p, op), (thr, mode, buckets)
+                assert np.array_equal(bits(d), bits(od)), (thr, mode, buckets)
     finally:
         ctx.kmer_exact_pass(-1)
+        ctx.kmer_hash_buckets(0)
 
 
 @pytest.mark.parametrize("K", [6, 8])

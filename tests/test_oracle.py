@@ -435,14 +435,36 @@ def test_kmer_rows_is_kmer_all_on_those_rows(K):
     assert np.array_equal(p[order], ap[sel]) and np.array_equal(d[order].view(np.uint32), ad[sel].view(np.uint32))
 
 
-def _bucket(codes):
-    """The k = 6 candidate filter's bucket of a code: the top 11 bits of
-    code * 2654435761 mod 2^32 (kmer_bucket in crispy_amd/csrc/kmer.hip)."""
-    return ((codes.astype(np.uint64) * 2654435761) & 0xFFFFFFFF) >> 21
+def _bucket(codes, buckets):
+    """The k = 6 candidate filter's bucket of a code: h = code * 2654435761
+    mod 2^32 scaled to the bucket range, (h * buckets) >> 32
+    (kmer_bucket in crispy_amd/csrc/kmer.hip)."""
+    h = (codes.astype(np.uint64) * 2654435761) & 0xFFFFFFFF
+    return (h * np.uint64(buckets)) >> 32
 
 
+def test_bucket_of_2048_is_the_top_11_bits():
+    """(h * 2048) >> 32 is h >> 21 (2,048 buckets: the hash's top 11 bits);
+    with 1,536 buckets every code lands in [0, 1536) and the bucket is
+    monotone in h."""
+    c = np.arange(4096, dtype=np.int64)
+    h = (c.astype(np.uint64) * 2654435761) & 0xFFFFFFFF
+    assert np.array_equal(_bucket(c, 2048), h >> 21)
+    b = _bucket(c, 1536)
+    assert b.min() >= 0 and b.max() <= 1535
+    # the scaled range is monotone in h: buckets split the hash range into
+    # 1,536 runs of 2^32 / 1536 values
+    hs = np.sort(h)
+    assert np.all(np.diff(_bucket_of_hash(hs, 1536).astype(np.int64)) >= 0)
+
+
+def _bucket_of_hash(h, buckets):
+    return (h * np.uint64(buckets)) >> 32
+
+
+@pytest.mark.parametrize("buckets", [1536, 2048])
 @pytest.mark.parametrize("K", [6, 7, 8])
-def test_hashed_filter_bound_is_sound(K):
+def test_hashed_filter_bound_is_sound(K, buckets):
     """The hashed k = 6 candidate filter may keep extra pairs but must never
     drop one: for every pair, popcount(bucket presence AND) + min(T - distinct
     buckets) >= the oracle's exact K-mer matches (kmerCPU.cpp:29-43), since
@@ -472,7 +494,7 @@ def test_hashed_filter_bound_is_sound(K):
     presence, hexcess = [], []
     for s in seqs:
         c = O.kmer_tuples(s, 6).astype(np.int64)
-        b = np.unique(_bucket(c))
+        b = np.unique(_bucket(c, buckets))
         presence.append(set(b.tolist()))
         hexcess.append(len(c) - len(b))
     pairs = np.array([[i, j] for i in range(n) for j in range(i + 1, n)], np.int32)
