@@ -36,11 +36,11 @@ constexpr int kGenericBucket = kKbSliceMax + 1;
 // k-mer tile geometry.
 constexpr int kTile = 128;        // reads per tile side
 constexpr int kChunkWords = 16;   // bitmap words staged in LDS per step
-// Buckets of the hashed k = 6 candidate filter: 1,536 or 2,048 (48 or 64
-// bitmap words per read, whole chunks of kChunkWords); fewer buckets mean less
+// Buckets of the hashed k = 6 candidate filter: 1,280, 1,536 or 2,048 (40,
+// 48 or 64 bitmap words per read, in 8-word groups); fewer buckets mean less
 // tile work and more candidates for the exact passes.
 constexpr int kHashWordsMax = 64;
-constexpr int kHashWordsMin = 48;
+__host__ __device__ constexpr bool valid_hash_words(int w) { return w == 40 || w == 48 || w == 64; }
 // The candidate tiles append to kCandSegs output segments, each with its own
 // counter kSegStride u64 words (4 KB) from the next: one counter for the
 // whole chip serialises the per-wave reservations (~600k atomics per

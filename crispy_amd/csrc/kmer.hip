@@ -48,10 +48,10 @@ namespace crispy {
 // range, (h * buckets) >> 32 (for 2,048 buckets its top 11 bits).  Merging
 // codes into buckets can only raise sum_b min(count_a(b), count_b(b)), so the
 // filter's bound stays an upper bound of the k-mer matches.
-constexpr int kDefaultHashWords = kHashWordsMin;  // 1,536 buckets: the tile work saved outweighs the extra candidates (reads of 400-900 bp)
+constexpr int kDefaultHashWords = 48;  // 1,536 buckets: the tile work saved outweighs the extra candidates (reads of 400-900 bp)
 template <int HW>
 __host__ __device__ inline uint32_t kmer_bucket(uint32_t code) {
-  static_assert(HW == kHashWordsMin || HW == kHashWordsMax, "1,536 or 2,048 buckets");
+  static_assert(valid_hash_words(HW), "1,280, 1,536 or 2,048 buckets");
   return (uint32_t)(((uint64_t)(code * 2654435761u) * (uint64_t)(32 * HW)) >> 32);
 }
 
@@ -306,8 +306,12 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
     int64_t tile_begin, int64_t ntiles, int64_t per_xcd, float thr, float cut, int tshift, int kshift,
     unsigned long long *__restrict__ counter, uint64_t *__restrict__ out_keys,
     float *__restrict__ out_dists, int64_t cap) {
-  static_assert(WP % kChunkWords == 0 && WP <= 128, "bitmap tiles hold at most 4,096 presence bits");
-  constexpr int NCH = WP / kChunkWords;
+  static_assert(WP % 8 == 0 && WP <= 128, "bitmap tiles hold at most 4,096 presence bits, in 8-word groups");
+  // chunks of kChunkWords words; with WP = 16c + 8 the last chunk holds one
+  // 8-word group (its other half, read past the row, is staged but not used:
+  // the bitmap buffers end with kChunkWords words of slack)
+  constexpr int NCH = (WP + kChunkWords - 1) / kChunkWords;
+  constexpr bool kHalfLast = WP % kChunkWords != 0;
   constexpr int kRow = kChunkWords + 4;  // padded row stride of a staged chunk (words)
   constexpr int kStageWords = 2 * kTile * kRow;
   __shared__ uint32_t Ps[kStageWords];  // bitmap chunk in the main loop, then repeat lists
@@ -442,6 +446,7 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
     }
 #pragma unroll
     for (int m = 0; m < 2; ++m) {  // two 8-word groups per chunk
+      if (kHalfLast && m == 1 && ch == NCH - 1) break;  // block-uniform
       uint4 xa[2], xb[4];
 #pragma unroll
       for (int fa = 0; fa < 2; ++fa)
@@ -1087,8 +1092,9 @@ static int launch_profile(crispy_ctx *ctx, ReadStore &rs) {
   KmerState &km = ctx->km;
   if constexpr (K == 6) {
     if (km.hashed) {
-      auto kernel = km.hash_words == kHashWordsMin ? kmer_profile_kernel<6, kHashWordsMin>
-                                                   : kmer_profile_kernel<6, kHashWordsMax>;
+      auto kernel = km.hash_words == 40   ? kmer_profile_kernel<6, 40>
+                    : km.hash_words == 48 ? kmer_profile_kernel<6, 48>
+                                          : kmer_profile_kernel<6, 64>;
       kernel<<<rs.n, 256, 0, ctx->stream>>>(
           rs.pack.as<uint32_t>(), rs.poff.as<int64_t>(), rs.len.as<int32_t>(), km.bitmap.as<uint32_t>(),
           km.T.as<int32_t>(), km.rcnt.as<int32_t>(), km.roff.as<int64_t>(), km.rlist.as<uint32_t>(),
@@ -1253,7 +1259,8 @@ static int build_profiles(crispy_ctx *ctx, int KP, int K, bool hash) {
   CRISPY_TRY(ensure(ctx, km.rcnt, sizeof(int32_t) * (n + 1)));
   CRISPY_TRY(ensure(ctx, km.excess, sizeof(int32_t) * (n + 1)));
   if (km.hashed) {
-    CRISPY_TRY(ensure(ctx, km.hbitmap, sizeof(uint32_t) * (size_t)n * km.hash_words));
+    // + kChunkWords: the tile's last half chunk stages a whole one
+    CRISPY_TRY(ensure(ctx, km.hbitmap, sizeof(uint32_t) * ((size_t)n * km.hash_words + kChunkWords)));
     CRISPY_TRY(ensure(ctx, km.hexcess, sizeof(int32_t) * (n + 1)));
   }
   CRISPY_TRY(ensure(ctx, km.roff, sizeof(int64_t) * (n + 1)));
@@ -1430,9 +1437,8 @@ int crispy_kmer_set_exact_pass(crispy_ctx *ctx, int mode) {
 
 int crispy_kmer_set_hash_buckets(crispy_ctx *ctx, int buckets) {
   if (!ctx) return CRISPY_ERR_ARG;
-  if (buckets != 0 && buckets != 32 * kHashWordsMin && buckets != 32 * kHashWordsMax)
-    return set_err(ctx, CRISPY_ERR_ARG, "hash buckets %d: 0 (default), %d or %d", buckets, 32 * kHashWordsMin,
-                   32 * kHashWordsMax);
+  if (buckets != 0 && (buckets % 32 != 0 || !valid_hash_words(buckets / 32)))
+    return set_err(ctx, CRISPY_ERR_ARG, "hash buckets %d: 0 (default), 1280, 1536 or 2048", buckets);
   ctx->km.hash_buckets = buckets;
   return CRISPY_OK;
 }
@@ -1446,8 +1452,7 @@ int crispy_kmer_autotune(crispy_ctx *ctx, int K, float threshold, int repeats, i
   std::vector<std::pair<int, int>> settings;  // (exact pass mode, hash buckets)
   if (K <= 6) settings.push_back({0, 0});
   if (K >= 6) {
-    settings.push_back({1, 32 * kHashWordsMax});
-    settings.push_back({1, 32 * kHashWordsMin});
+    for (int buckets : {2048, 1536, 1280}) settings.push_back({1, buckets});
     settings.push_back({2, 0});
   } else {
     settings.push_back({1, 0});
@@ -1527,8 +1532,9 @@ int crispy_kmer_run(crispy_ctx *ctx, int K, float threshold, int64_t tile_begin,
   CRISPY_TRY(build_profiles(ctx, KP, K, (K > 6 || separate) && km.exact_pass != 2));
   // the k = 6 candidate tiles: on the hashed bitmaps, or on the exact ones
   const TileFn tiles6 = !km.hashed                          ? launch_tiles<6, true>
-                        : km.hash_words == kHashWordsMin ? launch_tiles<6, true, kHashWordsMin>
-                                                         : launch_tiles<6, true, kHashWordsMax>;
+                        : km.hash_words == 40            ? launch_tiles<6, true, 40>
+                        : km.hash_words == 48            ? launch_tiles<6, true, 48>
+                                                         : launch_tiles<6, true, 64>;
   // first guess at the result size: the pairs in the range, at most 4 Mi
   const int64_t pairs_in_range = std::min<int64_t>(nt * kTile * kTile, (int64_t)n * (n - 1) / 2);
   const int64_t guess = std::min<int64_t>((int64_t)1 << 22, std::max<int64_t>(4096, pairs_in_range));

@@ -120,13 +120,13 @@ int64_t crispy_kmer_num_tiles(int32_t num_reads);
 /* Where K <= 6 pairs that the tile's bound cannot reject get their exact
  * count: 0 = in the tile kernel itself; 1 = the tile writes them out as
  * candidates and a second kernel counts them (for K = 6 the candidate tile
- * runs on the 4,096 codes hashed into 1,536 or 2,048 buckets, refined by the
+ * runs on the 4,096 codes hashed into 1,280, 1,536 or 2,048 buckets, refined by the
  * exact bitmaps); 2 = as 1 without the hashing; -1 (the default) = form 1 from
  * 10^9 read pairs up, form 0 below.  K = 7, 8 always take candidates: hashed
  * unless the mode is 2.  Results do not depend on it.  No counterpart in the
  * reference: a tuning and test switch. */
 int crispy_kmer_set_exact_pass(crispy_ctx *ctx, int mode);
-/* Buckets of the hashed k = 6 candidate filter: 1536, 2048, or 0 for the
+/* Buckets of the hashed k = 6 candidate filter: 1280, 1536, 2048, or 0 for the
  * default (1536).  Fewer buckets: less tile work, more candidates for the
  * exact passes.  Results do not depend on it.  No counterpart in the
  * reference: a tuning and test switch. */

@@ -192,7 +192,7 @@ def test_kmer_long_reads_vs_oracle(ctx, K):
 def test_kmer_exact_pass_forms(ctx, K):
     """Every exact-count form that applies to K gives the oracle's pairs and
     distances: in the tile kernel (K <= 6), in a pass of its own over the
-    tile's candidates (hashed for K >= 6, into 1,536 and into 2,048 buckets)
+    tile's candidates (hashed for K >= 6, into 1,280, 1,536 and 2,048 buckets)
     and over exact-bitmap candidates (K >= 6), at a threshold where most
     candidates pass and at one where most fail."""
     say("kmer exact pass forms K", K)
@@ -202,7 +202,7 @@ def test_kmer_exact_pass_forms(ctx, K):
     ro = O.Reads.from_list(seqs)
     ctx.upload_reads(ca.KMER, r)
     forms = [(0, 0)] if K <= 6 else []
-    forms += [(1, 1536), (1, 2048)] if K >= 6 else [(1, 0)]
+    forms += [(1, 1280), (1, 1536), (1, 2048)] if K >= 6 else [(1, 0)]
     forms += [(2, 0)] if K >= 6 else []
     try:
         for thr in (0.25, 0.6):

@@ -445,24 +445,25 @@ def _bucket(codes, buckets):
 
 def test_bucket_of_2048_is_the_top_11_bits():
     """(h * 2048) >> 32 is h >> 21 (2,048 buckets: the hash's top 11 bits);
-    with 1,536 buckets every code lands in [0, 1536) and the bucket is
-    monotone in h."""
+    with 1,280 or 1,536 buckets every code lands in [0, buckets) and the
+    bucket is monotone in h."""
     c = np.arange(4096, dtype=np.int64)
     h = (c.astype(np.uint64) * 2654435761) & 0xFFFFFFFF
     assert np.array_equal(_bucket(c, 2048), h >> 21)
-    b = _bucket(c, 1536)
-    assert b.min() >= 0 and b.max() <= 1535
-    # the scaled range is monotone in h: buckets split the hash range into
-    # 1,536 runs of 2^32 / 1536 values
     hs = np.sort(h)
-    assert np.all(np.diff(_bucket_of_hash(hs, 1536).astype(np.int64)) >= 0)
+    for buckets in (1280, 1536):
+        b = _bucket(c, buckets)
+        assert b.min() >= 0 and b.max() <= buckets - 1
+        # the scaled range is monotone in h: the buckets split the hash range
+        # into runs of 2^32 / buckets values
+        assert np.all(np.diff(_bucket_of_hash(hs, buckets).astype(np.int64)) >= 0)
 
 
 def _bucket_of_hash(h, buckets):
     return (h * np.uint64(buckets)) >> 32
 
 
-@pytest.mark.parametrize("buckets", [1536, 2048])
+@pytest.mark.parametrize("buckets", [1280, 1536, 2048])
 @pytest.mark.parametrize("K", [6, 7, 8])
 def test_hashed_filter_bound_is_sound(K, buckets):
     """The hashed k = 6 candidate filter may keep extra pairs but must never
