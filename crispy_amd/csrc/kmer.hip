@@ -543,47 +543,56 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
         lds_f4 gr[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) gr[q] = *reinterpret_cast<const lds_f4 *>(&sG[rb + 8 * q]);
-        // the smallest row bound of the lane's 16 rows (non-negative floats
+        // the smallest row bound of each of the lane's 4 row groups
+        // (rows rb + 8q .. rb + 8q + 3) and of all 16 (non-negative floats
         // order as their bits)
-        uint32_t grmin = __float_as_uint(gr[0][0]);
-        static_for<15>([&](auto R) {
-          constexpr int r = decltype(R)::value + 1;
-          grmin = min(grmin, __float_as_uint(gr[r >> 2][r & 3]));
-        });
+        uint32_t gq[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          gq[q] = min(min(__float_as_uint(gr[q][0]), __float_as_uint(gr[q][1])),
+                      min(__float_as_uint(gr[q][2]), __float_as_uint(gr[q][3])));
+        const uint32_t grmin = min(min(gq[0], gq[1]), min(gq[2], gq[3]));
         static_for<4>([&](auto FB) {
           constexpr int fb = decltype(FB)::value;
           if (qcount > kQueue - 1024) flush();
           const int cj = fb * 32 + r32;  // the lane's column in the tile
           const float gc = sG[kTile + cj];
-          // whole-fragment test first: a value can only be kept if the
-          // fragment's largest popcount in the lane reaches the lane's
-          // smallest bound min(G_i, G_j); most fragments of a large run
-          // keep nothing, and then cost 10 VALU instead of 32
-          uint32_t amax = __float_as_uint(acc[fa][fb][0]);
-          static_for<15>([&](auto R) {
-            constexpr int r = decltype(R)::value + 1;
-            amax = max(amax, __float_as_uint(acc[fa][fb][r]));
-          });
-          if (__ballot(amax >= min(grmin, __float_as_uint(gc))) == 0) return;  // wave-uniform
+          const uint32_t gcu = __float_as_uint(gc);
+          // a value can only be kept if its row group's largest popcount
+          // reaches the group's smallest bound min(G_i, G_j): the whole
+          // fragment is tested first (most fragments of a large run keep
+          // nothing), then each row group (a fragment that keeps a pair
+          // mostly keeps one), then the values of the groups that pass
+          uint32_t amq[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            amq[q] = max(max(__float_as_uint(acc[fa][fb][4 * q]), __float_as_uint(acc[fa][fb][4 * q + 1])),
+                         max(__float_as_uint(acc[fa][fb][4 * q + 2]), __float_as_uint(acc[fa][fb][4 * q + 3])));
+          const uint32_t amax = max(max(amq[0], amq[1]), max(amq[2], amq[3]));
+          if (__ballot(amax >= min(grmin, gcu)) == 0) return;  // wave-uniform
           const uint32_t ebase = (uint32_t)rb | ((uint32_t)cj << 7);
-          static_for<16>([&](auto R) {
-            constexpr int r = decltype(R)::value;
-            constexpr int dr = 8 * (r >> 2) + (r & 3);
-            // popcount >= min(G_i, G_j): the pair may pass
-            bool keep = acc[fa][fb][r] >= umin_f(gr[r >> 2][r & 3], gc);
-            if constexpr (P) {
-              const int ri = rb + dr;
-              keep = keep && (!diag || ri < cj) && ri < rlim && cj < clim;
-            }
-            const uint64_t ball = __ballot(keep);
-            if (ball) {  // wave-uniform
-              if (keep) {  // slot: the kept lanes below this one
-                const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(ball >> 32),
-                                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)ball, 0u));
-                queue[qcount + (int)below] = (ebase + (uint32_t)dr) | ((uint32_t)acc[fa][fb][r] << 14);
+          static_for<4>([&](auto Q) {
+            constexpr int q = decltype(Q)::value;
+            if (__ballot(amq[q] >= min(gq[q], gcu)) == 0) return;  // wave-uniform
+            static_for<4>([&](auto T) {
+              constexpr int r = 4 * q + decltype(T)::value;
+              constexpr int dr = 8 * q + (r & 3);
+              // popcount >= min(G_i, G_j): the pair may pass
+              bool keep = acc[fa][fb][r] >= umin_f(gr[q][r & 3], gc);
+              if constexpr (P) {
+                const int ri = rb + dr;
+                keep = keep && (!diag || ri < cj) && ri < rlim && cj < clim;
               }
-              qcount += __popcll(ball);
-            }
+              const uint64_t ball = __ballot(keep);
+              if (ball) {  // wave-uniform
+                if (keep) {  // slot: the kept lanes below this one
+                  const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(ball >> 32),
+                                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)ball, 0u));
+                  queue[qcount + (int)below] = (ebase + (uint32_t)dr) | ((uint32_t)acc[fa][fb][r] << 14);
+                }
+                qcount += __popcll(ball);
+              }
+            });
           });
         });
       });
