@@ -35,7 +35,8 @@ constexpr int kGenericBucket = kKbSliceMax + 1;
 
 // k-mer tile geometry.
 constexpr int kTile = 128;        // reads per tile side
-constexpr int kChunkWords = 16;   // bitmap words staged in LDS per step
+constexpr int kChunkWords = 16;   // bitmap words staged in LDS per step (24 for 40- and 48-word bitmaps)
+constexpr int kBitmapSlack = 32;  // words after the last bitmap row: a last chunk may be staged whole
 // Buckets of the hashed k = 6 candidate filter: 1,280, 1,536 or 2,048 (40,
 // 48 or 64 bitmap words per read, in 8-word groups); fewer buckets mean less
 // tile work and more candidates for the exact passes.

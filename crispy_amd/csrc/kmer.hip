@@ -298,7 +298,7 @@ __device__ __forceinline__ int wave_incl_sum(int v, int lane) {
   return v;
 }
 
-template <int WP, bool CAND>
+template <int WP, bool CAND, int CW = kChunkWords>
 __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
     const uint32_t *__restrict__ bitmap, const int32_t *__restrict__ Tn,
     const int32_t *__restrict__ rcnt, const int64_t *__restrict__ roff,
@@ -307,12 +307,14 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
     unsigned long long *__restrict__ counter, uint64_t *__restrict__ out_keys,
     float *__restrict__ out_dists, int64_t cap) {
   static_assert(WP % 8 == 0 && WP <= 128, "bitmap tiles hold at most 4,096 presence bits, in 8-word groups");
-  // chunks of kChunkWords words; with WP = 16c + 8 the last chunk holds one
-  // 8-word group (its other half, read past the row, is staged but not used:
-  // the bitmap buffers end with kChunkWords words of slack)
-  constexpr int NCH = (WP + kChunkWords - 1) / kChunkWords;
-  constexpr bool kHalfLast = WP % kChunkWords != 0;
-  constexpr int kRow = kChunkWords + 4;  // padded row stride of a staged chunk (words)
+  // chunks of CW words (16, or 24 for 40- and 48-word bitmaps: 2 chunks, 2
+  // barrier pairs per tile); the last chunk may hold fewer 8-word groups (the
+  // rest of it, read past the row, is staged but not used: the bitmap buffers
+  // end with kBitmapSlack words of slack)
+  static_assert(CW == 16 || CW == 24, "16- or 24-word chunks");
+  constexpr int NCH = (WP + CW - 1) / CW;
+  constexpr int kLastGroups = (WP - (NCH - 1) * CW) / 8;  // 8-word groups in the last chunk
+  constexpr int kRow = CW + 4;  // padded row stride of a staged chunk (words)
   constexpr int kStageWords = 2 * kTile * kRow;
   __shared__ uint32_t Ps[kStageWords];  // bitmap chunk in the main loop, then repeat lists
   __shared__ __align__(16) float sG[2 * kTile];
@@ -338,31 +340,38 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
   const int lane = tid & 63, wv = tid >> 6;
   const int g = lane >> 5, r32 = lane & 31;
 
-  // loader: a chunk is 16 words (64 bytes) of each of the 256 reads; lane quad
-  // (t >> 2) of the block loads reads (t >> 2) + 32k, k = 0..3, of the rows and
-  // of the columns, lane t & 3 one 16-byte quarter, so a wave instruction reads
-  // 16 whole 64-byte runs.  Rows past n load read 0 (always valid): their
-  // pairs are masked in the epilogue.  Chunk 0 is requested first and the
-  // per-read data right after it, so their latencies overlap.
-  const int lq = tid & 3, lr = tid >> 2;
-  uint32_t oa[4], ob[4];  // word offsets of this lane's quarter, chunk 0
+  // loader: a chunk is CW words of each of the 256 reads.  TPR lanes share a
+  // read, each loading UPT 16-byte pieces of it (16-word chunks: lane quads,
+  // one piece each; 24-word chunks: lane pairs, three each), so a wave
+  // instruction reads whole runs; the block's 128 lanes cover RP reads of the
+  // rows and of the columns per pass, NPASS passes.  Rows past n load read 0
+  // (always valid): their pairs are masked in the epilogue.  Chunk 0 is
+  // requested first and the per-read data right after it, so their
+  // latencies overlap.
+  constexpr int TPR = CW == 16 ? 4 : 2, UPT = CW / 4 / TPR;
+  constexpr int RP = 128 / TPR, NPASS = kTile / RP;
+  constexpr int NLD = NPASS * UPT;  // 16-byte pieces per lane per side
+  const int lq = tid % TPR, lr = tid / TPR;
+  uint32_t oa[NPASS], ob[NPASS];  // word offsets of this lane's first piece, chunk 0
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int ra = row0 + lr + 32 * k, rb = col0 + lr + 32 * k;
-    oa[k] = (uint32_t)(ra < n ? ra : 0) * WP + 4 * lq;
-    ob[k] = (uint32_t)(rb < n ? rb : 0) * WP + 4 * lq;
+  for (int k = 0; k < NPASS; ++k) {
+    const int ra = row0 + lr + RP * k, rb = col0 + lr + RP * k;
+    oa[k] = (uint32_t)(ra < n ? ra : 0) * WP + 4 * UPT * lq;
+    ob[k] = (uint32_t)(rb < n ? rb : 0) * WP + 4 * UPT * lq;
   }
   // (built field by field: a whole-struct copy into the arrays keeps them in scratch)
   auto ld = [bitmap](uint32_t o) {
     const uint4 v = *reinterpret_cast<const uint4 *>(bitmap + o);
     return make_uint4(v.x, v.y, v.z, v.w);
   };
-  uint4 pa[4], pb[4];
+  uint4 pa[NLD], pb[NLD];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    pa[k] = ld(oa[k]);
-    pb[k] = ld(ob[k]);
-  }
+  for (int k = 0; k < NPASS; ++k)
+#pragma unroll
+    for (int u = 0; u < UPT; ++u) {
+      pa[k * UPT + u] = ld(oa[k] + 4 * u);
+      pb[k * UPT + u] = ld(ob[k] + 4 * u);
+    }
 
   // per read of the tile (thread t: reads t and t + 128 of the 256)
   static_assert(2 * kTile == 2 * 128, "128 threads cover the tile's 256 reads in two steps");
@@ -432,21 +441,25 @@ __global__ __launch_bounds__(128, 2) void kmer_tile_mfma_kernel(
   for (int ch = 0; ch < NCH; ++ch) {
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      *reinterpret_cast<uint4 *>(&Ps[(lr + 32 * k) * kRow + 4 * lq]) = pa[k];
-      *reinterpret_cast<uint4 *>(&Ps[(kTile + lr + 32 * k) * kRow + 4 * lq]) = pb[k];
-    }
+    for (int k = 0; k < NPASS; ++k)
+#pragma unroll
+      for (int u = 0; u < UPT; ++u) {
+        *reinterpret_cast<uint4 *>(&Ps[(lr + RP * k) * kRow + 4 * (UPT * lq + u)]) = pa[k * UPT + u];
+        *reinterpret_cast<uint4 *>(&Ps[(kTile + lr + RP * k) * kRow + 4 * (UPT * lq + u)]) = pb[k * UPT + u];
+      }
     __syncthreads();
     if (ch + 1 < NCH) {  // prefetch the next chunk while this one is consumed
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        pa[k] = ld(oa[k] + (ch + 1) * kChunkWords);
-        pb[k] = ld(ob[k] + (ch + 1) * kChunkWords);
-      }
+      for (int k = 0; k < NPASS; ++k)
+#pragma unroll
+        for (int u = 0; u < UPT; ++u) {
+          pa[k * UPT + u] = ld(oa[k] + (ch + 1) * CW + 4 * u);
+          pb[k * UPT + u] = ld(ob[k] + (ch + 1) * CW + 4 * u);
+        }
     }
 #pragma unroll
-    for (int m = 0; m < 2; ++m) {  // two 8-word groups per chunk
-      if (kHalfLast && m == 1 && ch == NCH - 1) break;  // block-uniform
+    for (int m = 0; m < CW / 8; ++m) {  // 8-word groups
+      if (ch == NCH - 1 && m >= kLastGroups) break;  // block-uniform
       uint4 xa[2], xb[4];
 #pragma unroll
       for (int fa = 0; fa < 2; ++fa)
@@ -1184,6 +1197,7 @@ static int launch_tiles(crispy_ctx *ctx, int32_t n, int nb, int64_t tb, int64_t 
   constexpr bool HASHED = HW > 0;
   static_assert(!HASHED || (K == 6 && CAND), "the hashed filter is the k = 6 candidate form");
   constexpr int WP = HASHED ? HW : kmer_words(K);
+  constexpr int CW = CAND && (WP == 40 || WP == 48) ? 24 : kChunkWords;  // chunk words (2 chunks per tile)
   KmerState &km = ctx->km;
   const DevBuf &bits = HASHED ? km.hbitmap : km.bitmap;
   const DevBuf &excess = HASHED ? km.hexcess : km.excess;
@@ -1207,7 +1221,7 @@ static int launch_tiles(crispy_ctx *ctx, int32_t n, int nb, int64_t tb, int64_t 
         else
           zero_counter_kernel<<<1, 1, 0, ctx->stream>>>(counter);
         CRISPY_LAUNCHED(ctx);
-        kmer_tile_mfma_kernel<WP, CAND><<<(unsigned)grid, 128, 0, ctx->stream>>>(
+        kmer_tile_mfma_kernel<WP, CAND, CW><<<(unsigned)grid, 128, 0, ctx->stream>>>(
             bits.as<uint32_t>(), km.T.as<int32_t>(), km.rcnt.as<int32_t>(), km.roffc.as<int64_t>(),
             km.rlistc.as<uint32_t>(), excess.as<int32_t>(), n, nb, tb, nt, per_xcd, thr, cut, tshift,
             km.kshift, counter, out_keys, out_dists, cap);
@@ -1268,8 +1282,8 @@ static int build_profiles(crispy_ctx *ctx, int KP, int K, bool hash) {
   CRISPY_TRY(ensure(ctx, km.rcnt, sizeof(int32_t) * (n + 1)));
   CRISPY_TRY(ensure(ctx, km.excess, sizeof(int32_t) * (n + 1)));
   if (km.hashed) {
-    // + kChunkWords: the tile's last half chunk stages a whole one
-    CRISPY_TRY(ensure(ctx, km.hbitmap, sizeof(uint32_t) * ((size_t)n * km.hash_words + kChunkWords)));
+    // + kBitmapSlack: the tile's last part-filled chunk is staged whole
+    CRISPY_TRY(ensure(ctx, km.hbitmap, sizeof(uint32_t) * ((size_t)n * km.hash_words + kBitmapSlack)));
     CRISPY_TRY(ensure(ctx, km.hexcess, sizeof(int32_t) * (n + 1)));
   }
   CRISPY_TRY(ensure(ctx, km.roff, sizeof(int64_t) * (n + 1)));
