@@ -55,56 +55,68 @@ __host__ __device__ inline uint32_t kmer_bucket(uint32_t code) {
   return (uint32_t)(((uint64_t)(code * 2654435761u) * (uint64_t)(32 * HW)) >> 32);
 }
 
-// One block per read: the codes of its T = L-K+1 k-mers (first base most
-// significant, as READ::formTuples kmerMain.cpp:117-154) counted in an LDS
-// histogram over all 4^K codes; the bins then give the presence bitmap, the
-// repeat entries in code order and excess = T - (distinct codes).  HW > 0
-// (K = 6) also writes the presence of the codes' buckets (kmer_bucket<HW>,
-// HW words per read) and hexcess = T - (distinct buckets).
+// One block of 128 threads per read: the read's packed words are staged in
+// LDS, the codes of its T = L-K+1 k-mers (first base most significant, as
+// READ::formTuples kmerMain.cpp:117-154) counted in an LDS histogram of 16-bit
+// bins (two per word: a count is at most T <= 2,048) over all 4^K codes; the
+// bins then give the presence bitmap, the repeat entries in code order and
+// excess = T - (distinct codes).  HW > 0 (K = 6) also writes the presence of
+// the codes' buckets (kmer_bucket<HW>, HW words per read) and
+// hexcess = T - (distinct buckets).  (Small blocks and half-size bins: 16
+// reads in flight per CU, against 8 with 256 threads and 32-bit bins.)
 template <int K, int HW = 0>
-__global__ __launch_bounds__(256) void kmer_profile_kernel(
+__global__ __launch_bounds__(128) void kmer_profile_kernel(
     const uint32_t *__restrict__ pack, const int64_t *__restrict__ poff,
     const int32_t *__restrict__ len, uint32_t *__restrict__ bitmap, int32_t *__restrict__ Tout,
     int32_t *__restrict__ rcnt, const int64_t *__restrict__ roff, uint32_t *__restrict__ rlist,
     int32_t *__restrict__ excess, uint32_t *__restrict__ hbitmap, int32_t *__restrict__ hexcess) {
+  constexpr int NT = 128;
   constexpr int WP = kmer_words(K);
   constexpr int NBINS = 1 << (2 * K);
-  constexpr int BPT = NBINS >= 256 ? NBINS / 256 : 1;  // bins per thread (within one bitmap word)
+  constexpr int NHW = NBINS >= 2 ? NBINS / 2 : 1;       // histogram words (two 16-bit bins each)
+  constexpr int BPT = NBINS >= NT ? NBINS / NT : 1;     // bins per thread (within one bitmap word)
+  constexpr int kMaxWords = (CRISPY_MAX_READ_LEN + 15) / 16;
   constexpr bool HASH = HW > 0;
   static_assert(!HASH || K == 6, "the hashed filter covers k = 6");
-  __shared__ uint32_t hist[NBINS];
+  static_assert(BPT <= 32 && (BPT == 1 || BPT % 2 == 0), "a thread's bins lie in one bitmap word");
+  __shared__ uint32_t hist[NHW];
+  __shared__ uint32_t sread[kMaxWords + 1];
   __shared__ uint32_t bm[WP];
   __shared__ uint32_t hbm[HASH ? HW : 1];
-  __shared__ int wsum[4];
+  __shared__ int wsum[2];
   __shared__ int s_distinct, s_hdistinct;
   const int r = blockIdx.x;
   const int tid = threadIdx.x;
-  const int T = len[r] - K + 1;  // >= 1, checked by the host
+  const int L = len[r];
+  const int T = L - K + 1;  // >= 1, checked by the host
+  const int nwr = (L + 15) >> 4;
   const uint32_t *s = pack + poff[r];
-  for (int v = tid; v < NBINS; v += 256) hist[v] = 0;
-  for (int w = tid; w < WP; w += 256) bm[w] = 0;
+  for (int w = tid; w < nwr; w += NT) sread[w] = s[w];
+  if (tid == 0) sread[nwr] = 0;  // the last window's high word
+  for (int v = tid; v < NHW; v += NT) hist[v] = 0;
+  for (int w = tid; w < WP; w += NT) bm[w] = 0;
   if (HASH)
-    for (int w = tid; w < HW; w += 256) hbm[w] = 0;
+    for (int w = tid; w < HW; w += NT) hbm[w] = 0;
   if (tid == 0) s_distinct = s_hdistinct = 0;
   __syncthreads();
-  for (int t = tid; t < T; t += 256) {
+  for (int t = tid; t < T; t += NT) {
+    // bases t .. t+K-1 (2 bits each, base p at bits 2(p mod 16) of word p / 16)
+    const uint32_t win = __builtin_amdgcn_alignbit(sread[(t >> 4) + 1], sread[t >> 4], 2 * (t & 15));
     uint32_t c = 0;
 #pragma unroll
-    for (int q = 0; q < K; ++q) {
-      const int p = t + q;
-      c = (c << 2) | ((s[p >> 4] >> (2 * (p & 15))) & 3u);
-    }
-    atomicAdd(&hist[c], 1u);
+    for (int q = 0; q < K; ++q) c = (c << 2) | ((win >> (2 * q)) & 3u);
+    atomicAdd(&hist[c >> 1], 1u << (16 * (c & 1)));
   }
   __syncthreads();
   // thread tid owns bins [tid * BPT, tid * BPT + BPT)
   const int b0 = tid * BPT;
+  auto bin = [&](int v) -> uint32_t { return (hist[v >> 1] >> (16 * (v & 1))) & 0xFFFFu; };
   uint32_t bits = 0;
   int nrep = 0, ndist = 0;
   if (b0 < NBINS) {
 #pragma unroll
     for (int k = 0; k < BPT; ++k) {
-      const uint32_t c = hist[b0 + k];
+      const uint32_t c = bin(b0 + k);
       bits |= (c > 0 ? 1u : 0u) << k;
       ndist += c > 0;
       nrep += c >= 2;
@@ -117,7 +129,7 @@ __global__ __launch_bounds__(256) void kmer_profile_kernel(
   }
   if (ndist) atomicAdd(&s_distinct, ndist);
   // where this thread's repeat entries go: an inclusive scan of nrep within
-  // the wave, then the totals of the waves before it
+  // the wave, then the total of the wave before it
   const int lane = tid & 63, wv = tid >> 6;
   int incl = nrep;
 #pragma unroll
@@ -127,24 +139,22 @@ __global__ __launch_bounds__(256) void kmer_profile_kernel(
   }
   if (lane == 63) wsum[wv] = incl;
   __syncthreads();
-  int before = 0;
-#pragma unroll
-  for (int w = 0; w < 3; ++w) before += w < wv ? wsum[w] : 0;
+  const int before = wv > 0 ? wsum[0] : 0;
   if (nrep) {
     uint32_t *out = rlist + roff[r] + (before + incl - nrep);
 #pragma unroll
     for (int k = 0; k < BPT; ++k) {
-      const uint32_t c = hist[b0 + k];
+      const uint32_t c = bin(b0 + k);
       if (c >= 2) *out++ = ((uint32_t)(b0 + k) << 16) | (c - 1);
     }
   }
-  if (tid == 255) rcnt[r] = before + incl;  // the last thread's inclusive sum: every entry
+  if (tid == NT - 1) rcnt[r] = before + incl;  // the last thread's inclusive sum: every entry
   if (tid == 0) {
     Tout[r] = T;
     excess[r] = T - s_distinct;  // sum over codes of (count - 1)
   }
   uint32_t *dst = bitmap + (int64_t)r * WP;
-  for (int w = tid; w < WP; w += 256) dst[w] = bm[w];
+  for (int w = tid; w < WP; w += NT) dst[w] = bm[w];
   if (HASH) {  // the barrier after the scan ordered the bucket bits before these reads
     if (tid < HW) {
       const uint32_t w = hbm[tid];
@@ -1117,7 +1127,7 @@ static int launch_profile(crispy_ctx *ctx, ReadStore &rs) {
       auto kernel = km.hash_words == 40   ? kmer_profile_kernel<6, 40>
                     : km.hash_words == 48 ? kmer_profile_kernel<6, 48>
                                           : kmer_profile_kernel<6, 64>;
-      kernel<<<rs.n, 256, 0, ctx->stream>>>(
+      kernel<<<rs.n, 128, 0, ctx->stream>>>(
           rs.pack.as<uint32_t>(), rs.poff.as<int64_t>(), rs.len.as<int32_t>(), km.bitmap.as<uint32_t>(),
           km.T.as<int32_t>(), km.rcnt.as<int32_t>(), km.roff.as<int64_t>(), km.rlist.as<uint32_t>(),
           km.excess.as<int32_t>(), km.hbitmap.as<uint32_t>(), km.hexcess.as<int32_t>());
@@ -1125,7 +1135,7 @@ static int launch_profile(crispy_ctx *ctx, ReadStore &rs) {
       return CRISPY_OK;
     }
   }
-  kmer_profile_kernel<K><<<rs.n, 256, 0, ctx->stream>>>(
+  kmer_profile_kernel<K><<<rs.n, 128, 0, ctx->stream>>>(
       rs.pack.as<uint32_t>(), rs.poff.as<int64_t>(), rs.len.as<int32_t>(), km.bitmap.as<uint32_t>(),
       km.T.as<int32_t>(), km.rcnt.as<int32_t>(), km.roff.as<int64_t>(),
       km.rlist.as<uint32_t>(), km.excess.as<int32_t>(), nullptr, nullptr);
