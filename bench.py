@@ -458,7 +458,7 @@ def main():
 
     kt = kern["kmer_tile"]
     # the tile is an FP4 matrix product over the presence vectors (the k <= 6
-    # codes, or their 1,536 or 2,048 hash buckets for the k = 6 candidate
+    # codes, or their 1,280, 1,536 or 2,048 hash buckets for the k = 6 candidate
     # filter): 2 ops x kdim per pair
     kdim = r["tile_bits"]
     kmer_tile_tops = kt["units"] * 2 * kdim / (kt["total_ms"] * 1e-3) / 1e12 if kt["total_ms"] else 0
@@ -483,7 +483,7 @@ def main():
                     "peak": MFMA_FP4_PEAK_TOPS, "unit": "TFLOP/s (FP4 MFMA)",
                     "frac": kmer_tile_tops / MFMA_FP4_PEAK_TOPS,
                     "work": (f"2 x {kdim} FP4 ops (presence-vector dot product"
-                             f"{f', 4,096 6-mer codes hashed into {kdim:,} buckets' if kdim in (1536, 2048) else ''}) per read pair, "
+                             f"{f', 4,096 6-mer codes hashed into {kdim:,} buckets' if kdim in (1280, 1536, 2048) else ''}) per read pair, "
                              f"{int(kt['units'] / max(1, kt['launches']))} pairs per launch"),
                     "avg_launch_ms": kt["avg_ms"], "traffic": traffic, "traffic_source": src}
 

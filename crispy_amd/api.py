@@ -347,7 +347,7 @@ class Context:
 
     def kmer_hash_buckets(self, buckets):
         """Buckets of the hashed k = 6 candidate filter: 1280, 1536, 2048 or 0 (the
-        default, 1536).  The results are the same."""
+        default, 1280).  The results are the same."""
         self._check(self._L.crispy_kmer_set_hash_buckets(self._h, int(buckets)))
 
     def kmer_autotune(self, K, threshold, repeats=2):

@@ -48,7 +48,7 @@ namespace crispy {
 // range, (h * buckets) >> 32 (for 2,048 buckets its top 11 bits).  Merging
 // codes into buckets can only raise sum_b min(count_a(b), count_b(b)), so the
 // filter's bound stays an upper bound of the k-mer matches.
-constexpr int kDefaultHashWords = 48;  // 1,536 buckets: the tile work saved outweighs the extra candidates (reads of 400-900 bp)
+constexpr int kDefaultHashWords = 40;  // 1,280 buckets: the tile work saved outweighs the extra candidates (reads of 150-900 bp)
 template <int HW>
 __host__ __device__ inline uint32_t kmer_bucket(uint32_t code) {
   static_assert(valid_hash_words(HW), "1,280, 1,536 or 2,048 buckets");

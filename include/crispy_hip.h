@@ -127,7 +127,7 @@ int64_t crispy_kmer_num_tiles(int32_t num_reads);
  * reference: a tuning and test switch. */
 int crispy_kmer_set_exact_pass(crispy_ctx *ctx, int mode);
 /* Buckets of the hashed k = 6 candidate filter: 1280, 1536, 2048, or 0 for the
- * default (1536).  Fewer buckets: less tile work, more candidates for the
+ * default (1280).  Fewer buckets: less tile work, more candidates for the
  * exact passes.  Results do not depend on it.  No counterpart in the
  * reference: a tuning and test switch. */
 int crispy_kmer_set_hash_buckets(crispy_ctx *ctx, int buckets);
