@@ -1004,6 +1004,8 @@ __global__ __launch_bounds__(256) void kmer_cand_refine_kernel(
   constexpr int WP = kmer_words(6);
   static_assert(WP == 4 * 32, "32 lanes of 4 words cover a k = 6 bitmap");
   __shared__ uint64_t sq[4][kRefineQueue];
+  // half a wave per candidate, two candidates per half wave and step: the
+  // loads of 4 candidates in flight per wave, each a 512-byte run per row
   const int lane = threadIdx.x & 63, h = lane >> 5, l = lane & 31, wv = threadIdx.x >> 6;
   uint64_t *q = sq[wv];
   int qn = 0;  // wave-uniform
@@ -1021,31 +1023,55 @@ __global__ __launch_bounds__(256) void kmer_cand_refine_kernel(
   cand += (size_t)sg * seg_cap;
   const int64_t waves = (int64_t)(gridDim.x / kCandSegs) * (blockDim.x >> 6);
   const int64_t w0 = (int64_t)(blockIdx.x / kCandSegs) * (blockDim.x >> 6) + wv;
-  for (int64_t cb = 2 * w0; cb < ncand; cb += 2 * waves) {  // wave-uniform
-    const int64_t c = cb + h;
-    const bool valid = c < ncand;
-    uint64_t key = 0;
-    uint32_t i = 0, j = 0;
-    int p = 0;
-    if (valid) {
-      key = cand[c] & kCandKeyMask;
-      i = (uint32_t)(key >> kshift);
-      j = (uint32_t)(key & ((1ull << kshift) - 1));
-      const uint4 a = *reinterpret_cast<const uint4 *>(bitmap + (size_t)i * WP + 4 * l);
-      const uint4 b = *reinterpret_cast<const uint4 *>(bitmap + (size_t)j * WP + 4 * l);
-      p = __popc(a.x & b.x) + __popc(a.y & b.y) + __popc(a.z & b.z) + __popc(a.w & b.w);
+  // one memory round trip per step: the next keys are loaded a step ahead,
+  // and T and excess are requested with the bitmaps
+  uint64_t kn[2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) kn[u] = 4 * w0 + 2 * u + h < ncand ? cand[4 * w0 + 2 * u + h] : 0;
+  for (int64_t cb = 4 * w0; cb < ncand; cb += 4 * waves) {  // wave-uniform
+    uint64_t key[2];
+    bool valid[2];
+    int p[2], Ti[2], Tj[2], Ei[2], Ej[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int64_t c = cb + 2 * u + h;
+      valid[u] = c < ncand;
+      key[u] = valid[u] ? kn[u] & kCandKeyMask : 0;
+      kn[u] = c + 4 * waves < ncand ? cand[c + 4 * waves] : 0;
+      p[u] = Ti[u] = Tj[u] = Ei[u] = Ej[u] = 0;
+    }
+    uint4 a[2], b[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const uint32_t i = (uint32_t)(key[u] >> kshift), j = (uint32_t)(key[u] & ((1ull << kshift) - 1));
+      a[u] = b[u] = make_uint4(0u, 0u, 0u, 0u);
+      if (valid[u]) {
+        a[u] = *reinterpret_cast<const uint4 *>(bitmap + (size_t)i * WP + 4 * l);
+        b[u] = *reinterpret_cast<const uint4 *>(bitmap + (size_t)j * WP + 4 * l);
+        if (l == 0) {
+          Ti[u] = Tn[i];
+          Tj[u] = Tn[j];
+          Ei[u] = excess[i];
+          Ej[u] = excess[j];
+        }
+      }
     }
 #pragma unroll
-    for (int off = 16; off > 0; off >>= 1) p += __shfl_xor(p, off, 32);
-    bool keep = false;
-    if (valid && l == 0) {
-      const float F = fmaf((float)(min(Tn[i], Tn[j]) - tshift), cut, -1.0f);
-      keep = (float)(p + min(excess[i], excess[j])) >= F;
+    for (int u = 0; u < 2; ++u) {
+      p[u] = __popc(a[u].x & b[u].x) + __popc(a[u].y & b[u].y) + __popc(a[u].z & b[u].z) +
+             __popc(a[u].w & b[u].w);
+#pragma unroll
+      for (int off = 16; off > 0; off >>= 1) p[u] += __shfl_xor(p[u], off, 32);
+      bool keep = false;
+      if (valid[u] && l == 0) {
+        const float F = fmaf((float)(min(Ti[u], Tj[u]) - tshift), cut, -1.0f);
+        keep = (float)(p[u] + min(Ei[u], Ej[u])) >= F;
+      }
+      const uint64_t ball = __ballot(keep);
+      if (keep) q[qn + __popcll(ball & ((1ull << lane) - 1))] = key[u] | ((uint64_t)p[u] << kCandPopShift);
+      qn += __popcll(ball);
     }
-    const uint64_t ball = __ballot(keep);
-    if (keep) q[qn + __popcll(ball & ((1ull << lane) - 1))] = key | ((uint64_t)p << kCandPopShift);
-    qn += __popcll(ball);
-    if (qn > kRefineQueue - 2) flush();
+    if (qn > kRefineQueue - 4) flush();
   }
   if (qn > 0) flush();
 }
@@ -1411,7 +1437,7 @@ static int refine_candidates(crispy_ctx *ctx, int64_t ncand, float threshold, in
   timer_begin(ctx, "kmer_cand_pop", &ev);
   zero_counter_kernel<<<1, 1, 0, ctx->stream>>>(km.cand_n.as<unsigned long long>());
   CRISPY_LAUNCHED(ctx);
-  // 8 candidates per block step; at most 8 blocks per segment
+  // 16 candidates per block step; at most 8 blocks per segment
   const int64_t per_seg = (ncand + kCandSegs - 1) / kCandSegs;
   const int bps = (int)std::max<int64_t>(1, std::min<int64_t>(8, (per_seg + 63) / 64));
   kmer_cand_refine_kernel<<<kCandSegs * bps, 256, 0, ctx->stream>>>(
