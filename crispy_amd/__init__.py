@@ -14,7 +14,7 @@ if the library is missing or no GPU is present, calls raise.
 """
 from .api import (  # noqa: F401
     KMER, NW, MAX_NUM_READS, MAX_READ_LEN, CrispyError, Context, Reads, Records, Server, band_cells,
-    job_bytes, kmer_default_threshold, library_path, load_library, nw_default_threshold, passes,
+    job_bytes, kmer_default_threshold, library_path, load_library, nw_default_threshold, passes, pinned_empty,
     preprocess_length_filter, preprocess_read, preprocess_write, read_fasta, read_primer,
 )
 from .pipeline import compute_gen_dist, compute_kmer_dist, preprocess  # noqa: F401

@@ -71,6 +71,8 @@ def load_library():
         "crispy_set_graphs": ([_vp, ctypes.c_int], ctypes.c_int),
         "crispy_kmer_set_exact_pass": ([_vp, ctypes.c_int], ctypes.c_int),
         "crispy_kmer_set_hash_buckets": ([_vp, ctypes.c_int], ctypes.c_int),
+        "crispy_host_alloc": ([ctypes.c_int64, ctypes.POINTER(_vp)], ctypes.c_int),
+        "crispy_host_free": ([_vp], ctypes.c_int),
         "crispy_kmer_tile_bits": ([_vp], ctypes.c_int),
         "crispy_nw_autotune": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "crispy_kmer_autotune": ([_vp, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
@@ -235,6 +237,36 @@ def preprocess_write(path, rec: Records, order, freq, fastq=False):
         _p(rec.label_offsets, _i64p), _p(order, _i32p), _p(freq, _i32p), len(order))
     if rc != 0:
         raise CrispyError(rc, f"cannot write {path}")
+
+
+class _PinnedBlock:
+    """Owns one crispy_host_alloc block; frees it when the last array viewing
+    it is gone."""
+
+    def __init__(self, nbytes):
+        self._L = load_library()
+        p = _vp()
+        rc = self._L.crispy_host_alloc(nbytes, ctypes.byref(p))
+        if rc != 0:
+            raise CrispyError(rc, f"cannot allocate {nbytes} bytes of page-locked host memory")
+        self.ptr = p.value
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            self._L.crispy_host_free(self.ptr)
+            self.ptr = None
+
+
+def pinned_empty(shape, dtype):
+    """An uninitialised numpy array in page-locked host memory (copies from
+    the device into it run at the bus's rate); freed with its last view."""
+    dtype = np.dtype(dtype)
+    count = int(np.prod(shape)) if np.ndim(shape) else int(shape)
+    nbytes = max(1, count * dtype.itemsize)
+    block = _PinnedBlock(nbytes)
+    raw = (ctypes.c_byte * nbytes).from_address(block.ptr)
+    raw._owner = block  # the ctypes array, and through it every numpy view, keeps the block alive
+    return np.ctypeslib.as_array(raw)[:count * dtype.itemsize].view(dtype).reshape(shape)
 
 
 def kmer_default_threshold(n):

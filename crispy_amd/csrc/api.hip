@@ -282,6 +282,18 @@ int crispy_ctx_trim(crispy_ctx *ctx) {
 
 const char *crispy_ctx_error(const crispy_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
 
+int crispy_host_alloc(int64_t bytes, void **out) {
+  if (!out || bytes < 0) return CRISPY_ERR_ARG;
+  *out = nullptr;
+  if (bytes == 0) return CRISPY_OK;
+  return hipHostMalloc(out, (size_t)bytes, hipHostMallocDefault) == hipSuccess ? CRISPY_OK : CRISPY_ERR_NOMEM;
+}
+
+int crispy_host_free(void *ptr) {
+  if (ptr) (void)hipHostFree(ptr);
+  return CRISPY_OK;
+}
+
 int crispy_ctx_sync(crispy_ctx *ctx) {
   if (!ctx) return CRISPY_ERR_ARG;
   CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));

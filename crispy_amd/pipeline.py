@@ -80,9 +80,10 @@ def compute_gen_dist(ctx: Context, reads: Reads, in_pair_path: str, pair_prefix:
 
 
 class OutputBuffers:
-    """Grow-only host arrays that fused_output_files can fill instead of
-    allocating new ones per call (fresh host pages cost a page fault each on
-    the first copy); the files it returns are then views into them, valid
+    """Grow-only page-locked host arrays that fused_output_files can fill
+    instead of allocating new ones per call (fresh pageable pages cost a page
+    fault each on the first copy, and a copy into pageable memory goes through
+    a bounce buffer); the files it returns are then views into them, valid
     until the next call with the same buffers."""
 
     def __init__(self):
@@ -91,10 +92,9 @@ class OutputBuffers:
 
     def reserve(self, n):
         if len(self.dists) < n:
-            self.pairs = np.empty((n, 2), np.int32)
-            self.dists = np.empty(n, np.float32)
-            self.pairs.fill(0)  # touch the pages once
-            self.dists.fill(0)
+            from .api import pinned_empty
+            self.pairs = pinned_empty((n, 2), np.int32)
+            self.dists = pinned_empty(n, np.float32)
 
 
 def fused_output_files(ctx: Context, buffers: OutputBuffers = None):

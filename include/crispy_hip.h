@@ -69,6 +69,13 @@ int crispy_ctx_set_mem_limit(crispy_ctx *ctx, int64_t bytes);
  * streams stay.  Buffers otherwise only grow, so a long-lived context holds
  * the largest job it has run. */
 int crispy_ctx_trim(crispy_ctx *ctx);
+/* Page-locked host memory (hipHostMalloc) for the results' copies to the
+ * host: a fetch into it runs at the bus's rate, into pageable memory it is
+ * staged through a bounce buffer.  *out = NULL for bytes == 0.  Release with
+ * crispy_host_free.  Replaces the reference's page-locked pair and distance
+ * arrays (cudaMallocHost, genCUDA.cu:112-120). */
+int crispy_host_alloc(int64_t bytes, void **out);
+int crispy_host_free(void *ptr);
 
 /* ------------------------------------------------------------- host side */
 /* FASTA reader of the reference (kmerMain.cpp:298-397 for variant

@@ -627,6 +627,26 @@ def test_device_file_split_and_sorted_fetch(ctx):
             assert np.array_equal(p.astype(np.uint32), op) and np.array_equal(bits(d), bits(od))
 
 
+def test_pinned_host_arrays():
+    """crispy_host_alloc-backed numpy arrays: shapes and dtypes as asked,
+    writable, zero-size allowed, and a view keeps its block alive after the
+    array it came from is gone."""
+    say("pinned host arrays")
+    a = ca.pinned_empty((1000, 2), np.int32)
+    assert a.shape == (1000, 2) and a.dtype == np.int32 and a.flags.c_contiguous
+    a[:] = np.arange(2000, dtype=np.int32).reshape(1000, 2)
+    v = a[10:20]
+    del a
+    import gc
+    gc.collect()
+    assert np.array_equal(v, np.arange(20, 40, dtype=np.int32).reshape(10, 2))
+    f = ca.pinned_empty(5, np.float32)
+    f[:] = 1.5
+    assert f.shape == (5,) and float(f.sum()) == 7.5
+    z = ca.pinned_empty((0, 2), np.int32)
+    assert z.shape == (0, 2)
+
+
 def test_python_mirror_matches_cli(ctx, tmp_path):
     say("python mirror")
     fa = tmp_path / "c.fa"
