@@ -22,6 +22,31 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
+// Page-locked host block (crispy_host_alloc), move-only, freed with its
+// owner: a device-to-host copy into it runs at the bus's rate and touches no
+// fresh pages.
+struct PinnedBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+  PinnedBuf() = default;
+  PinnedBuf(const PinnedBuf &) = delete;
+  PinnedBuf &operator=(const PinnedBuf &) = delete;
+  PinnedBuf(PinnedBuf &&o) noexcept : p(o.p), cap(o.cap) { o.p = nullptr, o.cap = 0; }
+  PinnedBuf &operator=(PinnedBuf &&o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p, cap = o.cap;
+      o.p = nullptr, o.cap = 0;
+    }
+    return *this;
+  }
+  ~PinnedBuf() { release(); }
+  void release() {
+    if (p) crispy_host_free(p);
+    p = nullptr, cap = 0;
+  }
+};
+
 struct Job {
   int64_t id = 0;
   std::string kbases, nbases;
@@ -40,8 +65,12 @@ struct Job {
   int status = CRISPY_OK;
   std::string error;
   int64_t kmer_pairs = 0;
-  std::vector<int32_t> pairs;
-  std::vector<float> dists;
+  // the result: `count` pairs (int32 i, j) then `count` distances, in
+  // page-locked memory from the server's pool
+  PinnedBuf res;
+  int64_t count = 0;
+  const int32_t *pairs() const { return static_cast<const int32_t *>(res.p); }
+  const float *dists() const { return reinterpret_cast<const float *>(static_cast<const char *>(res.p) + 8 * count); }
   std::vector<int64_t> row_ptr;  // CSR rows of the result (built on the device)
   Clock::time_point t_submit, t_start, t_end;
 };
@@ -71,11 +100,46 @@ struct crispy_server {
   std::map<int64_t, std::string> errors;
   std::vector<crispy_ctx *> ctxs;
   std::vector<std::thread> workers;
+  // Result blocks of fetched jobs, reused by the next jobs (pool_mu).
+  std::mutex pool_mu;
+  std::vector<PinnedBuf> pool;
+
+  // A block of at least `bytes`: the smallest free one that fits, else the
+  // largest grown to fit, else a new one.
+  int get_block(size_t bytes, PinnedBuf &out) {
+    if (bytes == 0) return CRISPY_OK;
+    {
+      std::lock_guard<std::mutex> lk(pool_mu);
+      int pick = -1, largest = -1;
+      for (int k = 0; k < (int)pool.size(); ++k) {
+        if (pool[k].cap >= bytes && (pick < 0 || pool[k].cap < pool[pick].cap)) pick = k;
+        if (largest < 0 || pool[k].cap > pool[largest].cap) largest = k;
+      }
+      if (pick < 0) pick = largest;
+      if (pick >= 0) {
+        out = std::move(pool[pick]);
+        pool.erase(pool.begin() + pick);
+      }
+    }
+    if (out.cap >= bytes) return CRISPY_OK;
+    out.release();
+    void *p = nullptr;
+    const int rc = crispy_host_alloc((int64_t)bytes, &p);
+    if (rc != CRISPY_OK) return rc;
+    out.p = p;
+    out.cap = bytes;
+    return CRISPY_OK;
+  }
+  void put_block(PinnedBuf &&b) {
+    if (!b.p) return;
+    std::lock_guard<std::mutex> lk(pool_mu);
+    pool.push_back(std::move(b));
+  }
 };
 
 namespace {
 
-void run_job(crispy_ctx *ctx, Job &job) {
+void run_job(crispy_server *srv, crispy_ctx *ctx, Job &job) {
   int rc = crispy_reads_upload(ctx, CRISPY_READS_KMER, job.kbases.data(), job.koff.data(), job.n);
   if (rc == CRISPY_OK) {
     rc = job.same_reads
@@ -85,10 +149,11 @@ void run_job(crispy_ctx *ctx, Job &job) {
   if (rc == CRISPY_OK) rc = crispy_kmer_run(ctx, job.K, job.kthr, 0, -1, &job.kmer_pairs);
   int64_t c = 0;
   if (rc == CRISPY_OK) rc = crispy_nw_run_on_kmer(ctx, job.band, job.nthr, &c);
+  if (rc == CRISPY_OK) rc = srv->get_block((size_t)c * 12, job.res);
   if (rc == CRISPY_OK) {
-    job.pairs.resize(2 * c);
-    job.dists.resize(c);
-    rc = crispy_nw_fetch(ctx, 0, c, job.pairs.data(), job.dists.data());
+    job.count = c;
+    rc = crispy_nw_fetch(ctx, 0, c, static_cast<int32_t *>(job.res.p),
+                         reinterpret_cast<float *>(static_cast<char *>(job.res.p) + 8 * c));
   }
   if (rc == CRISPY_OK) {
     job.row_ptr.resize((size_t)job.n + 1);
@@ -137,7 +202,7 @@ void worker_loop(crispy_server *srv, int slot) {
         srv->cv_work.wait(lk);
       }
     }
-    run_job(ctx, *job);
+    run_job(srv, ctx, *job);
     {
       std::lock_guard<std::mutex> lk(srv->mu);
       const int64_t held = crispy_ctx_device_bytes(ctx);
@@ -152,8 +217,8 @@ void worker_loop(crispy_server *srv, int slot) {
         job->est = srv->budget;
         job->status = CRISPY_OK;
         job->error.clear();
-        job->pairs.clear();
-        job->dists.clear();
+        srv->put_block(std::move(job->res));
+        job->count = 0;
         job->row_ptr.clear();
         srv->queue.push_front(job);
         job.reset();
@@ -301,7 +366,7 @@ int crispy_server_wait(crispy_server *srv, int64_t job_id, int64_t *kmer_pairs, 
   srv->cv_done.wait(lk, [&] { return job->done; });
   if (job->consumed) return CRISPY_ERR_STATE;  // fetched meanwhile by another thread
   if (kmer_pairs) *kmer_pairs = job->kmer_pairs;
-  if (nw_pairs) *nw_pairs = (int64_t)job->dists.size();
+  if (nw_pairs) *nw_pairs = job->count;
   using ms = std::chrono::duration<double, std::milli>;
   if (queue_ms) *queue_ms = ms(job->t_start - job->t_submit).count();
   if (run_ms) *run_ms = ms(job->t_end - job->t_start).count();
@@ -345,12 +410,13 @@ int crispy_server_fetch(crispy_server *srv, int64_t job_id, int32_t *pairs, floa
   srv->cv_done.wait(lk, [&] { return job->done; });
   if (job->consumed) return CRISPY_ERR_STATE;  // fetched meanwhile by another thread
   const int status = job->status;
-  if (status == CRISPY_OK) {
-    if (pairs && !job->pairs.empty()) std::memcpy(pairs, job->pairs.data(), job->pairs.size() * sizeof(int32_t));
-    if (dists && !job->dists.empty()) std::memcpy(dists, job->dists.data(), job->dists.size() * sizeof(float));
+  if (status == CRISPY_OK && job->count > 0) {
+    if (pairs) std::memcpy(pairs, job->pairs(), (size_t)job->count * 2 * sizeof(int32_t));
+    if (dists) std::memcpy(dists, job->dists(), (size_t)job->count * sizeof(float));
   }
   if (!job->error.empty()) srv->errors[job_id] = job->error;
   job->consumed = true;
+  srv->put_block(std::move(job->res));
   srv->jobs.erase(job_id);
   srv->cv_done.notify_all();
   return status;
@@ -370,13 +436,14 @@ int crispy_server_fetch_csr(crispy_server *srv, int64_t job_id, int32_t n_rows, 
   const int status = job->status;
   if (status == CRISPY_OK) {
     std::memcpy(row_ptr, job->row_ptr.data(), job->row_ptr.size() * sizeof(int64_t));
-    const size_t c = job->dists.size();
+    const size_t c = (size_t)job->count;
     if (cols)
-      for (size_t k = 0; k < c; ++k) cols[k] = job->pairs[2 * k + 1];
-    if (vals && c) std::memcpy(vals, job->dists.data(), c * sizeof(float));
+      for (size_t k = 0; k < c; ++k) cols[k] = job->pairs()[2 * k + 1];
+    if (vals && c) std::memcpy(vals, job->dists(), c * sizeof(float));
   }
   if (!job->error.empty()) srv->errors[job_id] = job->error;
   job->consumed = true;
+  srv->put_block(std::move(job->res));
   srv->jobs.erase(job_id);
   srv->cv_done.notify_all();
   return status;
