@@ -49,8 +49,12 @@ struct PinnedBuf {
 
 struct Job {
   int64_t id = 0;
-  std::string kbases, nbases;
+  // the reads (bases in page-locked blocks from the server's pool, so the
+  // uploads run at the bus's rate), until the job has run
+  PinnedBuf kbases, nbases;
   std::vector<int64_t> koff, noff;
+  const char *kb() const { return kbases.p ? static_cast<const char *>(kbases.p) : ""; }
+  const char *nb() const { return nbases.p ? static_cast<const char *>(nbases.p) : ""; }
   bool same_reads = true;
   int32_t n = 0;
   int K = 6, band = 10;
@@ -140,11 +144,11 @@ struct crispy_server {
 namespace {
 
 void run_job(crispy_server *srv, crispy_ctx *ctx, Job &job) {
-  int rc = crispy_reads_upload(ctx, CRISPY_READS_KMER, job.kbases.data(), job.koff.data(), job.n);
+  int rc = crispy_reads_upload(ctx, CRISPY_READS_KMER, job.kb(), job.koff.data(), job.n);
   if (rc == CRISPY_OK) {
     rc = job.same_reads
-             ? crispy_reads_upload(ctx, CRISPY_READS_NW, job.kbases.data(), job.koff.data(), job.n)
-             : crispy_reads_upload(ctx, CRISPY_READS_NW, job.nbases.data(), job.noff.data(), job.n);
+             ? crispy_reads_upload(ctx, CRISPY_READS_NW, job.kb(), job.koff.data(), job.n)
+             : crispy_reads_upload(ctx, CRISPY_READS_NW, job.nb(), job.noff.data(), job.n);
   }
   if (rc == CRISPY_OK) rc = crispy_kmer_run(ctx, job.K, job.kthr, 0, -1, &job.kmer_pairs);
   int64_t c = 0;
@@ -203,6 +207,7 @@ void worker_loop(crispy_server *srv, int slot) {
       }
     }
     run_job(srv, ctx, *job);
+    (void)crispy_ctx_sync(ctx);  // no copy from the job's host blocks is in flight when they go back
     {
       std::lock_guard<std::mutex> lk(srv->mu);
       const int64_t held = crispy_ctx_device_bytes(ctx);
@@ -227,8 +232,8 @@ void worker_loop(crispy_server *srv, int slot) {
         job->done = true;
         job->held = held;
         // the input copies are no longer needed
-        std::string().swap(job->kbases);
-        std::string().swap(job->nbases);
+        srv->put_block(std::move(job->kbases));
+        srv->put_block(std::move(job->nbases));
       }
     }
     srv->cv_done.notify_all();
@@ -339,13 +344,17 @@ int64_t crispy_server_submit(crispy_server *srv, const char *kmer_bases, const i
     if (kmer_offsets[0] != 0 || (!job->same_reads && (!nw_offsets || nw_offsets[0] != 0)))
       return CRISPY_ERR_ARG;
     job->koff.assign(kmer_offsets, kmer_offsets + n + 1);
-    job->kbases.assign(kmer_bases, (size_t)job->koff[n]);
+    const size_t kn = (size_t)job->koff[n];
+    if (srv->get_block(kn, job->kbases) != CRISPY_OK) return CRISPY_ERR_NOMEM;
+    if (kn) std::memcpy(job->kbases.p, kmer_bases, kn);
     if (!job->same_reads) {
       job->noff.assign(nw_offsets, nw_offsets + n + 1);
-      job->nbases.assign(nw_bases, (size_t)job->noff[n]);
+      const size_t nn = (size_t)job->noff[n];
+      if (srv->get_block(nn, job->nbases) != CRISPY_OK) return CRISPY_ERR_NOMEM;
+      if (nn) std::memcpy(job->nbases.p, nw_bases, nn);
     }
   }
-  job->est = crispy_server_job_bytes(job->kbases.data(), job->koff.data(), n, K, band);
+  job->est = crispy_server_job_bytes(job->kb(), job->koff.data(), n, K, band);
   std::lock_guard<std::mutex> lk(srv->mu);
   if (job->est > srv->budget) return CRISPY_ERR_NOMEM;  // would never be admitted
   job->id = srv->next_id++;
