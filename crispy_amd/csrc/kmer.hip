@@ -986,27 +986,38 @@ __global__ __launch_bounds__(256) void kmer_cand_kernel(
 }
 
 // Refines the candidates that the hashed k = 6 filter let through (their
-// keys carry the bucket popcount): half a wave per candidate, lane l of the
-// half ANDs words [4l, 4l + 4) of the two reads' 4,096-bit presence bitmaps
-// for the exact presence popcount p, and the tile's exact-bitmap bound
-// p + min(excess_i, excess_j) >= ((T_min - tshift) * cut - 1) keeps the
-// candidates that might pass; those are queued per wave in LDS and appended
-// to `out` with p in the popcount field (one atomic on *nout per up to 256 of
-// them: a same-address atomic per wave step serialises across the chip).
+// keys carry the bucket popcount): LPC lanes per candidate (32 or 16), lane l
+// of a group ANDs words [128/LPC * l, 128/LPC * (l + 1)) of the two reads'
+// 4,096-bit presence bitmaps for the exact presence popcount p, and the
+// tile's exact-bitmap bound p + min(excess_i, excess_j) >=
+// ((T_min - tshift) * cut - 1) keeps the candidates that might pass; those
+// are queued per wave in LDS and appended to `out` with p in the popcount
+// field (one atomic on *nout per up to 256 of them: a same-address atomic per
+// wave step serialises across the chip).  A wave has 4 candidates' loads in
+// flight per step (two per half wave, or one per quarter wave), with one
+// memory round trip per step: the next keys are loaded a step ahead, and T
+// and excess are requested with the bitmaps.  Half waves read each row as
+// one 512-byte run per instruction (faster where candidates are sparse:
+// reads250k 1.42 against 1.54 ms), quarter waves have less to reduce (faster
+// where they are dense: reads100k 0.71 against 0.90 ms).
 // Input: the tile pass's kCandSegs segments (seg_cap keys each, counts in
 // `seg`); block b reads segment b mod kCandSegs, so the grid is a multiple of
 // kCandSegs.
 constexpr int kRefineQueue = 256;
+template <int LPC>
 __global__ __launch_bounds__(256) void kmer_cand_refine_kernel(
     const uint64_t *__restrict__ cand, const unsigned long long *__restrict__ seg, int64_t seg_cap,
     const uint32_t *__restrict__ bitmap, const int32_t *__restrict__ Tn, const int32_t *__restrict__ excess,
     float cut, int tshift, int kshift, unsigned long long *__restrict__ nout, uint64_t *__restrict__ out) {
   constexpr int WP = kmer_words(6);
-  static_assert(WP == 4 * 32, "32 lanes of 4 words cover a k = 6 bitmap");
+  static_assert(LPC == 32 || LPC == 16, "half or quarter waves");
+  constexpr int GW = 64 / LPC;   // candidate groups per wave
+  constexpr int U = 4 / GW;      // candidates per group and step
+  constexpr int WPL = WP / LPC;  // bitmap words per lane
+  constexpr int NV = WPL / 4;    // 16-byte loads per lane and row
+  static_assert(WPL % 4 == 0, "whole 16-byte loads");
   __shared__ uint64_t sq[4][kRefineQueue];
-  // half a wave per candidate, two candidates per half wave and step: the
-  // loads of 4 candidates in flight per wave, each a 512-byte run per row
-  const int lane = threadIdx.x & 63, h = lane >> 5, l = lane & 31, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, h = lane / LPC, l = lane % LPC, wv = threadIdx.x >> 6;
   uint64_t *q = sq[wv];
   int qn = 0;  // wave-uniform
   auto flush = [&]() {
@@ -1023,31 +1034,35 @@ __global__ __launch_bounds__(256) void kmer_cand_refine_kernel(
   cand += (size_t)sg * seg_cap;
   const int64_t waves = (int64_t)(gridDim.x / kCandSegs) * (blockDim.x >> 6);
   const int64_t w0 = (int64_t)(blockIdx.x / kCandSegs) * (blockDim.x >> 6) + wv;
-  // one memory round trip per step: the next keys are loaded a step ahead,
-  // and T and excess are requested with the bitmaps
-  uint64_t kn[2];
+  uint64_t kn[U];
 #pragma unroll
-  for (int u = 0; u < 2; ++u) kn[u] = 4 * w0 + 2 * u + h < ncand ? cand[4 * w0 + 2 * u + h] : 0;
+  for (int u = 0; u < U; ++u) kn[u] = 4 * w0 + GW * u + h < ncand ? cand[4 * w0 + GW * u + h] : 0;
   for (int64_t cb = 4 * w0; cb < ncand; cb += 4 * waves) {  // wave-uniform
-    uint64_t key[2];
-    bool valid[2];
-    int p[2], Ti[2], Tj[2], Ei[2], Ej[2];
+    uint64_t key[U];
+    bool valid[U];
+    int p[U], Ti[U], Tj[U], Ei[U], Ej[U];
+    uint4 a[U][NV], b[U][NV];
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      const int64_t c = cb + 2 * u + h;
+    for (int u = 0; u < U; ++u) {  // all next keys requested before any bitmap row
+      const int64_t c = cb + GW * u + h;
       valid[u] = c < ncand;
       key[u] = valid[u] ? kn[u] & kCandKeyMask : 0;
       kn[u] = c + 4 * waves < ncand ? cand[c + 4 * waves] : 0;
-      p[u] = Ti[u] = Tj[u] = Ei[u] = Ej[u] = 0;
+      Ti[u] = Tj[u] = Ei[u] = Ej[u] = 0;
     }
-    uint4 a[2], b[2];
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
+    for (int u = 0; u < U; ++u) {
       const uint32_t i = (uint32_t)(key[u] >> kshift), j = (uint32_t)(key[u] & ((1ull << kshift) - 1));
-      a[u] = b[u] = make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+      for (int v = 0; v < NV; ++v) a[u][v] = b[u][v] = make_uint4(0u, 0u, 0u, 0u);
       if (valid[u]) {
-        a[u] = *reinterpret_cast<const uint4 *>(bitmap + (size_t)i * WP + 4 * l);
-        b[u] = *reinterpret_cast<const uint4 *>(bitmap + (size_t)j * WP + 4 * l);
+        const uint4 *ra = reinterpret_cast<const uint4 *>(bitmap + (size_t)i * WP + WPL * l);
+        const uint4 *rb = reinterpret_cast<const uint4 *>(bitmap + (size_t)j * WP + WPL * l);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          a[u][v] = ra[v];
+          b[u][v] = rb[v];
+        }
         if (l == 0) {
           Ti[u] = Tn[i];
           Tj[u] = Tn[j];
@@ -1057,11 +1072,14 @@ __global__ __launch_bounds__(256) void kmer_cand_refine_kernel(
       }
     }
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      p[u] = __popc(a[u].x & b[u].x) + __popc(a[u].y & b[u].y) + __popc(a[u].z & b[u].z) +
-             __popc(a[u].w & b[u].w);
+    for (int u = 0; u < U; ++u) {
+      p[u] = 0;
 #pragma unroll
-      for (int off = 16; off > 0; off >>= 1) p[u] += __shfl_xor(p[u], off, 32);
+      for (int v = 0; v < NV; ++v)
+        p[u] += __popc(a[u][v].x & b[u][v].x) + __popc(a[u][v].y & b[u][v].y) + __popc(a[u][v].z & b[u][v].z) +
+                __popc(a[u][v].w & b[u][v].w);
+#pragma unroll
+      for (int off = LPC / 2; off > 0; off >>= 1) p[u] += __shfl_xor(p[u], off, LPC);
       bool keep = false;
       if (valid[u] && l == 0) {
         const float F = fmaf((float)(min(Ti[u], Tj[u]) - tshift), cut, -1.0f);
@@ -1429,7 +1447,7 @@ static int run_segmented(crispy_ctx *ctx, int64_t *count, F &&launch) {
 
 // The hashed filter's candidates (the tile pass's segments) refined by the
 // exact k = 6 bitmaps into km.cand_alt, their count left in km.cand_n.
-static int refine_candidates(crispy_ctx *ctx, int64_t ncand, float threshold, int tshift) {
+static int refine_candidates(crispy_ctx *ctx, int64_t ncand, double range_pairs, float threshold, int tshift) {
   KmerState &km = ctx->km;
   CRISPY_TRY(ensure(ctx, km.cand_alt, sizeof(uint64_t) * std::max<int64_t>(ncand, 1)));
   CRISPY_TRY(ensure(ctx, km.cand_n, sizeof(unsigned long long)));
@@ -1437,10 +1455,12 @@ static int refine_candidates(crispy_ctx *ctx, int64_t ncand, float threshold, in
   timer_begin(ctx, "kmer_cand_pop", &ev);
   zero_counter_kernel<<<1, 1, 0, ctx->stream>>>(km.cand_n.as<unsigned long long>());
   CRISPY_LAUNCHED(ctx);
-  // 16 candidates per block step; at most 8 blocks per segment
+  // 16 candidates per block step; at most 8 blocks per segment.  Quarter
+  // waves from 1 candidate per 1,000 pairs of the range up (about 16 per tile)
   const int64_t per_seg = (ncand + kCandSegs - 1) / kCandSegs;
   const int bps = (int)std::max<int64_t>(1, std::min<int64_t>(8, (per_seg + 63) / 64));
-  kmer_cand_refine_kernel<<<kCandSegs * bps, 256, 0, ctx->stream>>>(
+  const auto kernel = (double)ncand >= 1e-3 * range_pairs ? kmer_cand_refine_kernel<16> : kmer_cand_refine_kernel<32>;
+  kernel<<<kCandSegs * bps, 256, 0, ctx->stream>>>(
       km.cand.as<uint64_t>(), km.seg.as<unsigned long long>(), km.cand_seg_cap, km.bitmap.as<uint32_t>(),
       km.T.as<int32_t>(), km.excess.as<int32_t>(), 1.0f - threshold - 2e-5f, tshift, km.kshift,
       km.cand_n.as<unsigned long long>(), km.cand_alt.as<uint64_t>());
@@ -1620,7 +1640,7 @@ int crispy_kmer_run(crispy_ctx *ctx, int K, float threshold, int64_t tile_begin,
     int nseg = kCandSegs;
     int64_t seg_cap = km.cand_seg_cap;
     if (km.hashed && ncand > 0) {
-      CRISPY_TRY(refine_candidates(ctx, ncand, threshold, 0));
+      CRISPY_TRY(refine_candidates(ctx, ncand, (double)pairs_in_range, threshold, 0));
       exact_in = km.cand_alt.as<uint64_t>();
       exact_n = km.cand_n.as<unsigned long long>();
       nseg = 1;
@@ -1654,7 +1674,7 @@ int crispy_kmer_run(crispy_ctx *ctx, int K, float threshold, int64_t tile_begin,
     // (on exact-bitmap candidates it keeps them all: then it only compacts)
     int64_t nref = 0;
     if (ncand > 0) {
-      CRISPY_TRY(refine_candidates(ctx, ncand, threshold, K - 6));
+      CRISPY_TRY(refine_candidates(ctx, ncand, (double)pairs_in_range, threshold, K - 6));
       unsigned long long c = 0;
       CRISPY_HIP(ctx, hipMemcpyAsync(&c, km.cand_n.p, sizeof c, hipMemcpyDeviceToHost, ctx->stream));
       CRISPY_HIP(ctx, hipStreamSynchronize(ctx->stream));
