@@ -55,6 +55,15 @@ __host__ __device__ inline uint32_t kmer_bucket(uint32_t code) {
   return (uint32_t)(((uint64_t)(code * 2654435761u) * (uint64_t)(32 * HW)) >> 32);
 }
 
+// Bit i of the low 16 bits of x moved to bit 2i (the other bits zero).
+__device__ __forceinline__ uint32_t spread_bits16(uint32_t x) {
+  x &= 0xFFFFu;
+  x = (x | (x << 8)) & 0x00FF00FFu;
+  x = (x | (x << 4)) & 0x0F0F0F0Fu;
+  x = (x | (x << 2)) & 0x33333333u;
+  return (x | (x << 1)) & 0x55555555u;
+}
+
 // One block of 128 threads per read: the read's packed words are staged in
 // LDS, the codes of its T = L-K+1 k-mers (first base most significant, as
 // READ::formTuples kmerMain.cpp:117-154) counted in an LDS histogram of 16-bit
@@ -84,6 +93,7 @@ __global__ __launch_bounds__(128) void kmer_profile_kernel(
   __shared__ uint32_t bm[WP];
   __shared__ uint32_t hbm[HASH ? HW : 1];
   __shared__ int wsum[2];
+  __shared__ int wrep[NHW >= NT ? NHW / NT : 1][2];  // repeat entries per (step, wave)
   __shared__ int s_distinct, s_hdistinct;
   const int r = blockIdx.x;
   const int tid = threadIdx.x;
@@ -94,7 +104,8 @@ __global__ __launch_bounds__(128) void kmer_profile_kernel(
   for (int w = tid; w < nwr; w += NT) sread[w] = s[w];
   if (tid == 0) sread[nwr] = 0;  // the last window's high word
   for (int v = tid; v < NHW; v += NT) hist[v] = 0;
-  for (int w = tid; w < WP; w += NT) bm[w] = 0;
+  if (NHW < NT || NBINS / 32 < WP)  // words the transposed scan does not store (K = 4 pads to 16)
+    for (int w = tid; w < WP; w += NT) bm[w] = 0;
   if (HASH)
     for (int w = tid; w < HW; w += NT) hbm[w] = 0;
   if (tid == 0) s_distinct = s_hdistinct = 0;
@@ -108,50 +119,106 @@ __global__ __launch_bounds__(128) void kmer_profile_kernel(
     atomicAdd(&hist[c >> 1], 1u << (16 * (c & 1)));
   }
   __syncthreads();
-  // thread tid owns bins [tid * BPT, tid * BPT + BPT)
-  const int b0 = tid * BPT;
-  auto bin = [&](int v) -> uint32_t { return (hist[v >> 1] >> (16 * (v & 1))) & 0xFFFFu; };
-  uint32_t bits = 0;
-  int nrep = 0, ndist = 0;
-  if (b0 < NBINS) {
-#pragma unroll
-    for (int k = 0; k < BPT; ++k) {
-      const uint32_t c = bin(b0 + k);
-      bits |= (c > 0 ? 1u : 0u) << k;
-      ndist += c > 0;
-      nrep += c >= 2;
-      if (HASH && c > 0) {
-        const uint32_t h = kmer_bucket<HASH ? HW : kHashWordsMax>((uint32_t)(b0 + k));
-        atomicOr(&hbm[h >> 5], 1u << (h & 31));
+  const int lane = tid & 63, wv = tid >> 6;
+  if constexpr (NHW >= NT) {
+    // Transposed scan: at step k thread tid reads histogram word v = k * NT + tid
+    // (bins 2v and 2v + 1), so a wave's reads hit distinct banks.  Ballots over
+    // the even and odd bins give the wave's 128 presence bits: bitmap word
+    // 8k + 4wv + g holds lane 16g + i's even bin at bit 2i and its odd bin at
+    // bit 2i + 1, stored by lane g with no atomics.
+    constexpr int NSTEP = NHW / NT;
+    static_assert(NT == 128 && NSTEP * 8 <= WP, "two waves of 64 lanes, 2 bins each, per step");
+    int ndist = 0;  // wave-uniform
+    for (int k = 0; k < NSTEP; ++k) {
+      const int v = k * NT + tid;
+      const uint32_t hw = hist[v], c0 = hw & 0xFFFFu, c1 = hw >> 16;
+      const uint64_t e = __ballot(c0 > 0), o = __ballot(c1 > 0);
+      const uint64_t r0 = __ballot(c0 >= 2), r1 = __ballot(c1 >= 2);
+      ndist += __popcll(e) + __popcll(o);
+      if (lane < 4)
+        bm[8 * k + 4 * wv + lane] =
+            spread_bits16((uint32_t)(e >> (16 * lane))) | (spread_bits16((uint32_t)(o >> (16 * lane))) << 1);
+      if (lane == 0) wrep[k][wv] = __popcll(r0) + __popcll(r1);
+      if (HASH) {
+        if (c0) {
+          const uint32_t h = kmer_bucket<HASH ? HW : kHashWordsMax>((uint32_t)(2 * v));
+          atomicOr(&hbm[h >> 5], 1u << (h & 31));
+        }
+        if (c1) {
+          const uint32_t h = kmer_bucket<HASH ? HW : kHashWordsMax>((uint32_t)(2 * v + 1));
+          atomicOr(&hbm[h >> 5], 1u << (h & 31));
+        }
       }
     }
-    if (bits) atomicOr(&bm[b0 >> 5], bits << (b0 & 31));
-  }
-  if (ndist) atomicAdd(&s_distinct, ndist);
-  // where this thread's repeat entries go: an inclusive scan of nrep within
-  // the wave, then the total of the wave before it
-  const int lane = tid & 63, wv = tid >> 6;
-  int incl = nrep;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const int v = __shfl_up(incl, off, 64);
-    if (lane >= off) incl += v;
-  }
-  if (lane == 63) wsum[wv] = incl;
-  __syncthreads();
-  const int before = wv > 0 ? wsum[0] : 0;
-  if (nrep) {
-    uint32_t *out = rlist + roff[r] + (before + incl - nrep);
-#pragma unroll
-    for (int k = 0; k < BPT; ++k) {
-      const uint32_t c = bin(b0 + k);
-      if (c >= 2) *out++ = ((uint32_t)(b0 + k) << 16) | (c - 1);
+    if (lane == 0) wsum[wv] = ndist;
+    __syncthreads();
+    // The repeat entries in code order: step k's after every earlier step's,
+    // wave 0's before wave 1's, and within a wave by lane, even bin first.
+    uint32_t *out = rlist + roff[r];
+    int run = 0;  // entries of the steps before k
+    for (int k = 0; k < NSTEP; ++k) {
+      const int n0 = wrep[k][0], n1 = wrep[k][1];
+      if (wv ? n1 : n0) {  // wave-uniform
+        const int v = k * NT + tid;
+        const uint32_t hw = hist[v], c0 = hw & 0xFFFFu, c1 = hw >> 16;
+        const bool q0 = c0 >= 2, q1 = c1 >= 2;
+        const uint64_t below = (1ull << lane) - 1;
+        int pos = run + (wv ? n0 : 0) + __popcll(__ballot(q0) & below) + __popcll(__ballot(q1) & below);
+        if (q0) out[pos++] = ((uint32_t)(2 * v) << 16) | (c0 - 1);
+        if (q1) out[pos] = ((uint32_t)(2 * v + 1) << 16) | (c1 - 1);
+      }
+      run += n0 + n1;
     }
-  }
-  if (tid == NT - 1) rcnt[r] = before + incl;  // the last thread's inclusive sum: every entry
-  if (tid == 0) {
-    Tout[r] = T;
-    excess[r] = T - s_distinct;  // sum over codes of (count - 1)
+    if (tid == 0) {
+      rcnt[r] = run;
+      Tout[r] = T;
+      excess[r] = T - wsum[0] - wsum[1];  // sum over codes of (count - 1)
+    }
+  } else {
+    // thread tid owns bins [tid * BPT, tid * BPT + BPT)
+    const int b0 = tid * BPT;
+    auto bin = [&](int v) -> uint32_t { return (hist[v >> 1] >> (16 * (v & 1))) & 0xFFFFu; };
+    uint32_t bits = 0;
+    int nrep = 0, ndist = 0;
+    if (b0 < NBINS) {
+#pragma unroll
+      for (int k = 0; k < BPT; ++k) {
+        const uint32_t c = bin(b0 + k);
+        bits |= (c > 0 ? 1u : 0u) << k;
+        ndist += c > 0;
+        nrep += c >= 2;
+        if (HASH && c > 0) {
+          const uint32_t h = kmer_bucket<HASH ? HW : kHashWordsMax>((uint32_t)(b0 + k));
+          atomicOr(&hbm[h >> 5], 1u << (h & 31));
+        }
+      }
+      if (bits) atomicOr(&bm[b0 >> 5], bits << (b0 & 31));
+    }
+    if (ndist) atomicAdd(&s_distinct, ndist);
+    // where this thread's repeat entries go: an inclusive scan of nrep within
+    // the wave, then the total of the wave before it
+    int incl = nrep;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int v = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += v;
+    }
+    if (lane == 63) wsum[wv] = incl;
+    __syncthreads();
+    const int before = wv > 0 ? wsum[0] : 0;
+    if (nrep) {
+      uint32_t *out = rlist + roff[r] + (before + incl - nrep);
+#pragma unroll
+      for (int k = 0; k < BPT; ++k) {
+        const uint32_t c = bin(b0 + k);
+        if (c >= 2) *out++ = ((uint32_t)(b0 + k) << 16) | (c - 1);
+      }
+    }
+    if (tid == NT - 1) rcnt[r] = before + incl;  // the last thread's inclusive sum: every entry
+    if (tid == 0) {
+      Tout[r] = T;
+      excess[r] = T - s_distinct;  // sum over codes of (count - 1)
+    }
   }
   uint32_t *dst = bitmap + (int64_t)r * WP;
   for (int w = tid; w < WP; w += NT) dst[w] = bm[w];
