@@ -249,6 +249,41 @@ __device__ __forceinline__ uint32_t repeat_min_sum(P A, int na, P B, int nb) {
   return sum;
 }
 
+// The same sum over two lists in global memory, read W entries of each per
+// memory round trip.  Every entry of a window whose code is at most the
+// smaller of the two windows' last codes is consumed (so one window is used up
+// whole each time), and a common code always lies in both windows at once
+// (codes are unique within a list), so comparing all pairs of the two windows
+// finds each exactly once.  Past a list's end a window holds code 0xFFFF (A)
+// or 0xFFFE (B): these match nothing.
+template <int W>
+__device__ __forceinline__ uint32_t repeat_min_sum_windowed(const uint32_t *__restrict__ A, int na,
+                                                            const uint32_t *__restrict__ B, int nb) {
+  uint32_t sum = 0;
+  int x = 0, y = 0;
+  while (x < na && y < nb) {
+    uint32_t a[W], b[W];
+#pragma unroll
+    for (int s = 0; s < W; ++s) {
+      a[s] = x + s < na ? A[x + s] : 0xFFFFFFFFu;
+      b[s] = y + s < nb ? B[y + s] : 0xFFFEFFFFu;
+    }
+    const uint32_t lim = min(a[W - 1], b[W - 1]) >> 16;
+    int da = 0, db = 0;
+#pragma unroll
+    for (int s = 0; s < W; ++s) {
+      da += (a[s] >> 16) <= lim;
+      db += (b[s] >> 16) <= lim;
+#pragma unroll
+      for (int t = 0; t < W; ++t)
+        if ((a[s] ^ b[t]) < 0x10000u) sum += min(a[s] & 0xFFFFu, b[t] & 0xFFFFu);
+    }
+    x += da;
+    y += db;
+  }
+  return sum;
+}
+
 // Candidate entries of the tile pass: the pair key i << kshift | j (at most
 // 40 bits) and, above it, the pair's presence popcount (at most 4096).
 constexpr int kCandPopShift = 44;
@@ -1167,20 +1202,42 @@ __global__ __launch_bounds__(256) void kmer_cand_refine_kernel(
 // reference distance and predicate, the passing pairs appended.  Input:
 // `nseg` segments of seg_cap keys whose counts are ncnt[s * kSegStride] (the
 // tile pass's segments, or one run with its count at ncnt[0]); block b reads
-// segment b mod nseg, so the grid is a multiple of nseg.
+// segment b mod nseg, so the grid is a multiple of nseg.  The repeat lists
+// are merged a window of 8 entries per round trip, and the passing pairs are
+// queued per wave in LDS and appended with one atomic on *counter per up to
+// kExactQueue of them.
+constexpr int kExactQueue = 256;
 __global__ __launch_bounds__(256) void kmer_exact_kernel(
     const uint64_t *__restrict__ cand, const unsigned long long *__restrict__ ncnt, int nseg, int64_t seg_cap,
     const int32_t *__restrict__ Tn,
     const int32_t *__restrict__ rcnt, const int64_t *__restrict__ roffc, const uint32_t *__restrict__ rlistc,
     float thr, int kshift, unsigned long long *__restrict__ counter, uint64_t *__restrict__ out_keys,
     float *__restrict__ out_dists, int64_t cap) {
+  __shared__ uint64_t sk[4][kExactQueue];
+  __shared__ float sd[4][kExactQueue];
   const int sg = (int)(blockIdx.x % nseg);
   const int64_t ncand = min((int64_t)ncnt[(size_t)sg * kSegStride], seg_cap);
   cand += (size_t)sg * seg_cap;
-  const int lane = threadIdx.x & 63;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int qn = 0;  // wave-uniform
+  auto flush = [&]() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    unsigned long long basep = 0;
+    if (lane == 0) basep = atomicAdd(counter, (unsigned long long)qn);
+    basep = __shfl(basep, 0, 64);
+    for (int e = lane; e < qn; e += 64) {
+      const int64_t pos = (int64_t)basep + e;
+      if (pos < cap) {
+        out_keys[pos] = sk[wv][e];
+        out_dists[pos] = sd[wv][e];
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    qn = 0;
+  };
   const int64_t stride = (int64_t)(gridDim.x / nseg) * blockDim.x;
   const int64_t base0 = (int64_t)(blockIdx.x / nseg) * blockDim.x + (threadIdx.x & ~63);
-  for (int64_t b = base0; b < ncand; b += stride) {
+  for (int64_t b = base0; b < ncand; b += stride) {  // wave-uniform
     const int64_t c = b + lane;
     bool pass = false;
     float dist = 0.f;
@@ -1191,25 +1248,20 @@ __global__ __launch_bounds__(256) void kmer_exact_kernel(
       const int i = (int)(key >> kshift), j = (int)(key & ((1ull << kshift) - 1));
       uint32_t m = (uint32_t)(e >> kCandPopShift);
       const int na = rcnt[i], nb = rcnt[j];
-      if (na > 0 && nb > 0) m += repeat_min_sum(rlistc + roffc[i], na, rlistc + roffc[j], nb);
+      if (na > 0 && nb > 0) m += repeat_min_sum_windowed<8>(rlistc + roffc[i], na, rlistc + roffc[j], nb);
       dist = 1.0f - (float)m / (float)min(Tn[i], Tn[j]);
       pass = passes(dist, thr);
     }
     const uint64_t ball = __ballot(pass);
-    const int total = __popcll(ball);
-    unsigned long long basep = 0;
-    if (total > 0) {
-      if (lane == 0) basep = atomicAdd(counter, (unsigned long long)total);
-      basep = __shfl(basep, 0, 64);
-    }
     if (pass) {
-      const int64_t pos = (int64_t)basep + __popcll(ball & ((1ull << lane) - 1));
-      if (pos < cap) {
-        out_keys[pos] = key;
-        out_dists[pos] = dist;
-      }
+      const int q = qn + __popcll(ball & ((1ull << lane) - 1));
+      sk[wv][q] = key;
+      sd[wv][q] = dist;
     }
+    qn += __popcll(ball);
+    if (qn > kExactQueue - 64) flush();
   }
+  if (qn > 0) flush();
 }
 
 // The repeat lists back to back in read order: roffc = exclusive sum of the
