@@ -1013,8 +1013,10 @@ __global__ __launch_bounds__(256) void kmer_cand_kernel(
   // Each lane merges its candidate's two lists through a window of kCandWin
   // codes of each, staged in LDS; the lanes of a wave refill their windows in
   // the same round, so the wave waits for memory once per round (every ~13+
-  // codes) instead of whenever one of its lanes crosses a word.
-  __shared__ __align__(16) uint16_t win[256][2][kCandWin];
+  // codes) instead of whenever one of its lanes crosses a word.  The windows
+  // are stored lane-minor (two codes per word, word [list][pair][thread]), so
+  // the lanes of a wave read distinct banks whatever positions they are at.
+  __shared__ uint32_t win[2][kCandWin / 2][256];
   const int t = threadIdx.x, lane = t & 63;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   // every lane of a wave runs the same number of outer iterations (shuffles)
@@ -1052,12 +1054,16 @@ __global__ __launch_bounds__(256) void kmer_cand_kernel(
         }
 #pragma unroll
         for (int q = 0; q < kCandWin / 4; ++q) {
-          reinterpret_cast<uint64_t *>(win[t][0])[q] = wa[q];
-          reinterpret_cast<uint64_t *>(win[t][1])[q] = wb[q];
+          win[0][2 * q][t] = (uint32_t)wa[q];
+          win[0][2 * q + 1][t] = (uint32_t)(wa[q] >> 32);
+          win[1][2 * q][t] = (uint32_t)wb[q];
+          win[1][2 * q + 1][t] = (uint32_t)(wb[q] >> 32);
         }
         const int ea = min((int)l1, ka + kCandWin), eb = min((int)l2, lb + kCandWin);
         while (k < ea && l < eb) {
-          const uint32_t x = win[t][0][k - ka], y = win[t][1][l - lb];
+          const int u = k - ka, v = l - lb;
+          const uint32_t x = (win[0][u >> 1][t] >> (16 * (u & 1))) & 0xFFFFu;
+          const uint32_t y = (win[1][v >> 1][t] >> (16 * (v & 1))) & 0xFFFFu;
           matches = (unsigned short)(matches + (x == y));
           k += x <= y;
           l += x >= y;
