@@ -289,8 +289,9 @@ __device__ __forceinline__ uint32_t repeat_min_sum_windowed(const uint32_t *__re
 constexpr int kCandPopShift = 44;
 // Codes per list in a lane's merge window of the K = 7, 8 candidate kernel,
 // and the codes past the last K-mer list that a window may read.
-constexpr int kCandWin = 16;
+constexpr int kCandWin = 32;
 constexpr int kListSlack = 32;
+static_assert(kCandWin % 4 == 0 && kCandWin <= kListSlack, "a window starts on a word at or before the list's last code");
 // Pairs from which the K <= 6 exact counts run as a pass of their own (auto mode).
 constexpr int64_t kExactPassPairs = 1000000000;
 constexpr uint64_t kCandKeyMask = (1ull << 40) - 1;
@@ -1012,11 +1013,13 @@ __global__ __launch_bounds__(256) void kmer_cand_kernel(
     float *__restrict__ out_dists, int64_t cap) {
   // Each lane merges its candidate's two lists through a window of kCandWin
   // codes of each, staged in LDS; the lanes of a wave refill their windows in
-  // the same round, so the wave waits for memory once per round (every ~13+
-  // codes) instead of whenever one of its lanes crosses a word.  The windows
-  // are stored lane-minor (two codes per word, word [list][pair][thread]), so
-  // the lanes of a wave read distinct banks whatever positions they are at.
-  __shared__ uint32_t win[2][kCandWin / 2][256];
+  // the same round, so the wave waits for memory once per round (each
+  // round moves one list kCandWin - 3 codes or more) instead of whenever one
+  // of its lanes crosses a word.  The windows
+  // are stored lane-minor (code [list][position][thread]), so a merge step
+  // reads its two codes with no unpacking and the lanes of a wave meet in a
+  // bank at most in pairs whatever positions they are at.
+  __shared__ uint16_t win[2][kCandWin][256];
   const int t = threadIdx.x, lane = t & 63;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   // every lane of a wave runs the same number of outer iterations (shuffles)
@@ -1053,21 +1056,23 @@ __global__ __launch_bounds__(256) void kmer_cand_kernel(
           wb[q] = B[q];
         }
 #pragma unroll
-        for (int q = 0; q < kCandWin / 4; ++q) {
-          win[0][2 * q][t] = (uint32_t)wa[q];
-          win[0][2 * q + 1][t] = (uint32_t)(wa[q] >> 32);
-          win[1][2 * q][t] = (uint32_t)wb[q];
-          win[1][2 * q + 1][t] = (uint32_t)(wb[q] >> 32);
-        }
-        const int ea = min((int)l1, ka + kCandWin), eb = min((int)l2, lb + kCandWin);
-        while (k < ea && l < eb) {
-          const int u = k - ka, v = l - lb;
-          const uint32_t x = (win[0][u >> 1][t] >> (16 * (u & 1))) & 0xFFFFu;
-          const uint32_t y = (win[1][v >> 1][t] >> (16 * (v & 1))) & 0xFFFFu;
+        for (int q = 0; q < kCandWin / 4; ++q)
+#pragma unroll
+          for (int s = 0; s < 4; ++s) {
+            win[0][4 * q + s][t] = (uint16_t)(wa[q] >> (16 * s));
+            win[1][4 * q + s][t] = (uint16_t)(wb[q] >> (16 * s));
+          }
+        const uint16_t *wA = &win[0][0][t], *wB = &win[1][0][t];
+        int u = k - ka, v = l - lb;  // positions in the windows
+        const int ue = min((int)l1, ka + kCandWin) - ka, ve = min((int)l2, lb + kCandWin) - lb;
+        while (u < ue && v < ve) {
+          const uint32_t x = wA[u * 256], y = wB[v * 256];
           matches = (unsigned short)(matches + (x == y));
-          k += x <= y;
-          l += x >= y;
+          u += x <= y;
+          v += x >= y;
         }
+        k = ka + u;
+        l = lb + v;
         active = k < l1 && l < l2;
       }
     }
