@@ -55,15 +55,6 @@ __host__ __device__ inline uint32_t kmer_bucket(uint32_t code) {
   return (uint32_t)(((uint64_t)(code * 2654435761u) * (uint64_t)(32 * HW)) >> 32);
 }
 
-// Bit i of the low 16 bits of x moved to bit 2i (the other bits zero).
-__device__ __forceinline__ uint32_t spread_bits16(uint32_t x) {
-  x &= 0xFFFFu;
-  x = (x | (x << 8)) & 0x00FF00FFu;
-  x = (x | (x << 4)) & 0x0F0F0F0Fu;
-  x = (x | (x << 2)) & 0x33333333u;
-  return (x | (x << 1)) & 0x55555555u;
-}
-
 // One block of 128 threads per read: the read's packed words are staged in
 // LDS, the codes of its T = L-K+1 k-mers (first base most significant, as
 // READ::formTuples kmerMain.cpp:117-154) counted in an LDS histogram of 16-bit
@@ -124,8 +115,11 @@ __global__ __launch_bounds__(128) void kmer_profile_kernel(
     // Transposed scan: at step k thread tid reads histogram word v = k * NT + tid
     // (bins 2v and 2v + 1), so a wave's reads hit distinct banks.  Ballots over
     // the even and odd bins give the wave's 128 presence bits: bitmap word
-    // 8k + 4wv + g holds lane 16g + i's even bin at bit 2i and its odd bin at
-    // bit 2i + 1, stored by lane g with no atomics.
+    // w = 8k + 4wv + g holds lane 16g + i's even bin (code 32w + 2i) at bit i
+    // and its odd bin (code 32w + 2i + 1) at bit 16 + i, stored by lane g with
+    // no atomics.  (For K >= 4 the bits of a word are in this order, the same
+    // for every read; the passes that read the bitmaps take popcounts of two
+    // reads' words ANDed, which no fixed order within a word changes.)
     constexpr int NSTEP = NHW / NT;
     static_assert(NT == 128 && NSTEP * 8 <= WP, "two waves of 64 lanes, 2 bins each, per step");
     int ndist = 0;  // wave-uniform
@@ -135,9 +129,7 @@ __global__ __launch_bounds__(128) void kmer_profile_kernel(
       const uint64_t e = __ballot(c0 > 0), o = __ballot(c1 > 0);
       const uint64_t r0 = __ballot(c0 >= 2), r1 = __ballot(c1 >= 2);
       ndist += __popcll(e) + __popcll(o);
-      if (lane < 4)
-        bm[8 * k + 4 * wv + lane] =
-            spread_bits16((uint32_t)(e >> (16 * lane))) | (spread_bits16((uint32_t)(o >> (16 * lane))) << 1);
+      if (lane < 4) bm[8 * k + 4 * wv + lane] = ((uint32_t)(e >> (16 * lane)) & 0xFFFFu) | ((uint32_t)(o >> (16 * lane)) << 16);
       if (lane == 0) wrep[k][wv] = __popcll(r0) + __popcll(r1);
       if (HASH) {
         if (c0) {
